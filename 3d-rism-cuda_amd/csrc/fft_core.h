@@ -1,0 +1,243 @@
+// fft_core.h -- LDS-resident batched Stockham FFT building blocks for gfx950.
+//
+// A workgroup owns a set of whole lines (one axis of the 3D grid) in LDS.
+// Each radix stage reads every element once from LDS into registers, runs
+// radix-F butterflies (F in {2,3,4,5,8,16}), applies the Stockham twiddle and
+// writes the results back in autosorted order.  Stage parameters are
+// runtime values (one kernel serves every supported length); the butterfly
+// width and the per-thread butterfly count are compile-time so the data
+// stays in VGPRs.  Twiddles come from a per-axis table tw[k] = exp(-2 pi i
+// k / n) computed on the host in FP64.
+//
+// The 3D transform with the reference's half-shifted k grid
+// (src-cuda/fft3d.cu:22-94) is separable: along each axis
+//   F(j) = sum_x f(x) exp(-i k_j r_x),  r_x = (x - n/2) dr,
+//   k_j = 2 pi (j - n/2 + 1/2) / L
+//        = (-1)^(j + n/2) * DFT_n[ f(x) (-1)^x exp(-i pi (x - n/2) / n) ](j)
+// so every pass multiplies by its axis's pre-twiddle before and a sign
+// after (forward), or the sign before and the conjugate pre-twiddle after
+// (inverse).  Those multiplies are folded into the passes' load/store.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace rism {
+
+constexpr int kThreads = 256;
+constexpr int kMaxStages = 12;
+
+template <typename R> struct C2;
+template <> struct C2<double> { using T = double2; };
+template <> struct C2<float> { using T = float2; };
+
+template <typename T2> __device__ __forceinline__ T2 mk(decltype(T2::x) a, decltype(T2::x) b) {
+  T2 r; r.x = a; r.y = b; return r;
+}
+template <typename T2> __device__ __forceinline__ T2 cadd(T2 a, T2 b) { return mk<T2>(a.x + b.x, a.y + b.y); }
+template <typename T2> __device__ __forceinline__ T2 csub(T2 a, T2 b) { return mk<T2>(a.x - b.x, a.y - b.y); }
+template <typename T2> __device__ __forceinline__ T2 cmul(T2 a, T2 b) {
+  return mk<T2>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename T2> __device__ __forceinline__ T2 cmulc(T2 a, T2 b) {  // a * conj(b)
+  return mk<T2>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+template <typename T2> __device__ __forceinline__ T2 cscale(T2 a, decltype(T2::x) s) { return mk<T2>(a.x * s, a.y * s); }
+template <typename T2> __device__ __forceinline__ T2 cconj(T2 a) { return mk<T2>(a.x, -a.y); }
+// multiply by -i (forward) or +i (inverse)
+template <bool INV, typename T2> __device__ __forceinline__ T2 rot90(T2 a) {
+  return INV ? mk<T2>(-a.y, a.x) : mk<T2>(a.y, -a.x);
+}
+
+// Per-axis plan passed by value as a kernel argument.
+struct AxisPlan {
+  int n;
+  int nst;
+  int rad[kMaxStages];
+  int s[kMaxStages];      // Stockham stride before the stage
+  int m[kMaxStages];      // current length / radix
+  int tstep[kMaxStages];  // n / current length
+  int fixsign;            // (-1)^(n/2): 1 when n % 4 == 0
+  const void *tw;         // complex[n] exp(-2 pi i k/n)
+  const void *pre;        // complex[n] (-1)^x exp(-i pi (x - n/2)/n)
+};
+
+// ---------------------------------------------------------------------
+// small DFT kernels (in registers)
+// ---------------------------------------------------------------------
+template <int F, bool INV, typename T2> struct Dft;
+
+template <bool INV, typename T2> struct Dft<2, INV, T2> {
+  static __device__ __forceinline__ void run(T2 *v) {
+    T2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+
+template <bool INV, typename T2> struct Dft<4, INV, T2> {
+  static __device__ __forceinline__ void run(T2 *v) {
+    T2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    T2 t2 = cadd(v[1], v[3]), t3 = rot90<INV>(csub(v[1], v[3]));
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+  }
+};
+
+template <bool INV, typename T2> struct Dft<3, INV, T2> {
+  static __device__ __forceinline__ void run(T2 *v) {
+    using R = decltype(T2::x);
+    const R s3 = (R)0.86602540378443864676372317075294;  // sin(2 pi / 3)
+    T2 t = cadd(v[1], v[2]);
+    T2 mid = csub(v[0], cscale(t, (R)0.5));
+    // (v1 - v2) * (sign * i * sqrt(3)/2), sign = -1 forward
+    T2 d = cscale(rot90<INV>(csub(v[1], v[2])), s3);
+    v[0] = cadd(v[0], t);
+    v[1] = cadd(mid, d);
+    v[2] = csub(mid, d);
+  }
+};
+
+template <bool INV, typename T2> struct Dft<5, INV, T2> {
+  static __device__ __forceinline__ void run(T2 *v) {
+    using R = decltype(T2::x);
+    const R c1 = (R)0.30901699437494742410229341718282;   // cos(2pi/5)
+    const R c2 = (R)-0.80901699437494742410229341718282;  // cos(4pi/5)
+    const R s1 = (R)0.95105651629515357211643933337938;   // sin(2pi/5)
+    const R s2 = (R)0.58778525229247312916870595463907;   // sin(4pi/5)
+    T2 a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+    T2 a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+    T2 x0 = v[0];
+    T2 p1 = mk<T2>(x0.x + c1 * a1.x + c2 * a2.x, x0.y + c1 * a1.y + c2 * a2.y);
+    T2 p2 = mk<T2>(x0.x + c2 * a1.x + c1 * a2.x, x0.y + c2 * a1.y + c1 * a2.y);
+    // sign * i * (s1 b1 + s2 b2) and sign * i * (s2 b1 - s1 b2)
+    T2 q1 = rot90<INV>(mk<T2>(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y));
+    T2 q2 = rot90<INV>(mk<T2>(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y));
+    v[0] = mk<T2>(x0.x + a1.x + a2.x, x0.y + a1.y + a2.y);
+    v[1] = cadd(p1, q1);
+    v[4] = csub(p1, q1);
+    v[2] = cadd(p2, q2);
+    v[3] = csub(p2, q2);
+  }
+};
+
+template <bool INV, typename T2> struct Dft<8, INV, T2> {
+  static __device__ __forceinline__ void run(T2 *v) {
+    using R = decltype(T2::x);
+    const R h = (R)0.70710678118654752440084436210485;
+    T2 e[4] = {v[0], v[2], v[4], v[6]};
+    T2 o[4] = {v[1], v[3], v[5], v[7]};
+    Dft<4, INV, T2>::run(e);
+    Dft<4, INV, T2>::run(o);
+    // w8^k with sign: w1 = (1 - i)/sqrt2 fwd, (1 + i)/sqrt2 inv
+    T2 o1 = INV ? mk<T2>((o[1].x - o[1].y) * h, (o[1].x + o[1].y) * h)
+                : mk<T2>((o[1].x + o[1].y) * h, (o[1].y - o[1].x) * h);
+    T2 o2 = rot90<INV>(o[2]);
+    T2 o3 = INV ? mk<T2>(-(o[3].x + o[3].y) * h, (o[3].x - o[3].y) * h)
+                : mk<T2>((o[3].y - o[3].x) * h, -(o[3].x + o[3].y) * h);
+    v[0] = cadd(e[0], o[0]); v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);   v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);   v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);   v[7] = csub(e[3], o3);
+  }
+};
+
+template <bool INV, typename T2> struct Dft<16, INV, T2> {
+  static __device__ __forceinline__ void run(T2 *v) {
+    using R = decltype(T2::x);
+    // cos / sin of pi k / 8, k = 0..7
+    const R cs[8] = {(R)1.0, (R)0.92387953251128675612818318939679, (R)0.70710678118654752440084436210485,
+                     (R)0.38268343236508977172845998403040, (R)0.0, (R)-0.38268343236508977172845998403040,
+                     (R)-0.70710678118654752440084436210485, (R)-0.92387953251128675612818318939679};
+    const R sn[8] = {(R)0.0, (R)0.38268343236508977172845998403040, (R)0.70710678118654752440084436210485,
+                     (R)0.92387953251128675612818318939679, (R)1.0, (R)0.92387953251128675612818318939679,
+                     (R)0.70710678118654752440084436210485, (R)0.38268343236508977172845998403040};
+    T2 e[8], o[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) { e[i] = v[2 * i]; o[i] = v[2 * i + 1]; }
+    Dft<8, INV, T2>::run(e);
+    Dft<8, INV, T2>::run(o);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      T2 w = mk<T2>(cs[k], INV ? sn[k] : -sn[k]);
+      T2 t = (k == 0) ? o[0] : cmul(o[k], w);
+      v[k] = cadd(e[k], t);
+      v[k + 8] = csub(e[k], t);
+    }
+  }
+};
+
+// ---------------------------------------------------------------------
+// one Stockham stage over all lines held in LDS (in place: load phase,
+// barrier, store phase, barrier).  Element (line l, index i) lives at
+// lds[l * ls + i].  KMAX = butterflies per thread (compile-time bound).
+// ---------------------------------------------------------------------
+template <int F, int KMAX, bool INV, typename T2>
+__device__ __forceinline__ void fft_stage(T2 *lds, int nlines, int ls, int n, int s, int m,
+                                          int tstep, const T2 *__restrict__ tw) {
+  const int nbl = n / F;
+  const int total = nlines * nbl;
+  T2 v[KMAX][F];
+  int base[KMAX], ppv[KMAX];
+#pragma unroll
+  for (int k = 0; k < KMAX; ++k) {
+    const int bf = threadIdx.x + k * kThreads;
+    base[k] = -1;
+    if (bf < total) {
+      const int line = bf / nbl;
+      const int j = bf - line * nbl;
+      const int pp = j / s;
+      const int q = j - pp * s;
+      base[k] = line * ls + q;
+      ppv[k] = pp;
+#pragma unroll
+      for (int a = 0; a < F; ++a) v[k][a] = lds[base[k] + s * (pp + a * m)];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < KMAX; ++k) {
+    if (base[k] >= 0) {
+      Dft<F, INV, T2>::run(v[k]);
+      const int pp = ppv[k];
+      const int o = base[k] + s * (F * pp);
+      lds[o] = v[k][0];
+      if (pp == 0) {
+#pragma unroll
+        for (int b = 1; b < F; ++b) lds[o + s * b] = v[k][b];
+      } else {
+#pragma unroll
+        for (int b = 1; b < F; ++b) {
+          T2 w = tw[pp * b * tstep];
+          lds[o + s * b] = INV ? cmulc(v[k][b], w) : cmul(v[k][b], w);
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// Run the full plan on nlines lines of length p.n.  EPT = upper bound of
+// elements per thread (nlines * n <= kThreads * EPT).
+template <int EPT, bool INV, typename T2>
+__device__ __forceinline__ void fft_lines(T2 *lds, int nlines, int ls, const AxisPlan &p) {
+  const T2 *__restrict__ tw = (const T2 *)p.tw;
+  for (int st = 0; st < p.nst; ++st) {
+    const int F = p.rad[st], s = p.s[st], m = p.m[st], ts = p.tstep[st];
+    switch (F) {
+      case 16: fft_stage<16, (EPT + 15) / 16, INV>(lds, nlines, ls, p.n, s, m, ts, tw); break;
+      case 8: fft_stage<8, (EPT + 7) / 8, INV>(lds, nlines, ls, p.n, s, m, ts, tw); break;
+      case 4: fft_stage<4, (EPT + 3) / 4, INV>(lds, nlines, ls, p.n, s, m, ts, tw); break;
+      case 2: fft_stage<2, (EPT + 1) / 2, INV>(lds, nlines, ls, p.n, s, m, ts, tw); break;
+      case 3: fft_stage<3, (EPT + 2) / 3, INV>(lds, nlines, ls, p.n, s, m, ts, tw); break;
+      case 5: fft_stage<5, (EPT + 4) / 5, INV>(lds, nlines, ls, p.n, s, m, ts, tw); break;
+      default: break;
+    }
+  }
+}
+
+// sign (-1)^(j + n/2) applied after a forward / before an inverse pass
+__device__ __forceinline__ int jsign(int j, int fixsign) { return (j & 1) ? -fixsign : fixsign; }
+
+}  // namespace rism

@@ -1,0 +1,239 @@
+// k_potential.hip -- solute-solvent grid potentials on gfx950.
+//
+// Restates src-cuda/cal_LJ.cu:47-72 and cal_Coulomb.cu:100-155 as one pass
+// over the grid (every atom for every point, VALU-bound):
+//   ulj_v = 1/(N_A k_B T) * sum_u 4 eps_uv (s^12 - s^6),  s = sigma_uv / max(r, 0.01)
+//   de    = ubeta * sum_{r >= 1e-5} q_u / r
+//   fr    = ubeta * [sum_{r >= 1e-5} q_u (1 - exp(-r)) / r + sum_{r < 1e-5} q_u]
+//   sr    = de - fr, summed directly as the short-range remainder
+// Atom records are wave-uniform, so they arrive through scalar loads.
+// exp(-r) is evaluated only for r < 38 Angstrom: beyond that 1 - exp(-r)
+// rounds to 1 in FP64, so fr is unchanged and the dropped sr terms are below
+// 1e-16 of q/r.
+//
+// fk (cal_Coulomb.cu:125-140) = ubeta 4 pi / (k^2 (k^2 + 1)) sum_u q_u
+// exp(-i k.R_u) uses the separability exp(-i k.R) = ex(kx) ey(ky) ez(kz):
+// per-atom 1D phase tables replace the per-pair sincos.  Only the half grid
+// y < ny/2 is stored; fk(-k) = conj(fk(k)).
+#include "rism_kernels.h"
+
+namespace rism {
+
+__global__ __launch_bounds__(kThreads) void k_potentials(PotArgs a) {
+  const long ip = (long)blockIdx.x * kThreads + threadIdx.x;
+  if (ip >= a.N) return;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int x = (int)(ip % nx);
+  const int y = (int)((ip / nx) % ny);
+  const int z = (int)(ip / ((long)nx * ny));
+  const double rx = (x - nx / 2) * a.dr[0];
+  const double ry = (y - ny / 2) * a.dr[1];
+  const double rz = (z - nz / 2) * a.dr[2];
+  const double cut2 = 1.0e-2 * 1.0e-2;
+  double lj[kMaxV];
+#pragma unroll
+  for (int v = 0; v < kMaxV; ++v) lj[v] = 0.0;
+  double e = 0.0, f = 0.0, s = 0.0;
+  const int V = a.V, nu = a.natu;
+  for (int u = 0; u < nu; ++u) {
+    const double4 at = a.atoms[u];
+    const double dx = rx - at.x, dy = ry - at.y, dz = rz - at.z;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double r2c = r2 < cut2 ? cut2 : r2;
+    const double ir2 = 1.0 / r2c;
+#pragma unroll
+    for (int v = 0; v < kMaxV; ++v)
+      if (v < V) {
+        const double2 p = a.lj[v * nu + u];
+        const double s2 = p.x * ir2;
+        const double s6 = s2 * s2 * s2;
+        lj[v] += p.y * s6 * (s6 - 1.0);
+      }
+    const double ra = sqrt(r2);
+    if (ra >= 1.0e-5) {
+      const double qr = at.w / ra;
+      e += qr;
+      if (ra < 38.0) {
+        const double ex = exp(-ra);
+        f += qr * (1.0 - ex);
+        s += qr * ex;
+      } else {
+        f += qr;
+      }
+    } else {
+      f += at.w;
+      s -= at.w;
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < kMaxV; ++v)
+    if (v < V) a.ulj[(long)v * a.N + ip] = lj[v] * a.ikbt;
+  a.de[ip] = e * a.ubeta;
+  a.fr[ip] = f * a.ubeta;
+  a.sr[ip] = s * a.ubeta;
+}
+
+// phase[u][0..nx) = exp(-i kx(j) X_u), then [nx..nx+ny) for y, then z
+__global__ __launch_bounds__(kThreads) void k_phase(int nx, int ny, int nz, int natu,
+                                                    const double4 *atoms, double bx, double by,
+                                                    double bz, double2 *phase) {
+  const int nt = nx + ny + nz;
+  const long id = (long)blockIdx.x * kThreads + threadIdx.x;
+  if (id >= (long)natu * nt) return;
+  const int u = (int)(id / nt);
+  const int j = (int)(id - (long)u * nt);
+  double k, r;
+  if (j < nx) {
+    k = 2.0 * M_PI * ((double)j - nx / 2.0 + 0.5) / bx;
+    r = atoms[u].x;
+  } else if (j < nx + ny) {
+    k = 2.0 * M_PI * ((double)(j - nx) - ny / 2.0 + 0.5) / by;
+    r = atoms[u].y;
+  } else {
+    k = 2.0 * M_PI * ((double)(j - nx - ny) - nz / 2.0 + 0.5) / bz;
+    r = atoms[u].z;
+  }
+  double sn, cs;
+  sincos(k * r, &sn, &cs);
+  phase[id] = make_double2(cs, -sn);
+}
+
+__global__ __launch_bounds__(kThreads) void k_fk_half(int nx, int ny, int nz, int natu,
+                                                      const double4 *atoms, const double2 *phase,
+                                                      double bx, double by, double bz, double ubeta,
+                                                      double2 *fkh) {
+  const int nyh = ny / 2;
+  const long id = (long)blockIdx.x * kThreads + threadIdx.x;
+  if (id >= (long)nx * nyh * nz) return;
+  const int x = (int)(id % nx);
+  const int y = (int)((id / nx) % nyh);
+  const int z = (int)(id / ((long)nx * nyh));
+  const int nt = nx + ny + nz;
+  double re = 0.0, im = 0.0;
+  for (int u = 0; u < natu; ++u) {
+    const double2 px = phase[(long)u * nt + x];
+    const double2 py = phase[(long)u * nt + nx + y];
+    const double2 pz = phase[(long)u * nt + nx + ny + z];
+    const double ar = px.x * py.x - px.y * py.y, ai = px.x * py.y + px.y * py.x;
+    const double br = ar * pz.x - ai * pz.y, bi = ar * pz.y + ai * pz.x;
+    const double q = atoms[u].w;
+    re += q * br;
+    im += q * bi;
+  }
+  const double kx = 2.0 * M_PI * ((double)x - nx / 2.0 + 0.5) / bx;
+  const double ky = 2.0 * M_PI * ((double)y - ny / 2.0 + 0.5) / by;
+  const double kz = 2.0 * M_PI * ((double)z - nz / 2.0 + 0.5) / bz;
+  const double rk2 = kx * kx + ky * ky + kz * kz;
+  const double sc = 4.0 * M_PI / (rk2 * (rk2 + 1.0)) * ubeta;
+  fkh[id] = make_double2(re * sc, im * sc);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_combine_u(const double *ulj, const double *sr, double q0,
+                                                        double q1, double q2, double q3, double q4,
+                                                        double q5, double q6, double q7, int V,
+                                                        long N, R *u) {
+  const long ip = (long)blockIdx.x * kThreads + threadIdx.x;
+  if (ip >= N) return;
+  const double q[kMaxV] = {q0, q1, q2, q3, q4, q5, q6, q7};
+  const double s = sr[ip];
+#pragma unroll
+  for (int v = 0; v < kMaxV; ++v)
+    if (v < V) u[(long)v * N + ip] = (R)(ulj[(long)v * N + ip] + q[v] * s);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_convert(const double *src, R *dst, long n) {
+  const long i = (long)blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) dst[i] = (R)src[i];
+}
+
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_convert_c(const double2 *src, typename C2<R>::T *dst,
+                                                        long n) {
+  const long i = (long)blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) {
+    typename C2<R>::T v;
+    v.x = (R)src[i].x;
+    v.y = (R)src[i].y;
+    dst[i] = v;
+  }
+}
+
+template <typename S>
+__global__ __launch_bounds__(kThreads) void k_axpy_fr(S *x, const double *fr, double c0, double c1,
+                                                      double c2, double c3, double c4, double c5,
+                                                      double c6, double c7, int V, long N,
+                                                      int nslots) {
+  const long ip = (long)blockIdx.x * kThreads + threadIdx.x;
+  if (ip >= N) return;
+  const double c[kMaxV] = {c0, c1, c2, c3, c4, c5, c6, c7};
+  const double f = fr[ip];
+  for (int s = 0; s < nslots; ++s)
+#pragma unroll
+    for (int v = 0; v < kMaxV; ++v)
+      if (v < V) x[((long)s * V + v) * N + ip] += (S)(c[v] * f);
+}
+
+static inline dim3 grid1(long n) { return dim3((unsigned)((n + kThreads - 1) / kThreads)); }
+
+hipError_t launch_potentials(const PotArgs &a, hipStream_t st) {
+  hipLaunchKernelGGL(k_potentials, grid1(a.N), dim3(kThreads), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_fk_half(int nx, int ny, int nz, int natu, const double4 *atoms, const double *box,
+                          double ubeta, double2 *phase, double2 *fkh, hipStream_t st) {
+  const long nt = (long)natu * (nx + ny + nz);
+  if (nt > 0)
+    hipLaunchKernelGGL(k_phase, grid1(nt), dim3(kThreads), 0, st, nx, ny, nz, natu, atoms, box[0],
+                       box[1], box[2], phase);
+  hipLaunchKernelGGL(k_fk_half, grid1((long)nx * (ny / 2) * nz), dim3(kThreads), 0, st, nx, ny, nz,
+                     natu, atoms, phase, box[0], box[1], box[2], ubeta, fkh);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_combine_u(const double *ulj, const double *sr, const double *q, int V, long N, R *u,
+                            hipStream_t st) {
+  double c[kMaxV] = {0};
+  for (int v = 0; v < V && v < kMaxV; ++v) c[v] = q[v];
+  hipLaunchKernelGGL((k_combine_u<R>), grid1(N), dim3(kThreads), 0, st, ulj, sr, c[0], c[1], c[2], c[3],
+                     c[4], c[5], c[6], c[7], V, N, u);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_convert(const double *src, R *dst, long n, hipStream_t st) {
+  hipLaunchKernelGGL((k_convert<R>), grid1(n), dim3(kThreads), 0, st, src, dst, n);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_convert_c(const double2 *src, typename C2<R>::T *dst, long n, hipStream_t st) {
+  hipLaunchKernelGGL((k_convert_c<R>), grid1(n), dim3(kThreads), 0, st, src, dst, n);
+  return hipGetLastError();
+}
+
+template <typename S>
+hipError_t launch_axpy_fr(S *x, const double *fr, const double *coef, int V, long N, int nslots,
+                          hipStream_t st) {
+  double c[kMaxV] = {0};
+  for (int v = 0; v < V && v < kMaxV; ++v) c[v] = coef[v];
+  hipLaunchKernelGGL((k_axpy_fr<S>), grid1(N), dim3(kThreads), 0, st, x, fr, c[0], c[1], c[2], c[3], c[4],
+                     c[5], c[6], c[7], V, N, nslots);
+  return hipGetLastError();
+}
+
+template hipError_t launch_combine_u<double>(const double *, const double *, const double *, int, long,
+                                             double *, hipStream_t);
+template hipError_t launch_combine_u<float>(const double *, const double *, const double *, int, long,
+                                            float *, hipStream_t);
+template hipError_t launch_convert<double>(const double *, double *, long, hipStream_t);
+template hipError_t launch_convert<float>(const double *, float *, long, hipStream_t);
+template hipError_t launch_convert_c<double>(const double2 *, double2 *, long, hipStream_t);
+template hipError_t launch_convert_c<float>(const double2 *, float2 *, long, hipStream_t);
+template hipError_t launch_axpy_fr<double>(double *, const double *, const double *, int, long, int,
+                                           hipStream_t);
+
+}  // namespace rism

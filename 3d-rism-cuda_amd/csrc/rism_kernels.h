@@ -1,0 +1,150 @@
+// rism_kernels.h -- device-side data layout and host launchers for the
+// 3D-RISM iterate path on gfx950.
+//
+// HBM layout (N = nx*ny*nz, x fastest: ip = x + nx*(y + ny*z), V sites,
+// P = ceil(V/2) complex packs, H history slots):
+//   ts   S[V][N]      short-range state  ts_v = t_v - q_v*cf*fr
+//   res  S[V][N]      residual R = h + 1 - g  (iterate.cu's dtr)
+//   hist S[H][V][N]   Anderson history of ts and of R (two arrays)
+//   u    R[V][N]      beta*u_LJ + q_v*cf*(de - fr): the closure exponent
+//                     is ts - u = t - beta*u_LJ - q_v*cf*de (calculate_cuda.cu:6)
+//   fr   R[N]         long-range part of the solute field (cal_Coulomb.cu:117)
+//   fkh  C[nz][ny/2][nx]  its transform on the half grid y < ny/2
+//   Z    C[P][N]      work spectrum: two real sites per complex value
+//   chi  R4[V*V][ntab-1]  natural-spline coefficients of chi_vv'(k)
+// S is the state precision (double), R the field/FFT precision (double for
+// RISM_PREC_FP64, float for RISM_PREC_MIXED).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fft_core.h"
+
+namespace rism {
+
+constexpr int kMaxV = 8;    // reduced solvent sites
+constexpr int kHMax = 16;   // Anderson history slots
+
+// Solver control block in device memory.  Written by the single-workgroup
+// solve kernel, read by the field kernels to decide whether they run.
+struct Ctl {
+  int run_iter;   // iteration index whose closure/FFT/residual pass runs (-1 none)
+  int upd_iter;   // iteration whose first pass applies the pending update (-1 none)
+  int status;     // -1 running, else RISM_SOLVE_*
+  int step;       // steps done in this charge-up stage
+  int total;      // steps over all stages
+  int maxstep;
+  int count;      // remaining Picard warm-up steps (anderson.cu:43-49)
+  int write;      // next history slot to write
+  int filled;     // history slots holding data
+  int nhist;
+  int upd_nf;     // slots 0..upd_nf-1 enter the pending update
+  int upd_w;      // slot the pending update writes
+  double upd_theta[kHMax];
+  double upd_mix;
+  double tol, m, mp;
+  double rms;
+  double npts;    // V*N
+};
+
+template <typename R> struct R4T;
+template <> struct R4T<double> { using T = double4; };
+template <> struct R4T<float> { using T = float4; };
+
+template <typename R, typename S>
+struct IterArgs {
+  int nx, ny, nz;
+  long N;
+  int V, P;
+  S *ts, *res, *hist_t, *hist_r;
+  const R *u, *fr;
+  typename C2<R>::T *Z;
+  const typename C2<R>::T *fkh;
+  const typename R4T<R>::T *chi;
+  int ntab;
+  double dk;
+  double box[3];
+  double qcf[kMaxV];
+  double rho_inv[kMaxV];
+  double dv, ivol;
+  int closure;
+  Ctl *ctl;
+  double *partials;
+  int nv;           // accumulators per block in the residual pass
+  AxisPlan px, py, pz;
+  int Lx;           // x lines per workgroup
+  int Ty, Tz;       // tile widths (consecutive x) of the y / z passes
+  int xblocks;      // workgroups of the x passes (grid-stride over line groups)
+  int store_h;      // residual pass also writes real-space h into Z
+  int iter;
+};
+
+// launch configuration chosen on the host for a grid
+struct LaunchGeom {
+  int Lx, Ty, Tz;
+  int ept_x, ept_y, ept_z;  // elements-per-thread class (4, 8, 16, 32)
+  int xgroups;              // number of x line groups (ny*nz / Lx)
+  int xblocks;              // residual-pass grid size
+  size_t lds_x, lds_y, lds_z;
+};
+
+// ---- iteration launchers (k_iterate.hip) ----
+template <typename R, typename S>
+hipError_t launch_closure_fx(const IterArgs<R, S> &a, const LaunchGeom &g, bool update, int hm,
+                             hipStream_t st);
+template <typename R, typename S>
+hipError_t launch_fft_y(const IterArgs<R, S> &a, const LaunchGeom &g, bool inverse, hipStream_t st);
+template <typename R, typename S>
+hipError_t launch_fft_z_oz(const IterArgs<R, S> &a, const LaunchGeom &g, hipStream_t st);
+template <typename R, typename S>
+hipError_t launch_fx_resid(const IterArgs<R, S> &a, const LaunchGeom &g, int hm, hipStream_t st);
+hipError_t launch_anderson_solve(Ctl *ctl, const double *partials, int nblk, int nv, int hm,
+                                 double *rms_hist, int iter, hipStream_t st);
+// standalone 3D FFT on a caller buffer (reference FFT3D::execute seam)
+template <typename R>
+hipError_t launch_fft3d(typename C2<R>::T *data, const AxisPlan &px, const AxisPlan &py,
+                        const AxisPlan &pz, const LaunchGeom &g, int nx, int ny, int nz, double dv,
+                        double ivol, int dir, hipStream_t st);
+
+// ---- potentials (k_potential.hip) ----
+struct PotArgs {
+  int nx, ny, nz;
+  long N;
+  int V, natu;
+  double dr[3];
+  double box[3];
+  const double4 *atoms;  // x, y, z, q (centred coordinates)
+  const double2 *lj;     // [V][natu]: (sigma_uv^2, 4*eps_uv)
+  double ikbt, ubeta;
+  double *ulj, *de, *fr, *sr;  // sr = de - fr
+};
+hipError_t launch_potentials(const PotArgs &a, hipStream_t st);
+hipError_t launch_fk_half(int nx, int ny, int nz, int natu, const double4 *atoms, const double *box,
+                          double ubeta, double2 *phase /*[natu*(nx+ny+nz)]*/, double2 *fkh,
+                          hipStream_t st);
+template <typename R>
+hipError_t launch_combine_u(const double *ulj, const double *sr, const double *qcf, int V, long N,
+                            R *u, hipStream_t st);
+template <typename R>
+hipError_t launch_convert(const double *src, R *dst, long n, hipStream_t st);
+template <typename R>
+hipError_t launch_convert_c(const double2 *src, typename C2<R>::T *dst, long n, hipStream_t st);
+template <typename S>
+hipError_t launch_axpy_fr(S *x, const double *fr, const double *coef, int V, long N, int nslots,
+                          hipStream_t st);  // x[s][v] += coef[v] * fr
+
+// ---- thermodynamics (k_thermo.hip) ----
+template <typename R, typename S>
+hipError_t launch_gfield(const IterArgs<R, S> &a, double *g, hipStream_t st);
+template <typename R, typename S>
+hipError_t launch_exchem(const IterArgs<R, S> &a, const double *frd, double *partials, int nblk,
+                         hipStream_t st);
+hipError_t launch_euv(int nx, int ny, int nz, const double *dr, int V, int natu, const double4 *atoms,
+                      const double2 *lj, const double *sigh, const double *qv, const double *rho,
+                      const double *g, double *partials, int nblk, hipStream_t st);
+hipError_t launch_grad(int nx, int ny, int nz, const double *dr, int V, int natu,
+                       const double4 *atoms, const double2 *lj, const double *sigh,
+                       const double *qv, const double *rho, const double *g, double *partials,
+                       int nblk, double *out, hipStream_t st);
+
+}  // namespace rism

@@ -1,0 +1,1039 @@
+// solver.cpp -- implementation of the rism_* C ABI (include/rism3d_amd.h):
+// handle set-up, device buffers, the iterate driver and the thermodynamic
+// reductions.  The driver restates src-cuda/iterate.cu:6-90 (restart,
+// initial estimate, charge-up, step loop, divergence and step-limit
+// handling) on top of the fused kernels in k_iterate.hip.
+#include <hip/hip_runtime.h>
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rism_host.h"
+#include "rism_kernels.h"
+
+using namespace rism;
+
+namespace {
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+bool file_exists(const char *p) {
+  struct stat st;
+  return p && p[0] && stat(p, &st) == 0;
+}
+
+// factor n into radices {16, 8, 4, 2, 3, 5}: as few power-of-two stages as
+// possible with their exponents spread evenly, then the odd radices
+bool make_radices(int n, std::vector<int> &rad) {
+  rad.clear();
+  int e = 0, m = n;
+  while (m % 2 == 0) { m /= 2; ++e; }
+  if (e > 0) {
+    const int ns = (e + 3) / 4;
+    for (int s = 0; s < ns; ++s) {
+      const int b = e / ns + (s < e % ns ? 1 : 0);
+      rad.push_back(1 << b);
+    }
+  }
+  while (m % 3 == 0) { rad.push_back(3); m /= 3; }
+  while (m % 5 == 0) { rad.push_back(5); m /= 5; }
+  return m == 1 && (int)rad.size() <= kMaxStages;
+}
+
+int ept_class(long elems) {
+  const long per = (elems + kThreads - 1) / kThreads;
+  if (per <= 4) return 4;
+  if (per <= 8) return 8;
+  if (per <= 16) return 16;
+  if (per <= 32) return 32;
+  return -1;
+}
+
+int largest_divisor_le(long n, long cap) {
+  for (long d = std::min(n, cap); d >= 1; --d)
+    if (n % d == 0) return (int)d;
+  return 1;
+}
+
+}  // namespace
+
+struct rism_handle {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int nx = 0, ny = 0, nz = 0;
+  long N = 0;
+  double box[3] = {0, 0, 0}, dr[3] = {0, 0, 0}, dv = 0, volume = 0;
+  int V = 0, P = 0;
+  std::vector<double> rho, qv, sigv, epsv;
+  double temper = 0, xt = 0, dk = 0;
+  int ntab = 0;
+  std::vector<double> chi;  // [ntab][V][V]
+  int natu = 0;
+  std::vector<double> usig, ueps, uq, uxyz;
+  rism_opts opts{};
+  int hm = 2;               // compiled history class >= nhist
+  size_t rsz = 8;           // sizeof(R)
+  AxisPlan plan[3];
+  LaunchGeom geom{};
+  // device memory
+  std::vector<DevBuf> bufs;
+  double *ulj = nullptr, *de = nullptr, *fr = nullptr, *sr = nullptr;
+  double2 *fkh = nullptr, *phase = nullptr;
+  void *uR = nullptr, *frR = nullptr, *fkhR = nullptr, *chiR = nullptr, *Z = nullptr;
+  double *ts = nullptr, *res = nullptr, *hist_t = nullptr, *hist_r = nullptr;
+  double *partials = nullptr, *rms_hist = nullptr, *gfield = nullptr;
+  Ctl *ctl = nullptr;
+  Ctl *ctl_host = nullptr;  // pinned
+  double4 *atoms = nullptr;
+  double2 *lj = nullptr;
+  double *sigh = nullptr, *d_qv = nullptr, *d_rho = nullptr;
+  void *tables = nullptr;
+  // state
+  bool potentials = false, solved = false, evaluated = false;
+  double cf = 1.0;
+  int iter = 0;
+  int steps = 0;
+  double rms = 0.0;
+  double t_setup = 0, t_solve = 0, t_thermo = 0;
+  std::vector<double> rms_log;
+};
+
+#define HIP_OK(...)                                                                        \
+  do {                                                                                     \
+    hipError_t _e = (__VA_ARGS__);                                                         \
+    if (_e != hipSuccess) {                                                                \
+      set_error(std::string("HIP error: ") + hipGetErrorString(_e) + " at " #__VA_ARGS__); \
+      return RISM_ERR_HIP;                                                                 \
+    }                                                                                      \
+  } while (0)
+
+namespace {
+
+int dalloc(rism_handle *h, void **p, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    set_error("device allocation of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+    *p = nullptr;
+    return RISM_ERR_NOMEM;
+  }
+  h->bufs.push_back({*p, bytes});
+  return RISM_OK;
+}
+
+template <typename T> int dalloc_t(rism_handle *h, T **p, size_t count) {
+  void *q = nullptr;
+  int rc = dalloc(h, &q, count * sizeof(T));
+  *p = (T *)q;
+  return rc;
+}
+
+// per-axis plan + twiddle tables in the iteration precision
+template <typename R> int build_plans(rism_handle *h) {
+  using T2 = typename C2<R>::T;
+  const int ns[3] = {h->nx, h->ny, h->nz};
+  size_t tot = 0;
+  for (int a = 0; a < 3; ++a) tot += 2 * (size_t)ns[a];
+  T2 *dt = nullptr;
+  if (int rc = dalloc_t(h, &dt, tot)) return rc;
+  std::vector<T2> host(tot);
+  size_t off = 0;
+  for (int a = 0; a < 3; ++a) {
+    const int n = ns[a];
+    std::vector<int> rad;
+    if (!make_radices(n, rad)) {
+      set_error("unsupported grid length " + std::to_string(n));
+      return RISM_ERR_ARG;
+    }
+    AxisPlan &p = h->plan[a];
+    std::memset(&p, 0, sizeof p);
+    p.n = n;
+    p.nst = (int)rad.size();
+    int nn = n, s = 1;
+    for (int k = 0; k < p.nst; ++k) {
+      p.rad[k] = rad[k];
+      p.s[k] = s;
+      p.m[k] = nn / rad[k];
+      p.tstep[k] = n / nn;
+      nn /= rad[k];
+      s *= rad[k];
+    }
+    p.fixsign = ((n / 2) % 2) ? -1 : 1;
+    for (int k = 0; k < n; ++k) {
+      const double ang = 2.0 * M_PI * (double)k / (double)n;
+      host[off + k].x = (R)std::cos(ang);
+      host[off + k].y = (R)-std::sin(ang);
+      const double ph = M_PI * (double)(k - n / 2) / (double)n;
+      const double sg = (k & 1) ? -1.0 : 1.0;
+      host[off + n + k].x = (R)(sg * std::cos(ph));
+      host[off + n + k].y = (R)(-sg * std::sin(ph));
+    }
+    p.tw = dt + off;
+    p.pre = dt + off + n;
+    off += 2 * (size_t)n;
+  }
+  HIP_OK(hipMemcpy(dt, host.data(), tot * sizeof(T2), hipMemcpyHostToDevice));
+  h->tables = dt;
+  return RISM_OK;
+}
+
+long env_long(const char *name, long dflt) {
+  const char *s = std::getenv(name);
+  return (s && *s) ? std::atol(s) : dflt;
+}
+
+// Tile shapes: x passes hold Lx whole lines (about 1024 elements), the y
+// pass Ty consecutive-x columns, the z pass two mirrored tiles of Tz columns
+// per pack.  Every workgroup's LDS (padded lines) stays within RISM_LDS_CAP
+// bytes (64 KiB by default).  RISM_LX / RISM_TY / RISM_TZ override the
+// choices (tuning sweeps).
+int choose_geometry(rism_handle *h) {
+  LaunchGeom &g = h->geom;
+  const int nx = h->nx, ny = h->ny, nz = h->nz, P = h->P;
+  const long c2 = 2 * (long)h->rsz;
+  const long cap = env_long("RISM_LDS_CAP", 65536);
+  const long xe = env_long("RISM_X_ELEMS", 1024);
+  g.Lx = largest_divisor_le((long)ny * nz, std::max(1L, std::min(xe / ((long)P * nx), cap / ((long)P * (nx + 1) * c2))));
+  g.Ty = largest_divisor_le(nx, std::max(1L, std::min(8192L / ny, cap / ((ny + 1) * c2))));
+  g.Tz = largest_divisor_le(nx, std::max(1L, std::min(8192L / (2L * P * nz), cap / (2L * P * (nz + 1) * c2))));
+  const long lx = env_long("RISM_LX", 0), ty = env_long("RISM_TY", 0), tz = env_long("RISM_TZ", 0);
+  if (lx > 0 && ((long)ny * nz) % lx == 0) g.Lx = (int)lx;
+  if (ty > 0 && nx % ty == 0) g.Ty = (int)ty;
+  if (tz > 0 && nx % tz == 0) g.Tz = (int)tz;
+  g.ept_x = ept_class((long)P * g.Lx * nx);
+  g.xgroups = (int)((long)ny * nz / g.Lx);
+  g.xblocks = std::min(g.xgroups, 1024);
+  g.lds_x = (size_t)P * g.Lx * (nx + 1) * c2;
+  g.ept_y = ept_class((long)g.Ty * ny);
+  g.lds_y = (size_t)g.Ty * (ny + 1) * c2;
+  g.ept_z = ept_class(2L * P * g.Tz * nz);
+  g.lds_z = (size_t)2 * P * g.Tz * (nz + 1) * c2;
+  if (g.ept_x < 0 || g.ept_y < 0 || g.ept_z < 0 || (long)g.lds_x > cap || (long)g.lds_y > cap ||
+      (long)g.lds_z > cap) {
+    set_error("grid too large for the single-workgroup line transforms");
+    return RISM_ERR_ARG;
+  }
+  return RISM_OK;
+}
+
+template <typename R> IterArgs<R, double> make_args(rism_handle *h, int iter) {
+  IterArgs<R, double> a;
+  std::memset(&a, 0, sizeof a);
+  a.nx = h->nx; a.ny = h->ny; a.nz = h->nz; a.N = h->N;
+  a.V = h->V; a.P = h->P;
+  a.ts = h->ts; a.res = h->res; a.hist_t = h->hist_t; a.hist_r = h->hist_r;
+  a.u = (const R *)h->uR;
+  a.fr = (const R *)h->frR;
+  a.Z = (typename C2<R>::T *)h->Z;
+  a.fkh = (const typename C2<R>::T *)h->fkhR;
+  a.chi = (const typename R4T<R>::T *)h->chiR;
+  a.ntab = h->ntab;
+  a.dk = h->dk;
+  for (int k = 0; k < 3; ++k) a.box[k] = h->box[k];
+  for (int v = 0; v < h->V; ++v) {
+    a.qcf[v] = h->qv[v] * h->cf;
+    a.rho_inv[v] = 1.0 / h->rho[v];
+  }
+  a.dv = h->dv;
+  a.ivol = 1.0 / h->volume;
+  a.closure = h->opts.closure;
+  a.ctl = h->ctl;
+  a.partials = h->partials;
+  a.nv = 1 + h->hm + h->hm * (h->hm + 1) / 2;
+  a.px = h->plan[0]; a.py = h->plan[1]; a.pz = h->plan[2];
+  a.Lx = h->geom.Lx; a.Ty = h->geom.Ty; a.Tz = h->geom.Tz;
+  a.xblocks = h->geom.xblocks;
+  a.store_h = 0;
+  a.iter = iter;
+  return a;
+}
+
+template <typename R> int launch_iteration(rism_handle *h, int it, bool store_h, bool solve) {
+  IterArgs<R, double> a = make_args<R>(h, it);
+  a.store_h = store_h ? 1 : 0;
+  HIP_OK(launch_closure_fx<R, double>(a, h->geom, true, h->hm, h->stream));
+  HIP_OK(launch_fft_y<R, double>(a, h->geom, false, h->stream));
+  HIP_OK(launch_fft_z_oz<R, double>(a, h->geom, h->stream));
+  HIP_OK(launch_fft_y<R, double>(a, h->geom, true, h->stream));
+  HIP_OK(launch_fx_resid<R, double>(a, h->geom, h->hm, h->stream));
+  if (solve)
+    HIP_OK(launch_anderson_solve(h->ctl, h->partials, h->geom.xblocks, a.nv, h->hm, h->rms_hist, it,
+                                 h->stream));
+  return RISM_OK;
+}
+
+// the closure exponent uses u = beta u_LJ + q_v cf (de - fr) for this stage
+template <typename R> int stage_potential(rism_handle *h) {
+  std::vector<double> q(h->V);
+  for (int v = 0; v < h->V; ++v) q[v] = h->qv[v] * h->cf;
+  HIP_OK(launch_combine_u<R>(h->ulj, h->sr, q.data(), h->V, h->N, (R *)h->uR, h->stream));
+  return RISM_OK;
+}
+
+int put_ctl(rism_handle *h) {
+  HIP_OK(hipMemcpyAsync(h->ctl, h->ctl_host, sizeof(Ctl), hipMemcpyHostToDevice, h->stream));
+  HIP_OK(hipStreamSynchronize(h->stream));
+  return RISM_OK;
+}
+
+int get_ctl(rism_handle *h) {
+  HIP_OK(hipMemcpyAsync(h->ctl_host, h->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, h->stream));
+  HIP_OK(hipStreamSynchronize(h->stream));
+  return RISM_OK;
+}
+
+// full t = ts + q_v cf fr written to host [V][N]
+int fetch_t(rism_handle *h, double *out) {
+  const size_t VN = (size_t)h->V * h->N;
+  std::vector<double> fr(h->N);
+  HIP_OK(hipMemcpy(out, h->ts, VN * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(fr.data(), h->fr, h->N * sizeof(double), hipMemcpyDeviceToHost));
+  for (int v = 0; v < h->V; ++v) {
+    const double q = h->qv[v] * h->cf;
+    for (long i = 0; i < h->N; ++i) out[(size_t)v * h->N + i] += q * fr[i];
+  }
+  return RISM_OK;
+}
+
+int store_t(rism_handle *h, const double *t) {
+  const size_t VN = (size_t)h->V * h->N;
+  std::vector<double> fr(h->N), ts(VN);
+  HIP_OK(hipMemcpy(fr.data(), h->fr, h->N * sizeof(double), hipMemcpyDeviceToHost));
+  for (int v = 0; v < h->V; ++v) {
+    const double q = h->qv[v] * h->cf;
+    for (long i = 0; i < h->N; ++i) ts[(size_t)v * h->N + i] = t[(size_t)v * h->N + i] - q * fr[i];
+  }
+  HIP_OK(hipMemcpy(h->ts, ts.data(), VN * sizeof(double), hipMemcpyHostToDevice));
+  return RISM_OK;
+}
+
+int write_tuv_file(rism_handle *h, const char *path) {
+  // write_tuv.cu:7-33: header "nx ny nz natv", then one value per line,
+  // default stream precision (6 significant digits)
+  const size_t VN = (size_t)h->V * h->N;
+  std::vector<double> t(VN);
+  if (int rc = fetch_t(h, t.data())) return rc;
+  FILE *f = std::fopen(path, "w");
+  if (!f) {
+    set_error(std::string("cannot write ") + path);
+    return RISM_ERR_IO;
+  }
+  std::fprintf(f, "%d %d %d %d\n", h->nx, h->ny, h->nz, h->V);
+  for (size_t i = 0; i < VN; ++i) std::fprintf(f, "%g\n", t[i]);
+  std::fclose(f);
+  return RISM_OK;
+}
+
+int read_tuv_file(rism_handle *h, const char *path) {
+  FILE *f = std::fopen(path, "r");
+  if (!f) {
+    set_error(std::string("cannot open ") + path);
+    return RISM_ERR_IO;
+  }
+  int a, b, c, d;
+  if (std::fscanf(f, "%d %d %d %d", &a, &b, &c, &d) != 4) {
+    std::fclose(f);
+    set_error("malformed .tuv header");
+    return RISM_ERR_FORMAT;
+  }
+  if (a != h->nx || b != h->ny || c != h->nz || d != h->V) {
+    // read_tuv.cu:14-20 warns and keeps reading; a short file is an error here
+    std::printf(" RXRISM:  actual   NGr = %ld   NatV = %d\n           saved   NGr = %ld   NatV = %d\n",
+                h->N, h->V, (long)a * b * c, d);
+  }
+  const size_t VN = (size_t)h->V * h->N;
+  std::vector<double> t(VN);
+  for (size_t i = 0; i < VN; ++i)
+    if (std::fscanf(f, "%lf", &t[i]) != 1) {
+      std::fclose(f);
+      set_error(".tuv file is shorter than the grid");
+      return RISM_ERR_FORMAT;
+    }
+  std::fclose(f);
+  return store_t(h, t.data());
+}
+
+template <typename R> int solve_impl(rism_handle *h, const char *tuv_path, int *steps_out, double *rms_out) {
+  const double t0 = now_s();
+  const int V = h->V;
+  const long N = h->N;
+  const size_t VN = (size_t)V * N;
+  int cu = h->opts.chargeup;
+  double cuf = 0.0;
+  const bool verbose = h->opts.verbose != 0;
+  h->evaluated = false;
+  // history and state start from zero (anderson.cu allocates without clearing;
+  // unfilled slots are never read here)
+  HIP_OK(hipMemsetAsync(h->hist_t, 0, (size_t)h->opts.nhist * VN * sizeof(double), h->stream));
+  HIP_OK(hipMemsetAsync(h->hist_r, 0, (size_t)h->opts.nhist * VN * sizeof(double), h->stream));
+  HIP_OK(hipMemsetAsync(h->res, 0, VN * sizeof(double), h->stream));
+  if (file_exists(tuv_path)) {  // iterate.cu:25-33
+    h->cf = 1.0;
+    if (int rc = read_tuv_file(h, tuv_path)) return rc;
+    cu = 0;
+  } else if (cu == 0) {  // initialize_tuv.cu: t = q_v fr, i.e. ts = 0 at cf = 1
+    if (verbose) std::printf("synthesizing initial estimate for Tuv ...\n");
+    h->cf = 1.0;
+    HIP_OK(hipMemsetAsync(h->ts, 0, VN * sizeof(double), h->stream));
+  } else {  // charge-up from t = 0, cf = 0
+    h->cf = 0.0;
+    cuf = 1.0 / cu;
+    HIP_OK(hipMemsetAsync(h->ts, 0, VN * sizeof(double), h->stream));
+  }
+  Ctl &c = *h->ctl_host;
+  std::memset(&c, 0, sizeof c);
+  c.count = h->opts.count;
+  c.nhist = h->opts.nhist;
+  c.maxstep = h->opts.maxstep;
+  c.tol = h->opts.tol;
+  c.m = h->opts.m;
+  c.mp = h->opts.mp;
+  c.npts = (double)VN;
+  c.upd_iter = -1;
+  int status = RISM_SOLVE_CONVERGED;
+  h->rms_log.clear();
+  int it = h->iter;
+  const int chunk = (h->opts.verbose ? 1 : (N >= (1L << 21) ? 4 : 16));
+  int printed = 0;
+  for (int stage = 0; stage <= cu; ++stage) {
+    if (stage > 0) {  // add_tuv.cu: t += q_v cuf fr; ts absorbs any clamp remainder
+      const double cf_old = h->cf;
+      h->cf += cuf;
+      if (h->cf > 1.0) h->cf = 1.0;
+      const double rem = cuf - (h->cf - cf_old);
+      const double dcf = h->cf - cf_old;
+      std::vector<double> coef(V), hcoef(V);
+      for (int v = 0; v < V; ++v) {
+        coef[v] = h->qv[v] * rem;
+        hcoef[v] = -h->qv[v] * dcf;  // history keeps the same t in the new frame
+      }
+      if (rem != 0.0) HIP_OK(launch_axpy_fr<double>(h->ts, h->fr, coef.data(), V, N, 1, h->stream));
+      HIP_OK(launch_axpy_fr<double>(h->hist_t, h->fr, hcoef.data(), V, N, h->opts.nhist, h->stream));
+    }
+    if (int rc = stage_potential<R>(h)) return rc;
+    if (verbose) std::printf("relaxing 3D UV RISM: Charge Up Factor = %g\n", h->cf);
+    c.status = -1;
+    c.step = 0;
+    c.run_iter = it;
+    c.upd_iter = -1;
+    if (int rc = put_ctl(h)) return rc;
+    while (true) {
+      const int n_launch = (h->opts.ksave > 0) ? 1 : chunk;
+      for (int k = 0; k < n_launch; ++k, ++it)
+        if (int rc = launch_iteration<R>(h, it, false, true)) return rc;
+      if (int rc = get_ctl(h)) return rc;
+      // progress lines (iterate.cu:63)
+      if (c.total > printed) {
+        std::vector<double> tail(c.total - printed);
+        HIP_OK(hipMemcpy(tail.data(), h->rms_hist + printed, tail.size() * sizeof(double),
+                         hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < tail.size(); ++k) {
+          h->rms_log.push_back(tail[k]);
+          if (verbose)
+            std::printf(" Step = %d Reside = %g\n", (int)(c.step - (tail.size() - 1 - k)), tail[k]);
+        }
+        printed = c.total;
+      }
+      if (h->opts.ksave > 0 && c.step > 0 && c.step % h->opts.ksave == 0 && c.status != RISM_SOLVE_DIVERGED &&
+          tuv_path && tuv_path[0]) {
+        // iterate.cu:64-66 writes t after the Anderson update of this step:
+        // apply the pending update now (one first pass), then clear it so
+        // the next iteration does not apply it again
+        if (c.upd_iter == it) {
+          IterArgs<R, double> a = make_args<R>(h, it);
+          HIP_OK(launch_closure_fx<R, double>(a, h->geom, true, h->hm, h->stream));
+          c.upd_iter = -1;
+          if (int rc = put_ctl(h)) return rc;
+        }
+        if (verbose) std::printf("saving Tuv to file:  %s  ...\n", tuv_path);
+        if (int rc = write_tuv_file(h, tuv_path)) return rc;
+      }
+      if (c.status >= 0) break;
+    }
+    // a step-limit stop still applies its pending update (iterate.cu:61 runs
+    // before the loop ends)
+    if (c.upd_iter >= 0 && c.upd_iter == it) {
+      c.run_iter = -1;  // update only, no closure/FFT
+      if (int rc = put_ctl(h)) return rc;
+      IterArgs<R, double> a = make_args<R>(h, it);
+      HIP_OK(launch_closure_fx<R, double>(a, h->geom, true, h->hm, h->stream));
+      ++it;
+    }
+    status = c.status;
+    if (status == RISM_SOLVE_DIVERGED) {
+      if (verbose) std::printf("Calculation diverged.\n");
+    } else if (status == RISM_SOLVE_MAXSTEP) {
+      if (verbose) std::printf("3D UV RISM: reached limit # of relaxation steps: %d\n", h->opts.maxstep);
+      break;
+    } else if (h->opts.ksave != 0 && stage == cu && tuv_path && tuv_path[0]) {
+      if (verbose) std::printf("saving Tuv to file:  %s  ...\n", tuv_path);
+      if (int rc = write_tuv_file(h, tuv_path)) return rc;
+    }
+  }
+  HIP_OK(hipStreamSynchronize(h->stream));
+  h->iter = it;
+  h->steps = c.total;
+  h->rms = c.rms;
+  h->solved = true;
+  h->t_solve = now_s() - t0;
+  if (steps_out) *steps_out = c.total;
+  if (rms_out) *rms_out = c.rms;
+  return status;
+}
+
+// re-evaluate the map at the final state, keeping real-space h in Z and g
+template <typename R> int evaluate_impl(rism_handle *h) {
+  Ctl &c = *h->ctl_host;
+  if (int rc = get_ctl(h)) return rc;
+  const int it = h->iter++;
+  c.run_iter = it;
+  c.upd_iter = -1;
+  const int keep = c.status;
+  if (int rc = put_ctl(h)) return rc;
+  if (int rc = launch_iteration<R>(h, it, true, false)) return rc;
+  IterArgs<R, double> a = make_args<R>(h, it);
+  HIP_OK(launch_gfield<R, double>(a, h->gfield, h->stream));
+  c.run_iter = -1;
+  c.status = keep;
+  if (int rc = put_ctl(h)) return rc;
+  h->evaluated = true;
+  return RISM_OK;
+}
+
+int evaluate(rism_handle *h) {
+  return h->opts.precision == RISM_PREC_MIXED ? evaluate_impl<float>(h) : evaluate_impl<double>(h);
+}
+
+template <typename R> int thermo_impl(rism_handle *h, rism_xmu *x) {
+  const int V = h->V;
+  const int nblk = 1024;
+  IterArgs<R, double> a = make_args<R>(h, h->iter);
+  HIP_OK(launch_exchem<R, double>(a, h->fr, h->partials, nblk, h->stream));
+  std::vector<double> part((size_t)nblk * 4 * V);
+  HIP_OK(hipMemcpyAsync(part.data(), h->partials, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        h->stream));
+  HIP_OK(hipStreamSynchronize(h->stream));
+  std::vector<double> tot(4 * V, 0.0);
+  for (int b = 0; b < nblk; ++b)
+    for (int k = 0; k < 4 * V; ++k) tot[k] += part[(size_t)b * 4 * V + k];
+  HIP_OK(launch_euv(h->nx, h->ny, h->nz, h->dr, V, h->natu, h->atoms, h->lj, h->sigh, h->d_qv, h->d_rho,
+                    h->gfield, h->partials, nblk, h->stream));
+  std::vector<double> pe((size_t)nblk * 2);
+  HIP_OK(hipMemcpyAsync(pe.data(), h->partials, pe.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        h->stream));
+  HIP_OK(hipStreamSynchronize(h->stream));
+  double se_lj = 0.0, se_es = 0.0;
+  for (int b = 0; b < nblk; ++b) {
+    se_lj += pe[2 * b];
+    se_es += pe[2 * b + 1];
+  }
+  std::memset(x, 0, sizeof *x);
+  x->nsite = V;
+  const double ibeta = kAvogadro * kBoltzmann * h->temper;
+  double sc = 0.0, schnc = 0.0, gf = 0.0, cuv = 0.0;
+  for (int v = 0; v < V; ++v) {  // cal_exchem.cu:9-51, output_xmu.cu:31-73
+    const double s0 = tot[4 * v + 0] * h->rho[v] * h->dv;
+    const double s1 = tot[4 * v + 1] * h->rho[v] * h->dv;
+    const double s2 = tot[4 * v + 2] * h->rho[v] * h->dv;
+    x->sfec_sc[v] = ibeta * s0;
+    x->sfec_sc_hnc[v] = h->opts.closure == RISM_CLOSURE_KH ? ibeta * s1 : 0.0;
+    x->sfec_gf[v] = ibeta * s2;
+    sc += s0;
+    if (h->opts.closure == RISM_CLOSURE_KH) schnc += s1;
+    gf += s2;
+    cuv += tot[4 * v + 3] * h->rho[v];
+  }
+  x->sfe_sc = ibeta * sc;
+  x->sfe_sc_hnc = ibeta * schnc;
+  x->sfe_gf = ibeta * gf;
+  x->se_lj = se_lj * h->dv;
+  x->se_es = se_es * h->dv;
+  x->se = (se_lj + se_es) * h->dv;
+  cuv *= h->dv;  // cal_pmv.cu:5-22
+  const double pmv = h->xt * (1.0 - cuv) * ibeta;
+  double pr = 0.0;  // cal_pressure.cu:6-20
+  for (int v = 0; v < V; ++v) pr += h->rho[v];
+  pr *= 1.0e30 / kAvogadro;
+  pr = 0.5 * (pr + 1.0 / (h->xt * ibeta));
+  x->pmv = pmv * 1000.0;
+  x->pressure = ibeta * pr;
+  x->correction = -pr * pmv * ibeta;
+  return RISM_OK;
+}
+
+template <typename R> int alloc_precision(rism_handle *h) {
+  using T2 = typename C2<R>::T;
+  using T4 = typename R4T<R>::T;
+  const size_t VN = (size_t)h->V * h->N;
+  if (int rc = build_plans<R>(h)) return rc;
+  if (int rc = choose_geometry(h)) return rc;
+  if (int rc = dalloc(h, &h->uR, VN * sizeof(R))) return rc;
+  if (sizeof(R) == sizeof(double)) {
+    h->frR = h->fr;
+    h->fkhR = h->fkh;
+  } else {
+    if (int rc = dalloc(h, &h->frR, h->N * sizeof(R))) return rc;
+    if (int rc = dalloc(h, &h->fkhR, (size_t)h->nx * (h->ny / 2) * h->nz * sizeof(T2))) return rc;
+  }
+  if (int rc = dalloc(h, &h->Z, (size_t)h->P * h->N * sizeof(T2))) return rc;
+  // chi spline table [V*V][ntab-1] of (y_i, c0, c1, c2)
+  const int nt = h->ntab, V = h->V;
+  std::vector<double> kt(nt), y(nt), c0(nt), c1(nt), c2(nt);
+  for (int i = 0; i < nt; ++i) kt[i] = h->dk * (i + 1);
+  std::vector<T4> tab((size_t)V * V * (nt - 1));
+  for (int v = 0; v < V; ++v)
+    for (int v1 = 0; v1 < V; ++v1) {
+      for (int i = 0; i < nt; ++i) y[i] = h->chi[((size_t)i * V + v) * V + v1];
+      natural_spline(kt.data(), y.data(), nt, c0.data(), c1.data(), c2.data());
+      for (int i = 0; i < nt - 1; ++i) {
+        T4 &e = tab[(size_t)(v * V + v1) * (nt - 1) + i];
+        e.x = (R)y[i];
+        e.y = (R)c0[i];
+        e.z = (R)c1[i];
+        e.w = (R)c2[i];
+      }
+    }
+  if (int rc = dalloc(h, &h->chiR, tab.size() * sizeof(T4))) return rc;
+  HIP_OK(hipMemcpy(h->chiR, tab.data(), tab.size() * sizeof(T4), hipMemcpyHostToDevice));
+  return RISM_OK;
+}
+
+int create_impl(const rism_grid *grid, const rism_solvent *sv, const rism_solute *su, const rism_opts *o,
+                int device, rism_handle *h) {
+  const double t0 = now_s();
+  for (int k = 0; k < 3; ++k) {
+    if (!is_supported_length(grid->n[k])) {
+      set_error("grid length " + std::to_string(grid->n[k]) +
+                " unsupported: need even, 2,3,5-smooth, 4..1024");
+      return RISM_ERR_ARG;
+    }
+    if (!(grid->box[k] > 0.0)) {
+      set_error("box lengths must be positive");
+      return RISM_ERR_ARG;
+    }
+  }
+  if (sv->nsite < 1 || sv->nsite > kMaxV || sv->ntab < 4 || !(sv->dk > 0.0)) {
+    set_error("bad solvent description (1..8 sites, ntab >= 4, dk > 0)");
+    return RISM_ERR_ARG;
+  }
+  if (o->nhist < 1 || o->nhist > 10) {
+    set_error("nhist must be 1..10");
+    return RISM_ERR_ARG;
+  }
+  if (o->closure != RISM_CLOSURE_KH && o->closure != RISM_CLOSURE_HNC) {
+    set_error("unknown closure");
+    return RISM_ERR_ARG;
+  }
+  if (o->precision != RISM_PREC_FP64 && o->precision != RISM_PREC_MIXED) {
+    set_error("unknown precision");
+    return RISM_ERR_ARG;
+  }
+  if (o->maxstep < 1) {
+    set_error("maxstep must be >= 1");
+    return RISM_ERR_ARG;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    set_error("no HIP device available");
+    return RISM_ERR_HIP;
+  }
+  if (device < 0 || device >= ndev) {
+    set_error("device index out of range");
+    return RISM_ERR_ARG;
+  }
+  h->device = device;
+  HIP_OK(hipSetDevice(device));
+  HIP_OK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  h->nx = grid->n[0]; h->ny = grid->n[1]; h->nz = grid->n[2];
+  h->N = (long)h->nx * h->ny * h->nz;
+  for (int k = 0; k < 3; ++k) {  // cell.cc:8-22
+    h->box[k] = grid->box[k];
+    h->dr[k] = grid->box[k] / grid->n[k];
+  }
+  h->volume = h->box[0] * h->box[1] * h->box[2];
+  h->dv = h->volume / (double)h->N;
+  h->V = sv->nsite;
+  h->P = (h->V + 1) / 2;
+  h->rho.assign(sv->rho, sv->rho + h->V);
+  h->qv.assign(sv->q, sv->q + h->V);
+  h->sigv.assign(sv->sig, sv->sig + h->V);
+  h->epsv.assign(sv->eps, sv->eps + h->V);
+  h->temper = sv->temper;
+  h->xt = sv->xt;
+  h->dk = sv->dk;
+  h->ntab = sv->ntab;
+  h->chi.assign(sv->chi, sv->chi + (size_t)sv->ntab * h->V * h->V);
+  h->natu = su->natom;
+  h->usig.assign(su->sig, su->sig + su->natom);
+  h->ueps.assign(su->eps, su->eps + su->natom);
+  h->uq.assign(su->q, su->q + su->natom);
+  h->uxyz.assign(su->xyz, su->xyz + 3 * (size_t)su->natom);
+  h->opts = *o;
+  h->hm = o->nhist <= 2 ? 2 : (o->nhist <= 5 ? 5 : 10);
+  h->rsz = o->precision == RISM_PREC_MIXED ? sizeof(float) : sizeof(double);
+  // k range check (solvent_spline.cu:18-21)
+  double kmax2 = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    const double km = 2.0 * M_PI * (grid->n[k] - 1 - grid->n[k] / 2.0 + 0.5) / grid->box[k];
+    kmax2 += km * km;
+  }
+  if (std::sqrt(kmax2) > h->dk * h->ntab) {
+    set_error("insufficient maximal T tabulated");
+    return RISM_ERR_RANGE;
+  }
+  const size_t VN = (size_t)h->V * h->N;
+  const size_t nh = (size_t)h->nx * (h->ny / 2) * h->nz;
+  int rc = 0;
+  if ((rc = dalloc_t(h, &h->ulj, VN)) || (rc = dalloc_t(h, &h->de, h->N)) || (rc = dalloc_t(h, &h->fr, h->N)) ||
+      (rc = dalloc_t(h, &h->sr, h->N)) || (rc = dalloc_t(h, &h->fkh, nh)) ||
+      (rc = dalloc_t(h, &h->phase, (size_t)std::max(1, h->natu) * (h->nx + h->ny + h->nz))) ||
+      (rc = dalloc_t(h, &h->ts, VN)) || (rc = dalloc_t(h, &h->res, VN)) ||
+      (rc = dalloc_t(h, &h->hist_t, (size_t)o->nhist * VN)) || (rc = dalloc_t(h, &h->hist_r, (size_t)o->nhist * VN)) ||
+      (rc = dalloc_t(h, &h->gfield, VN)) || (rc = dalloc_t(h, &h->ctl, 1)) ||
+      (rc = dalloc_t(h, &h->rms_hist, (size_t)(o->maxstep + 1) * (o->chargeup + 1) + 4)) ||
+      (rc = dalloc_t(h, &h->atoms, std::max(1, h->natu))) || (rc = dalloc_t(h, &h->lj, (size_t)h->V * std::max(1, h->natu))) ||
+      (rc = dalloc_t(h, &h->sigh, (size_t)h->V * std::max(1, h->natu))) || (rc = dalloc_t(h, &h->d_qv, h->V)) ||
+      (rc = dalloc_t(h, &h->d_rho, h->V)))
+    return rc;
+  const size_t npart = std::max<size_t>((size_t)1024 * 4 * kMaxV, (size_t)1024 * 66);
+  if ((rc = dalloc_t(h, &h->partials, npart))) return rc;
+  HIP_OK(hipHostMalloc((void **)&h->ctl_host, sizeof(Ctl), hipHostMallocDefault));
+  std::memset(h->ctl_host, 0, sizeof(Ctl));
+  rc = (o->precision == RISM_PREC_MIXED) ? alloc_precision<float>(h) : alloc_precision<double>(h);
+  if (rc) return rc;
+  // solute records and Lorentz-Berthelot pair parameters (cal_LJ.cu:24-31)
+  std::vector<double4> at(std::max(1, h->natu));
+  std::vector<double2> lj((size_t)h->V * std::max(1, h->natu));
+  std::vector<double> sigh((size_t)h->V * std::max(1, h->natu));
+  for (int u = 0; u < h->natu; ++u) at[u] = make_double4(h->uxyz[3 * u], h->uxyz[3 * u + 1], h->uxyz[3 * u + 2], h->uq[u]);
+  for (int v = 0; v < h->V; ++v)
+    for (int u = 0; u < h->natu; ++u) {
+      const double s = (h->usig[u] + h->sigv[v]) * 0.5;
+      const double e = std::sqrt(h->ueps[u] * h->epsv[v]);
+      lj[(size_t)v * h->natu + u] = make_double2(s * s, e * 4.0);
+      sigh[(size_t)v * h->natu + u] = s * 0.5;
+    }
+  HIP_OK(hipMemcpy(h->atoms, at.data(), at.size() * sizeof(double4), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(h->lj, lj.data(), lj.size() * sizeof(double2), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(h->sigh, sigh.data(), sigh.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(h->d_qv, h->qv.data(), h->V * sizeof(double), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(h->d_rho, h->rho.data(), h->V * sizeof(double), hipMemcpyHostToDevice));
+  h->t_setup = now_s() - t0;
+  return RISM_OK;
+}
+
+}  // namespace
+
+// =====================================================================
+// C ABI
+// =====================================================================
+extern "C" {
+
+const char *rism_last_error(void) { return rism::last_error(); }
+const char *rism_version(void) { return "rism3d_amd 0.1 (gfx950)"; }
+
+int rism_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int rism_create(const rism_grid *grid, const rism_solvent *solvent, const rism_solute *solute,
+                const rism_opts *opts, int device, rism_handle **out) {
+  if (!grid || !solvent || !solute || !opts || !out) {
+    set_error("null argument");
+    return RISM_ERR_ARG;
+  }
+  *out = nullptr;
+  rism_handle *h = new rism_handle();
+  int rc = create_impl(grid, solvent, solute, opts, device, h);
+  if (rc != RISM_OK) {
+    std::string msg = rism::last_error();
+    rism_destroy(h);
+    set_error(msg);
+    return rc;
+  }
+  *out = h;
+  return RISM_OK;
+}
+
+int rism_build_potentials(rism_handle *h) {
+  if (!h) { set_error("null handle"); return RISM_ERR_ARG; }
+  const double t0 = now_s();
+  HIP_OK(hipSetDevice(h->device));
+  PotArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.nx = h->nx; a.ny = h->ny; a.nz = h->nz; a.N = h->N;
+  a.V = h->V; a.natu = h->natu;
+  for (int k = 0; k < 3; ++k) { a.dr[k] = h->dr[k]; a.box[k] = h->box[k]; }
+  a.atoms = h->atoms;
+  a.lj = h->lj;
+  a.ikbt = 1.0 / (kAvogadro * kBoltzmann * h->temper);
+  a.ubeta = kHartree * kBohr / (kBoltzmann * h->temper);
+  a.ulj = h->ulj; a.de = h->de; a.fr = h->fr; a.sr = h->sr;
+  HIP_OK(launch_potentials(a, h->stream));
+  HIP_OK(launch_fk_half(h->nx, h->ny, h->nz, h->natu, h->atoms, h->box, a.ubeta, h->phase, h->fkh, h->stream));
+  if (h->opts.precision == RISM_PREC_MIXED) {
+    HIP_OK(launch_convert<float>(h->fr, (float *)h->frR, h->N, h->stream));
+    HIP_OK(launch_convert_c<float>(h->fkh, (float2 *)h->fkhR, (long)h->nx * (h->ny / 2) * h->nz, h->stream));
+  }
+  HIP_OK(hipStreamSynchronize(h->stream));
+  h->potentials = true;
+  h->solved = h->evaluated = false;
+  h->t_setup += now_s() - t0;
+  return RISM_OK;
+}
+
+int rism_solve(rism_handle *h, const char *tuv_path, int *steps, double *rms) {
+  if (!h) { set_error("null handle"); return RISM_ERR_ARG; }
+  if (!h->potentials) { set_error("rism_build_potentials must run before rism_solve"); return RISM_ERR_STATE; }
+  HIP_OK(hipSetDevice(h->device));
+  return h->opts.precision == RISM_PREC_MIXED ? solve_impl<float>(h, tuv_path, steps, rms)
+                                              : solve_impl<double>(h, tuv_path, steps, rms);
+}
+
+int rism_thermo(rism_handle *h, rism_xmu *out) {
+  if (!h || !out) { set_error("null argument"); return RISM_ERR_ARG; }
+  if (!h->solved) { set_error("rism_solve must run before rism_thermo"); return RISM_ERR_STATE; }
+  HIP_OK(hipSetDevice(h->device));
+  const double t0 = now_s();
+  if (!h->evaluated)
+    if (int rc = evaluate(h)) return rc;
+  int rc = h->opts.precision == RISM_PREC_MIXED ? thermo_impl<float>(h, out) : thermo_impl<double>(h, out);
+  h->t_thermo = now_s() - t0;
+  return rc;
+}
+
+int rism_grad(rism_handle *h, double *out) {
+  if (!h || !out) { set_error("null argument"); return RISM_ERR_ARG; }
+  if (!h->solved) { set_error("rism_solve must run before rism_grad"); return RISM_ERR_STATE; }
+  HIP_OK(hipSetDevice(h->device));
+  if (!h->evaluated)
+    if (int rc = evaluate(h)) return rc;
+  const int nu = h->natu;
+  if (nu == 0) return RISM_OK;
+  const int nblk = (int)((h->N + (long)kThreads * 16 - 1) / ((long)kThreads * 16));
+  double *part = nullptr, *dout = nullptr;
+  HIP_OK(hipMalloc((void **)&part, (size_t)nblk * nu * 6 * sizeof(double)));
+  HIP_OK(hipMalloc((void **)&dout, (size_t)nu * 6 * sizeof(double)));
+  hipError_t e = launch_grad(h->nx, h->ny, h->nz, h->dr, h->V, nu, h->atoms, h->lj, h->sigh, h->d_qv, h->d_rho,
+                             h->gfield, part, nblk, dout, h->stream);
+  std::vector<double> g((size_t)nu * 6);
+  if (e == hipSuccess) e = hipMemcpyAsync(g.data(), dout, g.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  (void)hipFree(part);
+  (void)hipFree(dout);
+  if (e != hipSuccess) {
+    set_error(std::string("HIP error in rism_grad: ") + hipGetErrorString(e));
+    return RISM_ERR_HIP;
+  }
+  // kernel layout per atom: LJ xyz then ES xyz, already x dv (output_grad.cu:15-24)
+  std::memcpy(out, g.data(), g.size() * sizeof(double));
+  return RISM_OK;
+}
+
+int rism_get_field(rism_handle *h, int which, double *out) {
+  if (!h || !out) { set_error("null argument"); return RISM_ERR_ARG; }
+  HIP_OK(hipSetDevice(h->device));
+  HIP_OK(hipStreamSynchronize(h->stream));
+  const long N = h->N;
+  const int V = h->V;
+  const size_t VN = (size_t)V * N;
+  switch (which) {
+    case RISM_FIELD_T: return fetch_t(h, out);
+    case RISM_FIELD_RES: HIP_OK(hipMemcpy(out, h->res, VN * sizeof(double), hipMemcpyDeviceToHost)); return RISM_OK;
+    case RISM_FIELD_ULJ: HIP_OK(hipMemcpy(out, h->ulj, VN * sizeof(double), hipMemcpyDeviceToHost)); return RISM_OK;
+    case RISM_FIELD_DE: HIP_OK(hipMemcpy(out, h->de, N * sizeof(double), hipMemcpyDeviceToHost)); return RISM_OK;
+    case RISM_FIELD_FR: HIP_OK(hipMemcpy(out, h->fr, N * sizeof(double), hipMemcpyDeviceToHost)); return RISM_OK;
+    case RISM_FIELD_FK: {
+      const int nx = h->nx, ny = h->ny, nz = h->nz, nyh = ny / 2;
+      std::vector<double2> half((size_t)nx * nyh * nz);
+      HIP_OK(hipMemcpy(half.data(), h->fkh, half.size() * sizeof(double2), hipMemcpyDeviceToHost));
+      for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+          for (int x = 0; x < nx; ++x) {
+            const size_t ip = ((size_t)z * ny + y) * nx + x;
+            double2 v;
+            if (y < nyh) {
+              v = half[((size_t)z * nyh + y) * nx + x];
+            } else {  // fk(-k) = conj(fk(k))
+              const double2 m = half[((size_t)(nz - 1 - z) * nyh + (ny - 1 - y)) * nx + (nx - 1 - x)];
+              v = make_double2(m.x, -m.y);
+            }
+            out[2 * ip] = v.x;
+            out[2 * ip + 1] = v.y;
+          }
+      return RISM_OK;
+    }
+    case RISM_FIELD_CHI: {
+      const int nt = h->ntab;
+      std::vector<double> kt(nt), y(nt), c0(nt), c1(nt), c2(nt);
+      for (int i = 0; i < nt; ++i) kt[i] = h->dk * (i + 1);
+      for (int v = 0; v < V; ++v)
+        for (int v1 = 0; v1 < V; ++v1) {
+          for (int i = 0; i < nt; ++i) y[i] = h->chi[((size_t)i * V + v) * V + v1];
+          natural_spline(kt.data(), y.data(), nt, c0.data(), c1.data(), c2.data());
+          double *o = out + (size_t)(v * V + v1) * N;
+          for (long ip = 0; ip < N; ++ip) {
+            const int x = (int)(ip % h->nx), yy = (int)((ip / h->nx) % h->ny), z = (int)(ip / ((long)h->nx * h->ny));
+            const double kx = 2.0 * M_PI * ((double)x - h->nx / 2.0 + 0.5) / h->box[0];
+            const double ky = 2.0 * M_PI * ((double)yy - h->ny / 2.0 + 0.5) / h->box[1];
+            const double kz = 2.0 * M_PI * ((double)z - h->nz / 2.0 + 0.5) / h->box[2];
+            const double kk = std::sqrt(kx * kx + ky * ky + kz * kz);
+            const int lo = table_interval(kk, h->dk, nt);
+            const double t = kk - h->dk * (lo + 1);
+            o[ip] = ((c2[lo] * t + c1[lo]) * t + c0[lo]) * t + y[lo];
+          }
+        }
+      return RISM_OK;
+    }
+    case RISM_FIELD_G:
+    case RISM_FIELD_H:
+    case RISM_FIELD_C: {
+      if (!h->solved) { set_error("fields g/h/c need a solve"); return RISM_ERR_STATE; }
+      if (!h->evaluated)
+        if (int rc = evaluate(h)) return rc;
+      if (which == RISM_FIELD_G) {
+        HIP_OK(hipMemcpy(out, h->gfield, VN * sizeof(double), hipMemcpyDeviceToHost));
+        return RISM_OK;
+      }
+      std::vector<double> hh(VN);
+      if (h->opts.precision == RISM_PREC_MIXED) {
+        std::vector<float2> z((size_t)h->P * N);
+        HIP_OK(hipMemcpy(z.data(), h->Z, z.size() * sizeof(float2), hipMemcpyDeviceToHost));
+        for (int v = 0; v < V; ++v)
+          for (long i = 0; i < N; ++i) hh[(size_t)v * N + i] = (v & 1) ? z[(size_t)(v >> 1) * N + i].y : z[(size_t)(v >> 1) * N + i].x;
+      } else {
+        std::vector<double2> z((size_t)h->P * N);
+        HIP_OK(hipMemcpy(z.data(), h->Z, z.size() * sizeof(double2), hipMemcpyDeviceToHost));
+        for (int v = 0; v < V; ++v)
+          for (long i = 0; i < N; ++i) hh[(size_t)v * N + i] = (v & 1) ? z[(size_t)(v >> 1) * N + i].y : z[(size_t)(v >> 1) * N + i].x;
+      }
+      if (which == RISM_FIELD_H) {
+        std::memcpy(out, hh.data(), VN * sizeof(double));
+        return RISM_OK;
+      }
+      if (int rc = fetch_t(h, out)) return rc;
+      for (size_t i = 0; i < VN; ++i) out[i] = hh[i] - out[i];
+      return RISM_OK;
+    }
+    default:
+      set_error("unknown field");
+      return RISM_ERR_ARG;
+  }
+}
+
+int rism_set_t(rism_handle *h, const double *t) {
+  if (!h || !t) { set_error("null argument"); return RISM_ERR_ARG; }
+  if (!h->potentials) { set_error("potentials needed before rism_set_t"); return RISM_ERR_STATE; }
+  HIP_OK(hipSetDevice(h->device));
+  h->evaluated = false;
+  return store_t(h, t);
+}
+
+int rism_save_tuv(rism_handle *h, const char *path) {
+  if (!h || !path) { set_error("null argument"); return RISM_ERR_ARG; }
+  HIP_OK(hipSetDevice(h->device));
+  return write_tuv_file(h, path);
+}
+
+int rism_load_tuv(rism_handle *h, const char *path) {
+  if (!h || !path) { set_error("null argument"); return RISM_ERR_ARG; }
+  if (!h->potentials) { set_error("potentials needed before rism_load_tuv"); return RISM_ERR_STATE; }
+  HIP_OK(hipSetDevice(h->device));
+  h->cf = 1.0;
+  h->evaluated = false;
+  return read_tuv_file(h, path);
+}
+
+int rism_stats(rism_handle *h, int *steps, double *rms, double *t_setup, double *t_solve, double *t_thermo) {
+  if (!h) { set_error("null handle"); return RISM_ERR_ARG; }
+  if (steps) *steps = h->steps;
+  if (rms) *rms = h->rms;
+  if (t_setup) *t_setup = h->t_setup;
+  if (t_solve) *t_solve = h->t_solve;
+  if (t_thermo) *t_thermo = h->t_thermo;
+  return RISM_OK;
+}
+
+int rism_rms_history(rism_handle *h, double *out, int n) {
+  if (!h) { set_error("null handle"); return RISM_ERR_ARG; }
+  const int k = std::min(n, (int)h->rms_log.size());
+  for (int i = 0; i < k; ++i) out[i] = h->rms_log[i];
+  return (int)h->rms_log.size();
+}
+
+void rism_destroy(rism_handle *h) {
+  if (!h) return;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (auto &b : h->bufs) (void)hipFree(b.p);
+  if (h->ctl_host) (void)hipHostFree(h->ctl_host);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+int rism_fft3d(rism_handle *h, void *data, int dir) {
+  if (!h || !data || (dir != -1 && dir != 1)) { set_error("bad argument"); return RISM_ERR_ARG; }
+  HIP_OK(hipSetDevice(h->device));
+  hipError_t e;
+  if (h->opts.precision == RISM_PREC_MIXED)
+    e = launch_fft3d<float>((float2 *)data, h->plan[0], h->plan[1], h->plan[2], h->geom, h->nx, h->ny, h->nz,
+                            h->dv, 1.0 / h->volume, dir, h->stream);
+  else
+    e = launch_fft3d<double>((double2 *)data, h->plan[0], h->plan[1], h->plan[2], h->geom, h->nx, h->ny, h->nz,
+                             h->dv, 1.0 / h->volume, dir, h->stream);
+  HIP_OK(e);
+  HIP_OK(hipStreamSynchronize(h->stream));
+  return RISM_OK;
+}
+
+int rism_calculate(rism_handle *h, const double *t_dev, double *res_dev) {
+  if (!h || !t_dev || !res_dev) { set_error("null argument"); return RISM_ERR_ARG; }
+  if (!h->potentials) { set_error("potentials needed before rism_calculate"); return RISM_ERR_STATE; }
+  HIP_OK(hipSetDevice(h->device));
+  const size_t VN = (size_t)h->V * h->N;
+  h->cf = 1.0;
+  // ts = t - q_v fr
+  HIP_OK(hipMemcpyAsync(h->ts, t_dev, VN * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+  std::vector<double> coef(h->V);
+  for (int v = 0; v < h->V; ++v) coef[v] = -h->qv[v];
+  HIP_OK(launch_axpy_fr<double>(h->ts, h->fr, coef.data(), h->V, h->N, 1, h->stream));
+  int rc = (h->opts.precision == RISM_PREC_MIXED) ? stage_potential<float>(h) : stage_potential<double>(h);
+  if (rc) return rc;
+  Ctl &c = *h->ctl_host;
+  std::memset(&c, 0, sizeof c);
+  c.nhist = h->opts.nhist;
+  c.maxstep = 1;
+  c.npts = (double)VN;
+  c.status = -1;
+  const int it = h->iter++;
+  c.run_iter = it;
+  c.upd_iter = -1;
+  if ((rc = put_ctl(h))) return rc;
+  rc = (h->opts.precision == RISM_PREC_MIXED) ? launch_iteration<float>(h, it, true, false)
+                                              : launch_iteration<double>(h, it, true, false);
+  if (rc) return rc;
+  HIP_OK(hipMemcpyAsync(res_dev, h->res, VN * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+  HIP_OK(hipStreamSynchronize(h->stream));
+  h->solved = true;
+  h->evaluated = false;
+  return RISM_OK;
+}
+
+}  // extern "C"

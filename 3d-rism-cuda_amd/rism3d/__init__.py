@@ -1,0 +1,356 @@
+"""rism3d -- Python view of librism3d_amd.so (MI355X / gfx950 3D-RISM).
+
+A thin ctypes layer over the C ABI declared in include/rism3d_amd.h.  It
+mirrors the reference's RISM3D life cycle (src-cuda/rism3d.h:25-27:
+initialize -> iterate -> output) as :class:`Solver` (create + potentials ->
+solve -> thermo / grad / fields), plus the .inp/.xvk deck reader.  There is
+no fallback: if the shared library is missing, or no GPU is present when a
+GPU call is made, a RuntimeError is raised.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "_build", "librism3d_amd.so")
+
+# status codes (include/rism3d_amd.h)
+OK = 0
+SOLVE_CONVERGED, SOLVE_DIVERGED, SOLVE_MAXSTEP = 0, 1, 2
+ERR_ARG, ERR_IO, ERR_FORMAT, ERR_VERSION, ERR_RANGE, ERR_HIP, ERR_NOMEM, ERR_STATE = (
+    -1, -2, -3, -4, -5, -6, -7, -8)
+KH, HNC = 0, 1
+FP64, MIXED = 0, 1
+FIELD_T, FIELD_G, FIELD_H, FIELD_C, FIELD_RES, FIELD_ULJ, FIELD_DE, FIELD_FR, FIELD_FK, FIELD_CHI = range(10)
+
+EXPORTED = [
+    "rism_last_error", "rism_version", "rism_device_count", "rism_deck_read", "rism_deck_free",
+    "rism_deck_views", "rism_xvk_read", "rism_create", "rism_build_potentials", "rism_solve",
+    "rism_thermo", "rism_grad", "rism_get_field", "rism_set_t", "rism_save_tuv", "rism_load_tuv",
+    "rism_write_xmu", "rism_write_gra", "rism_stats", "rism_rms_history", "rism_destroy",
+    "rism_fft3d", "rism_calculate",
+]
+
+
+class Grid(C.Structure):
+    _fields_ = [("n", C.c_int * 3), ("box", C.c_double * 3)]
+
+
+class SolventC(C.Structure):
+    _fields_ = [("nsite", C.c_int), ("ntab", C.c_int), ("dk", C.c_double), ("temper", C.c_double),
+                ("xt", C.c_double), ("rho", C.POINTER(C.c_double)), ("q", C.POINTER(C.c_double)),
+                ("sig", C.POINTER(C.c_double)), ("eps", C.POINTER(C.c_double)),
+                ("chi", C.POINTER(C.c_double))]
+
+
+class SoluteC(C.Structure):
+    _fields_ = [("natom", C.c_int), ("sig", C.POINTER(C.c_double)), ("eps", C.POINTER(C.c_double)),
+                ("q", C.POINTER(C.c_double)), ("xyz", C.POINTER(C.c_double))]
+
+
+class Opts(C.Structure):
+    _fields_ = [("closure", C.c_int), ("tol", C.c_double), ("maxstep", C.c_int), ("count", C.c_int),
+                ("m", C.c_double), ("mp", C.c_double), ("nhist", C.c_int), ("precision", C.c_int),
+                ("chargeup", C.c_int), ("verbose", C.c_int), ("ksave", C.c_int)]
+
+
+class Xmu(C.Structure):
+    _fields_ = [("nsite", C.c_int), ("sfe_sc", C.c_double), ("sfe_sc_hnc", C.c_double),
+                ("sfe_gf", C.c_double), ("sfec_sc", C.c_double * 16), ("sfec_sc_hnc", C.c_double * 16),
+                ("sfec_gf", C.c_double * 16), ("se", C.c_double), ("se_lj", C.c_double),
+                ("se_es", C.c_double), ("pmv", C.c_double), ("pressure", C.c_double),
+                ("correction", C.c_double)]
+
+    def as_dict(self):
+        v = self.nsite
+        return {
+            "SFE_SC": self.sfe_sc, "SFE_SC_HNC": self.sfe_sc_hnc, "SFE_GF": self.sfe_gf,
+            "SFEC_SC": list(self.sfec_sc[:v]), "SFEC_SC_HNC": list(self.sfec_sc_hnc[:v]),
+            "SFEC_GF": list(self.sfec_gf[:v]), "SE": self.se, "SE_LJ": self.se_lj, "SE_ES": self.se_es,
+            "PMV": self.pmv, "Pressure": self.pressure, "Correction_Term": self.correction,
+        }
+
+
+class DeckC(C.Structure):
+    _fields_ = [("outlist", C.c_char * 64), ("ksave", C.c_int), ("solvent_path", C.c_char * 4096),
+                ("fname", C.c_char * 4096), ("grid", Grid), ("opts", Opts), ("natom", C.c_int),
+                ("sig", C.POINTER(C.c_double)), ("eps", C.POINTER(C.c_double)),
+                ("q", C.POINTER(C.c_double)), ("xyz", C.POINTER(C.c_double)), ("shift", C.c_double * 3),
+                ("nsite", C.c_int), ("ntab", C.c_int), ("dk", C.c_double), ("temper", C.c_double),
+                ("xt", C.c_double), ("rho", C.POINTER(C.c_double)), ("sq", C.POINTER(C.c_double)),
+                ("ssig", C.POINTER(C.c_double)), ("seps", C.POINTER(C.c_double)),
+                ("chi", C.POINTER(C.c_double))]
+
+
+_lib = None
+
+
+def lib():
+    """Load librism3d_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} not built: run __graft_entry__.build() or make -C 3d-rism-cuda_amd")
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        dp = P(C.c_double)
+        L.rism_last_error.restype = C.c_char_p
+        L.rism_version.restype = C.c_char_p
+        L.rism_device_count.restype = C.c_int
+        L.rism_deck_read.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, P(P(DeckC))]
+        L.rism_deck_free.argtypes = [P(DeckC)]
+        L.rism_deck_views.argtypes = [P(DeckC), P(Grid), P(SolventC), P(SoluteC), P(Opts)]
+        L.rism_xvk_read.argtypes = [C.c_char_p, P(DeckC)]
+        L.rism_create.argtypes = [P(Grid), P(SolventC), P(SoluteC), P(Opts), C.c_int, P(C.c_void_p)]
+        L.rism_build_potentials.argtypes = [C.c_void_p]
+        L.rism_solve.argtypes = [C.c_void_p, C.c_char_p, P(C.c_int), dp]
+        L.rism_thermo.argtypes = [C.c_void_p, P(Xmu)]
+        L.rism_grad.argtypes = [C.c_void_p, dp]
+        L.rism_get_field.argtypes = [C.c_void_p, C.c_int, dp]
+        L.rism_set_t.argtypes = [C.c_void_p, dp]
+        L.rism_save_tuv.argtypes = [C.c_void_p, C.c_char_p]
+        L.rism_load_tuv.argtypes = [C.c_void_p, C.c_char_p]
+        L.rism_write_xmu.argtypes = [P(Xmu), C.c_int, C.c_char_p]
+        L.rism_write_gra.argtypes = [dp, C.c_int, C.c_char_p]
+        L.rism_stats.argtypes = [C.c_void_p, P(C.c_int), dp, dp, dp, dp]
+        L.rism_rms_history.argtypes = [C.c_void_p, dp, C.c_int]
+        L.rism_destroy.argtypes = [C.c_void_p]
+        L.rism_destroy.restype = None
+        L.rism_fft3d.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+        L.rism_calculate.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return lib().rism_last_error().decode()
+
+
+def _check(rc: int, what: str):
+    if rc < 0:
+        raise RuntimeError(f"{what} failed ({rc}): {last_error()}")
+    return rc
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _arr(p, n):
+    return np.ctypeslib.as_array(p, shape=(n,)).copy() if n > 0 else np.zeros(0)
+
+
+@dataclass
+class Solvent:
+    """Reduced solvent model as read from a .xvk file (solvent_read.cc:10-86)."""
+    rho: np.ndarray
+    q: np.ndarray
+    sig: np.ndarray
+    eps: np.ndarray
+    chi: np.ndarray  # [ntab][V][V]
+    dk: float
+    temper: float
+    xt: float
+
+    @property
+    def nsite(self):
+        return len(self.rho)
+
+
+@dataclass
+class Solute:
+    sig: np.ndarray
+    eps: np.ndarray
+    q: np.ndarray
+    xyz: np.ndarray  # [n][3]
+
+
+@dataclass
+class Options:
+    closure: int = KH
+    tol: float = 1e-6
+    maxstep: int = 10000
+    count: int = 2
+    m: float = 0.6
+    mp: float = 0.2
+    nhist: int = 2
+    precision: int = FP64
+    chargeup: int = 0
+    verbose: int = 0
+    ksave: int = 0
+
+
+@dataclass
+class Deck:
+    """A parsed .inp deck with its solvent (read_input.cu:21-89)."""
+    outlist: str
+    ksave: int
+    fname: str
+    n: tuple
+    box: tuple
+    opts: Options
+    solute: Solute
+    solvent: Solvent
+    shift: tuple = field(default=(0.0, 0.0, 0.0))
+
+
+def read_deck(control: str, structure: Optional[str] = None, centering: bool = True,
+              zero: bool = False) -> Deck:
+    L = lib()
+    p = C.POINTER(DeckC)()
+    _check(L.rism_deck_read(control.encode(), structure.encode() if structure else None,
+                            int(centering), int(zero), C.byref(p)), "rism_deck_read")
+    d = p.contents
+    try:
+        V, nt, n = d.nsite, d.ntab, d.natom
+        o = d.opts
+        opts = Options(o.closure, o.tol, o.maxstep, o.count, o.m, o.mp, 2, FP64, 0, 0, d.ksave)
+        su = Solute(_arr(d.sig, n), _arr(d.eps, n), _arr(d.q, n), _arr(d.xyz, 3 * n).reshape(n, 3))
+        sv = Solvent(_arr(d.rho, V), _arr(d.sq, V), _arr(d.ssig, V), _arr(d.seps, V),
+                     _arr(d.chi, nt * V * V).reshape(nt, V, V), d.dk, d.temper, d.xt)
+        return Deck(d.outlist.decode(), d.ksave, d.fname.decode(), tuple(d.grid.n), tuple(d.grid.box),
+                    opts, su, sv, tuple(d.shift))
+    finally:
+        L.rism_deck_free(p)
+
+
+def read_xvk(path: str) -> Solvent:
+    """Read a .xvk solvent file (solvent_read.cc:10-86)."""
+    L = lib()
+    d = DeckC()
+    try:
+        _check(L.rism_xvk_read(path.encode(), C.byref(d)), "rism_xvk_read")
+        V, nt = d.nsite, d.ntab
+        return Solvent(_arr(d.rho, V), _arr(d.sq, V), _arr(d.ssig, V), _arr(d.seps, V),
+                       _arr(d.chi, nt * V * V).reshape(nt, V, V), d.dk, d.temper, d.xt)
+    finally:
+        libc = C.CDLL(None)
+        libc.free.argtypes = [C.c_void_p]
+        for name in ("rho", "sq", "ssig", "seps", "chi"):
+            p = getattr(d, name)
+            if p:
+                libc.free(C.cast(p, C.c_void_p))
+
+
+class Solver:
+    """One 3D-RISM solve on one GPU (handle over the C ABI)."""
+
+    def __init__(self, n, box, solvent: Solvent, solute: Solute, opts: Options = Options(),
+                 device: int = 0):
+        L = lib()
+        self._keep = []
+        g = Grid((C.c_int * 3)(*n), (C.c_double * 3)(*box))
+
+        def keep(a):
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            self._keep.append(a)
+            return _dptr(a)
+
+        sv = SolventC(solvent.nsite, solvent.chi.shape[0], solvent.dk, solvent.temper, solvent.xt,
+                      keep(solvent.rho), keep(solvent.q), keep(solvent.sig), keep(solvent.eps),
+                      keep(solvent.chi))
+        su = SoluteC(len(solute.q), keep(solute.sig), keep(solute.eps), keep(solute.q),
+                     keep(np.asarray(solute.xyz).reshape(-1)))
+        o = Opts(opts.closure, opts.tol, opts.maxstep, opts.count, opts.m, opts.mp, opts.nhist,
+                 opts.precision, opts.chargeup, opts.verbose, opts.ksave)
+        h = C.c_void_p()
+        _check(L.rism_create(C.byref(g), C.byref(sv), C.byref(su), C.byref(o), device, C.byref(h)),
+               "rism_create")
+        self.h = h
+        self.n = tuple(n)
+        self.N = int(np.prod(n))
+        self.V = solvent.nsite
+        self.natom = len(solute.q)
+        self.opts = opts
+
+    @classmethod
+    def from_deck(cls, deck: Deck, **over):
+        o = Options(**{**deck.opts.__dict__, **over})
+        return cls(deck.n, deck.box, deck.solvent, deck.solute, o)
+
+    def build_potentials(self):
+        _check(lib().rism_build_potentials(self.h), "rism_build_potentials")
+        return self
+
+    def solve(self, tuv_path: Optional[str] = None):
+        steps, rms = C.c_int(), C.c_double()
+        st = _check(lib().rism_solve(self.h, tuv_path.encode() if tuv_path else None, C.byref(steps),
+                                     C.byref(rms)), "rism_solve")
+        return st, steps.value, rms.value
+
+    def thermo(self) -> dict:
+        x = Xmu()
+        _check(lib().rism_thermo(self.h, C.byref(x)), "rism_thermo")
+        self._xmu = x
+        return x.as_dict()
+
+    def write_xmu(self, path: str):
+        _check(lib().rism_write_xmu(C.byref(self._xmu), self.opts.closure, path.encode()), "rism_write_xmu")
+
+    def grad(self) -> np.ndarray:
+        out = np.zeros(self.natom * 6)
+        _check(lib().rism_grad(self.h, _dptr(out)), "rism_grad")
+        return out.reshape(self.natom, 6)
+
+    def field(self, which: int) -> np.ndarray:
+        if which == FIELD_FK:
+            out = np.zeros(2 * self.N)
+            _check(lib().rism_get_field(self.h, which, _dptr(out)), "rism_get_field")
+            return (out[0::2] + 1j * out[1::2]).reshape(self.n[::-1])
+        size = {FIELD_DE: self.N, FIELD_FR: self.N, FIELD_CHI: self.V * self.V * self.N}.get(
+            which, self.V * self.N)
+        out = np.zeros(size)
+        _check(lib().rism_get_field(self.h, which, _dptr(out)), "rism_get_field")
+        shape = self.n[::-1]
+        if which in (FIELD_DE, FIELD_FR):
+            return out.reshape(shape)
+        if which == FIELD_CHI:
+            return out.reshape((self.V, self.V) + shape)
+        return out.reshape((self.V,) + shape)
+
+    def set_t(self, t: np.ndarray):
+        t = np.ascontiguousarray(t, dtype=np.float64).reshape(-1)
+        _check(lib().rism_set_t(self.h, _dptr(t)), "rism_set_t")
+
+    def save_tuv(self, path):
+        _check(lib().rism_save_tuv(self.h, path.encode()), "rism_save_tuv")
+
+    def load_tuv(self, path):
+        _check(lib().rism_load_tuv(self.h, path.encode()), "rism_load_tuv")
+
+    def stats(self) -> dict:
+        s, r, a, b, c = C.c_int(), C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        _check(lib().rism_stats(self.h, C.byref(s), C.byref(r), C.byref(a), C.byref(b), C.byref(c)),
+               "rism_stats")
+        return {"steps": s.value, "rms": r.value, "t_setup": a.value, "t_solve": b.value,
+                "t_thermo": c.value}
+
+    def rms_history(self) -> np.ndarray:
+        n = lib().rism_rms_history(self.h, None, 0)
+        out = np.zeros(max(n, 1))
+        lib().rism_rms_history(self.h, _dptr(out), n)
+        return out[:n]
+
+    def fft3d(self, data_ptr: int, direction: int):
+        """In-place transform of caller device memory (FFT3D::execute seam)."""
+        _check(lib().rism_fft3d(self.h, C.c_void_p(data_ptr), int(direction)), "rism_fft3d")
+
+    def calculate(self, t_ptr: int, res_ptr: int):
+        """One fixed-point map t -> R on caller device memory (calculate.cu)."""
+        _check(lib().rism_calculate(self.h, C.c_void_p(t_ptr), C.c_void_p(res_ptr)), "rism_calculate")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rism_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,208 @@
+/*
+ * rism3d_amd.h -- C ABI of the MI355X (gfx950) 3D-RISM calculate/iterate
+ * library (librism3d_amd.so).
+ *
+ * The reference (rismical-dev/3D-RISM-CUDA) has no library API: its seam is
+ * the class RISM3D { initialize, iterate, output } (src-cuda/rism3d.h:25-27)
+ * driven by the CLI in src-cuda/main.cu.  Each entry point below names the
+ * reference routine it replaces.  Conventions:
+ *   - plain C types only; host inputs are copied in, the handle owns all
+ *     device memory;
+ *   - every call returns an int status (RISM_OK == 0 or a RISM_ERR_* code;
+ *     rism_solve returns a RISM_SOLVE_* outcome) and never exits the process
+ *     (the reference prints and calls exit(1), e.g. read_input.cu:31-32);
+ *   - rism_last_error() gives a thread-local message for the last failure;
+ *   - one handle is used from one host thread; calls on one handle are
+ *     ordered on the handle's HIP stream.
+ * Units follow the reference: lengths in Angstrom, LJ epsilon in J/mol,
+ * charges in e, densities in 1/Angstrom^3, energies out in J/mol.
+ */
+#ifndef RISM3D_AMD_H
+#define RISM3D_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define RISM_OK 0
+#define RISM_SOLVE_CONVERGED 0 /* iterate.cu:58-59 */
+#define RISM_SOLVE_DIVERGED 1  /* iterate.cu:54-57 "Calculation diverged." */
+#define RISM_SOLVE_MAXSTEP 2   /* iterate.cu:76-79 "reached limit" */
+#define RISM_ERR_ARG -1        /* bad argument / unsupported size */
+#define RISM_ERR_IO -2         /* file missing or unreadable */
+#define RISM_ERR_FORMAT -3     /* malformed .inp/.xvk/.tuv content */
+#define RISM_ERR_VERSION -4    /* .inp version tag != 20220301 (read_input.cu:30-33) */
+#define RISM_ERR_RANGE -5      /* |k| beyond the chi table (solvent_spline.cu:18-21) */
+#define RISM_ERR_HIP -6        /* HIP runtime failure (message has details) */
+#define RISM_ERR_NOMEM -7      /* device allocation failed */
+#define RISM_ERR_STATE -8      /* call out of order (e.g. thermo before solve) */
+
+/* closure switch (read_input.cu:35-44) */
+#define RISM_CLOSURE_KH 0
+#define RISM_CLOSURE_HNC 1
+
+/* arithmetic of the iteration */
+#define RISM_PREC_FP64 0  /* everything FP64 (reference arithmetic) */
+#define RISM_PREC_MIXED 1 /* FP32 FFT/closure/convolution, FP64 state,
+                             Anderson dot products, solve and free-energy sums */
+
+/* fields for rism_get_field */
+#define RISM_FIELD_T 0   /* t = h - c, [V][N] (iterate.cu dt) */
+#define RISM_FIELD_G 1   /* g, [V][N] (dguv.x) */
+#define RISM_FIELD_H 2   /* h, [V][N] (dhuv.x) */
+#define RISM_FIELD_C 3   /* c = h - t, [V][N] */
+#define RISM_FIELD_RES 4 /* last residual R = h + 1 - g, [V][N] (dtr) */
+#define RISM_FIELD_ULJ 5 /* beta*u_LJ, [V][N] (du, cal_LJ.cu:47-72) */
+#define RISM_FIELD_DE 6  /* beta*phi per unit charge, [N] (de) */
+#define RISM_FIELD_FR 7  /* long-range real-space part, [N] (dfr) */
+#define RISM_FIELD_FK 8  /* long-range k-space part, interleaved complex [N] (dfk) */
+#define RISM_FIELD_CHI 9 /* chi_vv'(|k|) on the grid, [V][V][N] (sv->dx) */
+
+typedef struct rism_handle rism_handle;
+
+/* grid/cell: the .inp box and grid lines (read_input.cu:48-49, cell.cc) */
+typedef struct rism_grid {
+  int n[3];       /* points per axis; even, 2,3,5-smooth, <= 1024 */
+  double box[3];  /* Angstrom */
+} rism_grid;
+
+/* solvent, as read from a .xvk file (solvent_read.cc:10-86) */
+typedef struct rism_solvent {
+  int nsite;          /* V: reduced (symmetry-merged) sites */
+  int ntab;           /* k-table length */
+  double dk;          /* table spacing; k_i = dk*(i+1), 1/Angstrom */
+  double temper;      /* K */
+  double xt;          /* isothermal compressibility, 1/Pa */
+  const double *rho;  /* [V] number density, 1/Angstrom^3 */
+  const double *q;    /* [V] charge, e */
+  const double *sig;  /* [V] LJ sigma, Angstrom */
+  const double *eps;  /* [V] LJ epsilon, J/mol */
+  const double *chi;  /* [ntab][V][V]: h_v = sum_v' chi[v][v'] c_v' */
+} rism_solvent;
+
+/* solute (read_input.cu:72-76, solute.cu); coordinates used as given */
+typedef struct rism_solute {
+  int natom;
+  const double *sig;  /* [n] Angstrom */
+  const double *eps;  /* [n] J/mol */
+  const double *q;    /* [n] e */
+  const double *xyz;  /* [n][3] Angstrom */
+} rism_solute;
+
+/* solver options (.inp lines 2,4,5 and main.cu -c; plus extensions) */
+typedef struct rism_opts {
+  int closure;    /* RISM_CLOSURE_KH / _HNC */
+  double tol;     /* RMS residual target (co->convergence) */
+  int maxstep;    /* per charge-up stage */
+  int count;      /* Anderson Picard warm-up steps (ma->count) */
+  double m;       /* Anderson mixing (ma->m) */
+  double mp;      /* extra weight of the most recent slot (ma->mp) */
+  int nhist;      /* Anderson history slots; 2 = the reference's AN2; <= 16 */
+  int precision;  /* RISM_PREC_FP64 / RISM_PREC_MIXED */
+  int chargeup;   /* charge-up stages (main.cu -c); 0 = start from t = q_v*fr */
+  int verbose;    /* print the reference's progress lines on stdout */
+  int ksave;      /* .inp ksave: write the .tuv every ksave steps (iterate.cu:64-71) */
+} rism_opts;
+
+/* the .xmu quantities (output_xmu.cu:7-111), J/mol unless noted */
+typedef struct rism_xmu {
+  int nsite;
+  double sfe_sc, sfe_sc_hnc, sfe_gf;
+  double sfec_sc[16], sfec_sc_hnc[16], sfec_gf[16];
+  double se, se_lj, se_es;
+  double pmv;        /* L/mol */
+  double pressure;   /* J/m^3 */
+  double correction; /* J/mol */
+} rism_xmu;
+
+/* a parsed deck: .inp (+ optional separate solute file) and its .xvk */
+typedef struct rism_deck {
+  char outlist[64];
+  int ksave;
+  char solvent_path[4096];
+  char fname[4096];     /* output base name (set_fname.cu:5-12) */
+  rism_grid grid;
+  rism_opts opts;
+  int natom;
+  double *sig, *eps, *q, *xyz;   /* solute, centred if requested */
+  double shift[3];
+  int nsite, ntab;
+  double dk, temper, xt;
+  double *rho, *sq, *ssig, *seps, *chi;
+} rism_deck;
+
+/* ---- library ---------------------------------------------------------- */
+const char *rism_last_error(void);
+const char *rism_version(void);
+/* number of HIP devices visible (0 without a GPU) */
+int rism_device_count(void);
+
+/* ---- input surface (host only, no GPU needed) ------------------------- */
+/* read_input.cu:21-89 + solute.cu:13-49 + solvent_read.cc:10-86.
+   control: .inp path; structure: separate solute file or NULL;
+   centering: solute.cu:13-43 shift; zero: main.cu -z (zero solute charges).
+   The solvent path in the deck is opened as written, relative to the
+   current directory, as the reference does; if absent there, it is tried
+   relative to the directory of the control file. */
+int rism_deck_read(const char *control, const char *structure, int centering,
+                   int zero, rism_deck **out);
+void rism_deck_free(rism_deck *d);
+/* views of a deck as create() arguments (pointers into the deck) */
+void rism_deck_views(const rism_deck *d, rism_grid *g, rism_solvent *sv,
+                     rism_solute *su, rism_opts *o);
+/* .xvk reader alone (solvent_read.cc:10-86); arrays owned by the deck */
+int rism_xvk_read(const char *path, rism_deck *into);
+
+/* ---- solve lifecycle (RISM3D::initialize/iterate/output) -------------- */
+/* RISM3D::initialize (initialize.cu:4-13) minus parsing: k-grid + chi
+   spline (initialize_g.cu, solvent_spline.cu), device allocation.  device
+   selects the HIP device (main.cu -d / cudaSetDevice). */
+int rism_create(const rism_grid *grid, const rism_solvent *solvent,
+                const rism_solute *solute, const rism_opts *opts, int device,
+                rism_handle **out);
+/* RISM3D::cal_potential (cal_potential.cu:3-14): LJ, Coulomb, fr, fk */
+int rism_build_potentials(rism_handle *h);
+/* RISM3D::iterate (iterate.cu:6-90).  If tuv_path names an existing file
+   the solve restarts from it and charge-up is skipped (iterate.cu:25-33).
+   Returns RISM_SOLVE_* or RISM_ERR_*. */
+int rism_solve(rism_handle *h, const char *tuv_path, int *steps, double *rms);
+/* cal_exchem/cal_pmv/cal_pressure/cal_euv (output.cu:10-27) */
+int rism_thermo(rism_handle *h, rism_xmu *out);
+/* cal_grad (cal_grad.cu:8-48) scaled as written by output_grad.cu:15-24:
+   out[n][6] = (dLJ_x, dLJ_y, dLJ_z, dES_x, dES_y, dES_z), J/mol/Angstrom */
+int rism_grad(rism_handle *h, double *out);
+/* copy a field (RISM_FIELD_*) to host memory */
+int rism_get_field(rism_handle *h, int which, double *out);
+/* replace t (host [V][N]), e.g. a restart from memory */
+int rism_set_t(rism_handle *h, const double *t);
+/* write_tuv.cu:7-33 / read_tuv.cu:6-34 (.tuv text, 6 significant digits) */
+int rism_save_tuv(rism_handle *h, const char *path);
+int rism_load_tuv(rism_handle *h, const char *path);
+/* .xmu writer (output_xmu.cu:7-111) and .gra writer (output_grad.cu:7-30) */
+int rism_write_xmu(const rism_xmu *x, int closure, const char *path);
+int rism_write_gra(const double *grad, int natom, const char *path);
+/* per-solve statistics: steps, final rms, and timings in seconds */
+int rism_stats(rism_handle *h, int *steps, double *rms, double *t_setup,
+               double *t_solve, double *t_thermo);
+/* rms after each step of the last solve (n <= maxstep); returns count */
+int rism_rms_history(rism_handle *h, double *out, int n);
+void rism_destroy(rism_handle *h);
+
+/* ---- hot-path seams on caller device memory --------------------------- */
+/* FFT3D::execute (fft3d.cu:22-38): in-place transform of one x-fastest
+   complex field with the reference's convention (dir -1: x dv with the
+   half-shifted k grid; +1: / volume).  data: device pointer to
+   double2[N] (precision FP64) or float2[N] (mixed).  Runs on the handle's
+   stream and synchronises before returning. */
+int rism_fft3d(rism_handle *h, void *data, int dir);
+/* RISM3D::calculate (calculate.cu:4-58) for charge factor 1 on device
+   memory: t [V][N] in, residual [V][N] out (both double). */
+int rism_calculate(rism_handle *h, const double *t_dev, double *res_dev);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RISM3D_AMD_H */

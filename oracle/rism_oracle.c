@@ -1186,3 +1186,35 @@ void orc_copy_field(const orc_sys *s, int which, double *out) {
     default: break;
   }
 }
+
+void orc_set_nhist(orc_sys *s, int nhist) {
+  free_fields(s);
+  s->nhist = nhist > 0 ? nhist : 2;
+}
+
+void orc_set_iter(orc_sys *s, double tol, int maxstep, int count, double m, double mp) {
+  s->tol = tol; s->maxstep = maxstep; s->count = count; s->m = m; s->mp = mp;
+}
+
+int orc_alloc_test(orc_sys *s, int natv, long ng, int nhist, const double *rho) {
+  free_fields(s);
+  FREE0(s->rhov);
+  s->natv = natv;
+  s->ng = ng;
+  s->nhist = nhist > 0 ? nhist : 2;
+  s->rhov = xcalloc(natv, sizeof(double));
+  memcpy(s->rhov, rho, natv * sizeof(double));
+  ensure_fields(s);
+  orc_anderson_reset(s);
+  return 0;
+}
+
+void orc_set_tr(orc_sys *s, const double *tr) {
+  ensure_fields(s);
+  memcpy(s->tr, tr, sizeof(double) * s->natv * s->ng);
+}
+
+void orc_copy_hist(const orc_sys *s, int which, int slot, double *out) {
+  size_t VN = (size_t)s->natv * s->ng;
+  memcpy(out, (which ? s->rp : s->tp) + (size_t)slot * VN, VN * sizeof(double));
+}

@@ -144,6 +144,12 @@ int orc_natu(const orc_sys *s);
 void orc_copy_field(const orc_sys *s, int which, double *out);
 void orc_set_t(orc_sys *s, const double *t);
 void orc_set_threads(int n);
+void orc_set_nhist(orc_sys *s, int nhist);
+/* test hooks: a bare field set for exercising the Anderson step alone */
+int orc_alloc_test(orc_sys *s, int natv, long ng, int nhist, const double *rho);
+void orc_set_tr(orc_sys *s, const double *tr);
+void orc_copy_hist(const orc_sys *s, int which, int slot, double *out);
+void orc_set_iter(orc_sys *s, double tol, int maxstep, int count, double m, double mp);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,156 @@
+"""GPU parity: the HIP path through the C ABI against the FP64 oracle.
+
+Tolerances: FP64 path -- FFT 1e-12 relative to the field's max modulus,
+potentials 1e-12 relative, one fixed-point map 1e-10 absolute on R, full
+solves: g max-abs 1e-4 (observed ~1e-9) and Delta-mu within
+0.05 J/mol.  Mixed path -- FFT 2e-5 relative; solves: g max-abs 1e-4 and
+Delta-mu within 41.84 J/mol (0.01 kcal/mol).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import rism3d
+from conftest import example
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _solvent(name="h2o_vv.xvk"):
+    return rism3d.read_xvk(example(name))
+
+
+def _atoms(k, seed=7, spread=3.0):
+    rng = np.random.default_rng(seed)
+    xyz = rng.uniform(-spread, spread, size=(k, 3))
+    sig = rng.uniform(2.5, 3.8, size=k)
+    eps = rng.uniform(200.0, 800.0, size=k)
+    q = rng.normal(0.0, 0.5, size=k)
+    q -= q.mean()
+    return sig, eps, q, xyz
+
+
+def _solver(n, box, prec=rism3d.FP64, natom=1, **kw):
+    sig, eps, q, xyz = _atoms(natom)
+    su = rism3d.Solute(sig, eps, q, xyz)
+    opts = rism3d.Options(precision=prec, **kw)
+    return rism3d.Solver(n, box, _solvent(), su, opts)
+
+
+FFT_SIZES = [(32, 32, 32), (24, 32, 40), (48, 48, 48), (20, 20, 20), (18, 30, 50), (64, 60, 72),
+             (16, 128, 8)]
+
+
+@pytest.mark.parametrize("n", FFT_SIZES)
+def test_fft_fp64_matches_oracle(gpu, n):
+    print(f"fft fp64 n={n}", flush=True)
+    rng = np.random.default_rng(11)
+    box = (n[0] * 0.5, n[1] * 0.45, n[2] * 0.55)
+    a = rng.standard_normal(n[::-1]) + 1j * rng.standard_normal(n[::-1])
+    s = _solver(n, box)
+    d = torch.from_numpy(a.copy()).to("cuda")
+    s.fft3d(d.data_ptr(), -1)
+    got = d.cpu().numpy()
+    ref = orc.fft3d(a, n, box, -1)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err < 1e-12, err
+    s.fft3d(d.data_ptr(), +1)
+    back = d.cpu().numpy()
+    assert np.abs(back - a).max() < 1e-12 * np.abs(a).max()
+    s.close()
+
+
+@pytest.mark.parametrize("n", [(32, 32, 32), (48, 60, 40), (18, 30, 50)])
+def test_fft_fp32_matches_oracle(gpu, n):
+    print(f"fft fp32 n={n}", flush=True)
+    rng = np.random.default_rng(12)
+    box = (n[0] * 0.5, n[1] * 0.5, n[2] * 0.5)
+    a = rng.standard_normal(n[::-1]) + 1j * rng.standard_normal(n[::-1])
+    s = _solver(n, box, prec=rism3d.MIXED)
+    d = torch.from_numpy(a.astype(np.complex64)).to("cuda")
+    s.fft3d(d.data_ptr(), -1)
+    got = d.cpu().numpy().astype(np.complex128)
+    ref = orc.fft3d(a, n, box, -1)
+    assert np.abs(got - ref).max() / np.abs(ref).max() < 2e-6
+    s.close()
+
+
+def test_potentials_match_oracle(gpu, tmp_path):
+    print("potentials 40^3, 6 atoms", flush=True)
+    n, box = (40, 40, 40), (20.0, 20.0, 20.0)
+    sig, eps, q, xyz = _atoms(6)
+    s = rism3d.Solver(n, box, _solvent(), rism3d.Solute(sig, eps, q, xyz), rism3d.Options())
+    s.build_potentials()
+    o = orc.Oracle.from_arrays(n, box, example("h2o_vv.xvk"), sig, eps, q, xyz).setup()
+    for which, ow in ((rism3d.FIELD_ULJ, orc.F_DU), (rism3d.FIELD_DE, orc.F_DE),
+                      (rism3d.FIELD_FR, orc.F_FR), (rism3d.FIELD_FK, orc.F_FK)):
+        got = s.field(which).reshape(-1)
+        ref = o.field(ow)
+        rel = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)
+        scale = np.abs(ref).max()
+        assert np.all((rel < 1e-11) | (np.abs(got - ref) < 1e-12 * scale)), (which, rel.max())
+    chi = s.field(rism3d.FIELD_CHI).reshape(-1)
+    assert np.abs(chi - o.field(orc.F_CHI)).max() < 1e-12
+    s.close()
+
+
+def test_calculate_matches_oracle(gpu, deckdir):
+    """One fixed-point map t -> R (calculate.cu) from the initial estimate."""
+    print("calculate test1 deck", flush=True)
+    deck = rism3d.read_deck("test1.inp")
+    o = orc.Oracle.from_deck("test1.inp").read_xvk("h2o_vv.xvk").setup()
+    fr = o.field(orc.F_FR)
+    qv = deck.solvent.q
+    t0 = np.concatenate([qv[v] * fr for v in range(len(qv))])
+    rng = np.random.default_rng(3)
+    t0 = t0 + 0.05 * rng.standard_normal(t0.shape)
+    o.set_t(t0)
+    o.calculate(1.0)
+    ref_r = o.field(orc.F_TR)
+    ref_h = o.field(orc.F_H)
+    s = rism3d.Solver.from_deck(deck).build_potentials()
+    td = torch.from_numpy(t0.copy()).to("cuda")
+    rd = torch.zeros_like(td)
+    s.calculate(td.data_ptr(), rd.data_ptr())
+    r = rd.cpu().numpy()
+    assert np.abs(r - ref_r).max() < 1e-10, np.abs(r - ref_r).max()
+    h = s.field(rism3d.FIELD_H).reshape(-1)
+    assert np.abs(h - ref_h).max() < 1e-10
+    s.close()
+
+
+@pytest.mark.parametrize("prec", [rism3d.FP64, rism3d.MIXED])
+def test_solve_test1_matches_oracle(gpu, deckdir, prec):
+    print(f"solve test1 prec={prec}", flush=True)
+    deck = rism3d.read_deck("test1.inp")
+    o = orc.Oracle.from_deck("test1.inp").read_xvk("h2o_vv.xvk").setup()
+    ost, osteps, orms = o.iterate()
+    ox = o.thermo()
+    og = o.field(orc.F_G)
+    s = rism3d.Solver.from_deck(deck, precision=prec).build_potentials()
+    st, steps, rms = s.solve()
+    x = s.thermo()
+    g = s.field(rism3d.FIELD_G).reshape(-1)
+    print("oracle", ost, osteps, orms, ox["SFE_SC"], "gpu", st, steps, rms, x["SFE_SC"], flush=True)
+    assert st == rism3d.SOLVE_CONVERGED and ost == 0
+    assert rms <= deck.opts.tol
+    gerr = np.abs(g - og).max()
+    if prec == rism3d.FP64:
+        assert abs(steps - osteps) <= 1
+        hist = s.rms_history()
+        assert len(hist) == steps
+        assert abs(x["SFE_SC"] - ox["SFE_SC"]) < 0.05
+        assert abs(x["SFE_GF"] - ox["SFE_GF"]) < 0.05
+        assert abs(x["SE"] - ox["SE"]) < 0.05
+        assert abs(x["PMV"] - ox["PMV"]) < 1e-8
+        assert gerr < 1e-6, gerr
+    else:
+        assert abs(x["SFE_SC"] - ox["SFE_SC"]) < 41.84
+        assert gerr < 1e-4, gerr
+    assert x["Pressure"] == pytest.approx(ox["Pressure"], rel=1e-14)
+    s.close()
