@@ -56,6 +56,8 @@ struct AxisPlan {
   int s[kMaxStages];      // Stockham stride before the stage
   int m[kMaxStages];      // current length / radix
   int tstep[kMaxStages];  // n / current length
+  int ls2[kMaxStages];    // log2(s) when s is a power of two, else -1
+  int lb2[kMaxStages];    // log2(n / radix) when a power of two, else -1
   int fixsign;            // (-1)^(n/2): 1 when n % 4 == 0
   const void *tw;         // complex[n] exp(-2 pi i k/n)
   const void *pre;        // complex[n] (-1)^x exp(-i pi (x - n/2)/n)
@@ -168,6 +170,13 @@ template <bool INV, typename T2> struct Dft<16, INV, T2> {
   }
 };
 
+// LDS position of element i of a line: one pad slot every 16 elements, so
+// the stride-F writes of a Stockham stage and the column loads of the y/z
+// passes spread over the banks instead of hitting a few of them.
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
+// padded line stride (odd, so consecutive lines also start on other banks)
+__host__ __device__ constexpr int line_stride(int n) { return n + (n >> 4) + 1; }
+
 // ---------------------------------------------------------------------
 // one Stockham stage over all lines held in LDS (in place: load phase,
 // barrier, store phase, barrier).  Element (line l, index i) lives at
@@ -175,42 +184,43 @@ template <bool INV, typename T2> struct Dft<16, INV, T2> {
 // ---------------------------------------------------------------------
 template <int F, int KMAX, bool INV, typename T2>
 __device__ __forceinline__ void fft_stage(T2 *lds, int nlines, int ls, int n, int s, int m,
-                                          int tstep, const T2 *__restrict__ tw) {
+                                          int tstep, int ls2, int lb2, const T2 *__restrict__ tw) {
   const int nbl = n / F;
   const int total = nlines * nbl;
   T2 v[KMAX][F];
-  int base[KMAX], ppv[KMAX];
+  int lbase[KMAX], q0[KMAX], ppv[KMAX];
 #pragma unroll
   for (int k = 0; k < KMAX; ++k) {
     const int bf = threadIdx.x + k * kThreads;
-    base[k] = -1;
+    lbase[k] = -1;
     if (bf < total) {
-      const int line = bf / nbl;
+      const int line = lb2 >= 0 ? (bf >> lb2) : bf / nbl;
       const int j = bf - line * nbl;
-      const int pp = j / s;
+      const int pp = ls2 >= 0 ? (j >> ls2) : j / s;
       const int q = j - pp * s;
-      base[k] = line * ls + q;
+      lbase[k] = line * ls;
+      q0[k] = q;
       ppv[k] = pp;
 #pragma unroll
-      for (int a = 0; a < F; ++a) v[k][a] = lds[base[k] + s * (pp + a * m)];
+      for (int a = 0; a < F; ++a) v[k][a] = lds[lbase[k] + lpad(q + s * (pp + a * m))];
     }
   }
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < KMAX; ++k) {
-    if (base[k] >= 0) {
+    if (lbase[k] >= 0) {
       Dft<F, INV, T2>::run(v[k]);
       const int pp = ppv[k];
-      const int o = base[k] + s * (F * pp);
-      lds[o] = v[k][0];
+      const int o = q0[k] + s * (F * pp);
+      lds[lbase[k] + lpad(o)] = v[k][0];
       if (pp == 0) {
 #pragma unroll
-        for (int b = 1; b < F; ++b) lds[o + s * b] = v[k][b];
+        for (int b = 1; b < F; ++b) lds[lbase[k] + lpad(o + s * b)] = v[k][b];
       } else {
 #pragma unroll
         for (int b = 1; b < F; ++b) {
           T2 w = tw[pp * b * tstep];
-          lds[o + s * b] = INV ? cmulc(v[k][b], w) : cmul(v[k][b], w);
+          lds[lbase[k] + lpad(o + s * b)] = INV ? cmulc(v[k][b], w) : cmul(v[k][b], w);
         }
       }
     }
@@ -225,13 +235,14 @@ __device__ __forceinline__ void fft_lines(T2 *lds, int nlines, int ls, const Axi
   const T2 *__restrict__ tw = (const T2 *)p.tw;
   for (int st = 0; st < p.nst; ++st) {
     const int F = p.rad[st], s = p.s[st], m = p.m[st], ts = p.tstep[st];
+    const int a2 = p.ls2[st], b2 = p.lb2[st];
     switch (F) {
-      case 16: fft_stage<16, (EPT + 15) / 16, INV>(lds, nlines, ls, p.n, s, m, ts, tw); break;
-      case 8: fft_stage<8, (EPT + 7) / 8, INV>(lds, nlines, ls, p.n, s, m, ts, tw); break;
-      case 4: fft_stage<4, (EPT + 3) / 4, INV>(lds, nlines, ls, p.n, s, m, ts, tw); break;
-      case 2: fft_stage<2, (EPT + 1) / 2, INV>(lds, nlines, ls, p.n, s, m, ts, tw); break;
-      case 3: fft_stage<3, (EPT + 2) / 3, INV>(lds, nlines, ls, p.n, s, m, ts, tw); break;
-      case 5: fft_stage<5, (EPT + 4) / 5, INV>(lds, nlines, ls, p.n, s, m, ts, tw); break;
+      case 16: fft_stage<16, (EPT + 15) / 16, INV>(lds, nlines, ls, p.n, s, m, ts, a2, b2, tw); break;
+      case 8: fft_stage<8, (EPT + 7) / 8, INV>(lds, nlines, ls, p.n, s, m, ts, a2, b2, tw); break;
+      case 4: fft_stage<4, (EPT + 3) / 4, INV>(lds, nlines, ls, p.n, s, m, ts, a2, b2, tw); break;
+      case 2: fft_stage<2, (EPT + 1) / 2, INV>(lds, nlines, ls, p.n, s, m, ts, a2, b2, tw); break;
+      case 3: fft_stage<3, (EPT + 2) / 3, INV>(lds, nlines, ls, p.n, s, m, ts, a2, b2, tw); break;
+      case 5: fft_stage<5, (EPT + 4) / 5, INV>(lds, nlines, ls, p.n, s, m, ts, a2, b2, tw); break;
       default: break;
     }
   }

@@ -1,4 +1,4 @@
-// k_iterate.hip -- the fused fixed-point iteration of 3D-RISM on gfx950.
+// k_iterate_impl.h -- the fused fixed-point iteration of 3D-RISM on gfx950.
 //
 // One iteration (src-cuda/iterate.cu:51-73 with calculate.cu:4-58 and
 // anderson.cu:35-84) is six launches on one stream:
@@ -18,6 +18,7 @@
 //                     test, Picard/Anderson coefficients (leg), control block
 // The control block gates every kernel so the host can enqueue iterations
 // ahead without a synchronisation per step.
+#pragma once
 #include "rism_kernels.h"
 
 namespace rism {
@@ -36,7 +37,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 // ---------------------------------------------------------------------
 // 1. update + closure + forward x
 // ---------------------------------------------------------------------
-template <typename R, typename S, int EPT, bool UPDATE>
+template <typename R, typename S, int EPT, bool UPDATE, int VC>
 __global__ __launch_bounds__(kThreads) void k_closure_fx(IterArgs<R, S> a) {
   using T2 = typename C2<R>::T;
   const Ctl *ctl = a.ctl;
@@ -45,7 +46,7 @@ __global__ __launch_bounds__(kThreads) void k_closure_fx(IterArgs<R, S> a) {
   if (!do_upd && !do_run) return;
   extern __shared__ unsigned char smem_raw[];
   T2 *lds = reinterpret_cast<T2 *>(smem_raw);
-  const int nx = a.nx, L = a.Lx, ls = nx + 1, V = a.V, P = a.P;
+  const int nx = a.nx, L = a.Lx, ls = line_stride(nx), V = a.V, P = a.P;
   const long N = a.N;
   const long base = (long)blockIdx.x * L * nx;
   const int nel = L * nx;
@@ -62,9 +63,9 @@ __global__ __launch_bounds__(kThreads) void k_closure_fx(IterArgs<R, S> a) {
     const long ip = base + e;
     const int i = e / nx;
     const int x = e - i * nx;
-    R cs[kMaxV];
+    R cs[VC];
 #pragma unroll
-    for (int v = 0; v < kMaxV; ++v) {
+    for (int v = 0; v < VC; ++v) {
       if (v < V) {
         const long iv = (long)v * N + ip;
         S t = a.ts[iv];
@@ -82,9 +83,23 @@ __global__ __launch_bounds__(kThreads) void k_closure_fx(IterArgs<R, S> a) {
           a.ts[iv] = t;
         }
         if (do_run) {
-          const R tr = (R)t;
-          const R g = closure_g<R>(tr - a.u[iv], a.closure);
-          cs[v] = g - (R)1 - tr;
+          if (a.src) {
+            cs[v] = (R)a.cref[iv];
+          } else {
+            // closure in FP64 in both precisions: near a charged solute its
+            // argument ts - u is the small difference of large terms
+            const double td = (double)t;
+            const double g = closure_g<double>(td - (double)a.u[iv], a.closure);
+            double c = g - 1.0 - td;
+            if (a.refresh) {  // new reference; the rounding remainder goes to the FP32 passes
+              const float cr = (float)c;
+              a.cref_out[iv] = cr;
+              c -= (double)cr;
+            } else if (a.delta) {
+              c -= (double)a.cref[iv];
+            }
+            cs[v] = (R)c;
+          }
         }
       } else {
         cs[v] = (R)0;
@@ -93,8 +108,8 @@ __global__ __launch_bounds__(kThreads) void k_closure_fx(IterArgs<R, S> a) {
     if (do_run) {
       const T2 pw = pre[x];
 #pragma unroll
-      for (int p = 0; p < (kMaxV + 1) / 2; ++p)
-        if (p < P) lds[(p * L + i) * ls + x] = cmul(mk<T2>(cs[2 * p], cs[2 * p + 1]), pw);
+      for (int p = 0; p < (VC + 1) / 2; ++p)
+        if (p < P) lds[(p * L + i) * ls + lpad(x)] = cmul(mk<T2>(cs[2 * p], 2 * p + 1 < VC ? cs[2 * p + 1] : (R)0), pw);
     }
   }
   if (!do_run) return;
@@ -105,7 +120,7 @@ __global__ __launch_bounds__(kThreads) void k_closure_fx(IterArgs<R, S> a) {
     const int i = e / nx;
     const int j = e - i * nx;
     const R sc = dv * (R)jsign(j, a.px.fixsign);
-    for (int p = 0; p < P; ++p) a.Z[p * N + base + e] = cscale(lds[(p * L + i) * ls + j], sc);
+    for (int p = 0; p < P; ++p) a.Z[p * N + base + e] = cscale(lds[(p * L + i) * ls + lpad(j)], sc);
   }
 }
 
@@ -119,7 +134,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_y(typename C2<R>::T *Z, const 
   if (ctl && ctl->run_iter != iter) return;
   extern __shared__ unsigned char smem_raw[];
   T2 *lds = reinterpret_cast<T2 *>(smem_raw);
-  const int ls = ny + 1;
+  const int ls = line_stride(ny);
   const int x0 = blockIdx.x * T;
   const long zoff = (long)blockIdx.y * nx * ny;
   T2 *Zp = Z + (long)blockIdx.z * N + zoff + x0;
@@ -130,14 +145,14 @@ __global__ __launch_bounds__(kThreads) void k_fft_y(typename C2<R>::T *Z, const 
     const int i = e - y * T;
     T2 v = Zp[(long)y * nx + i];
     v = INV ? cscale(v, (decltype(v.x))jsign(y, py.fixsign)) : cmul(v, pre[y]);
-    lds[i * ls + y] = v;
+    lds[i * ls + lpad(y)] = v;
   }
   __syncthreads();
   fft_lines<EPT, INV>(lds, T, ls, py);
   for (int e = threadIdx.x; e < nel; e += kThreads) {
     const int y = e / T;
     const int i = e - y * T;
-    T2 v = lds[i * ls + y];
+    T2 v = lds[i * ls + lpad(y)];
     v = INV ? cmulc(v, pre[y]) : cscale(v, (decltype(v.x))jsign(y, py.fixsign));
     Zp[(long)y * nx + i] = v;
   }
@@ -148,7 +163,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_y(typename C2<R>::T *Z, const 
 // A workgroup owns tile A = (x in [xa, xa+T), y) and its mirror tile
 // B = (x in [nx-xa-T, nx-xa), ny-1-y), so every k and -k are local.
 // ---------------------------------------------------------------------
-template <typename R, typename S, int EPT>
+template <typename R, typename S, int EPT, int VC>
 __global__ __launch_bounds__(kThreads) void k_fft_z_oz(IterArgs<R, S> a) {
   using T2 = typename C2<R>::T;
   using T4 = typename R4T<R>::T;
@@ -157,7 +172,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_oz(IterArgs<R, S> a) {
   T2 *lds = reinterpret_cast<T2 *>(smem_raw);
   const int nx = a.nx, ny = a.ny, nz = a.nz, T = a.Tz, P = a.P, V = a.V;
   const long N = a.N, nxy = (long)nx * ny;
-  const int ls = nz + 1;
+  const int ls = line_stride(nz);
   const int xa = blockIdx.x * T, ya = blockIdx.y;
   const int xb = nx - xa - T, yb = ny - 1 - ya;
   const T2 *pre = (const T2 *)a.pz.pre;
@@ -169,7 +184,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_oz(IterArgs<R, S> a) {
     const int z = r % nz; r /= nz;
     const int tile = r & 1, p = r >> 1;
     const long off = (long)p * N + (long)z * nxy + (tile ? (long)yb * nx + xb : (long)ya * nx + xa) + i;
-    lds[((p * 2 + tile) * T + i) * ls + z] = cmul(a.Z[off], pre[z]);
+    lds[((p * 2 + tile) * T + i) * ls + lpad(z)] = cmul(a.Z[off], pre[z]);
   }
   __syncthreads();
   fft_lines<EPT, false>(lds, 2 * P * T, ls, a.pz);
@@ -194,31 +209,33 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_oz(IterArgs<R, S> a) {
     const R tt = (R)(kk - a.dk * (lo + 1));
     const R sa = (R)jsign(j, a.pz.fixsign), sb = (R)jsign(jb, a.pz.fixsign);
     const T2 fk = a.fkh[((long)j * (ny / 2) + ya) * nx + x];
-    T2 c[kMaxV];
+    T2 c[VC + 1];
 #pragma unroll
-    for (int p = 0; p < (kMaxV + 1) / 2; ++p) {
+    for (int p = 0; p < (VC + 1) / 2; ++p) {
       if (p < P) {
-        const T2 za = cscale(lds[((p * 2 + 0) * T + i) * ls + j], sa);
-        const T2 zb = cscale(lds[((p * 2 + 1) * T + ib) * ls + jb], sb);
+        const T2 za = cscale(lds[((p * 2 + 0) * T + i) * ls + lpad(j)], sa);
+        const T2 zb = cscale(lds[((p * 2 + 1) * T + ib) * ls + lpad(jb)], sb);
         // C_{2p} = (za + conj zb)/2 ; C_{2p+1} = -i (za - conj zb)/2
         c[2 * p] = mk<T2>((R)0.5 * (za.x + zb.x), (R)0.5 * (za.y - zb.y));
         c[2 * p + 1] = mk<T2>((R)0.5 * (za.y + zb.y), (R)-0.5 * (za.x - zb.x));
       }
     }
+    if (a.use_fk) {
 #pragma unroll
-    for (int v = 0; v < kMaxV; ++v)
-      if (v < V) {
-        const R q = (R)a.qcf[v];
-        c[v] = mk<T2>(c[v].x - q * fk.x, c[v].y - q * fk.y);  // mqvfk
-      }
-    T2 h[kMaxV];
+      for (int v = 0; v < VC; ++v)
+        if (v < V) {
+          const R q = (R)a.qcf[v];
+          c[v] = mk<T2>(c[v].x - q * fk.x, c[v].y - q * fk.y);  // mqvfk
+        }
+    }
+    T2 h[VC + 1];
 #pragma unroll
-    for (int v = 0; v < kMaxV; ++v) {
+    for (int v = 0; v < VC + 1; ++v) {
       h[v] = mk<T2>((R)0, (R)0);
       if (v < V) {
         R hr = 0, hi = 0;
 #pragma unroll
-        for (int v1 = 0; v1 < kMaxV; ++v1)
+        for (int v1 = 0; v1 < VC; ++v1)
           if (v1 < V) {  // oz: h_v = sum_v' chi[v][v'] c_v'
             const T4 cf = a.chi[(long)(v * V + v1) * nseg + lo];
             const R xv = ((cf.w * tt + cf.z) * tt + cf.y) * tt + cf.x;
@@ -229,14 +246,14 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_oz(IterArgs<R, S> a) {
       }
     }
 #pragma unroll
-    for (int p = 0; p < (kMaxV + 1) / 2; ++p) {
+    for (int p = 0; p < (VC + 1) / 2; ++p) {
       if (p < P) {
         const T2 h0 = h[2 * p], h1 = h[2 * p + 1];
         // H(k) = h0 + i h1 ; H(-k) = conj(h0) + i conj(h1)
         const T2 ha = mk<T2>(h0.x - h1.y, h0.y + h1.x);
         const T2 hb = mk<T2>(h0.x + h1.y, h1.x - h0.y);
-        lds[((p * 2 + 0) * T + i) * ls + j] = cscale(ha, sa);
-        lds[((p * 2 + 1) * T + ib) * ls + jb] = cscale(hb, sb);
+        lds[((p * 2 + 0) * T + i) * ls + lpad(j)] = cscale(ha, sa);
+        lds[((p * 2 + 1) * T + ib) * ls + lpad(jb)] = cscale(hb, sb);
       }
     }
   }
@@ -248,14 +265,14 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_oz(IterArgs<R, S> a) {
     const int z = r % nz; r /= nz;
     const int tile = r & 1, p = r >> 1;
     const long off = (long)p * N + (long)z * nxy + (tile ? (long)yb * nx + xb : (long)ya * nx + xa) + i;
-    a.Z[off] = cmulc(lds[((p * 2 + tile) * T + i) * ls + z], pre[z]);
+    a.Z[off] = cmulc(lds[((p * 2 + tile) * T + i) * ls + lpad(z)], pre[z]);
   }
 }
 
 // ---------------------------------------------------------------------
 // 5. inverse x + residual + Anderson inner products
 // ---------------------------------------------------------------------
-template <typename R, typename S, int EPT, int HM>
+template <typename R, typename S, int EPT, int HM, int VC>
 __global__ __launch_bounds__(kThreads) void k_fx_resid(IterArgs<R, S> a) {
   using T2 = typename C2<R>::T;
   constexpr int NV = 1 + HM + HM * (HM + 1) / 2;
@@ -263,7 +280,7 @@ __global__ __launch_bounds__(kThreads) void k_fx_resid(IterArgs<R, S> a) {
   if (ctl->run_iter != a.iter) return;
   extern __shared__ unsigned char smem_raw[];
   T2 *lds = reinterpret_cast<T2 *>(smem_raw);
-  const int nx = a.nx, L = a.Lx, ls = nx + 1, V = a.V, P = a.P;
+  const int nx = a.nx, L = a.Lx, ls = line_stride(nx), V = a.V, P = a.P;
   const long N = a.N, VN = (long)V * N;
   const int nel = L * nx;
   const int groups = (int)((long)a.ny * a.nz / L);
@@ -280,7 +297,7 @@ __global__ __launch_bounds__(kThreads) void k_fx_resid(IterArgs<R, S> a) {
       const int i = e / nx;
       const int j = e - i * nx;
       const R sg = (R)jsign(j, a.px.fixsign);
-      for (int p = 0; p < P; ++p) lds[(p * L + i) * ls + j] = cscale(a.Z[p * N + base + e], sg);
+      for (int p = 0; p < P; ++p) lds[(p * L + i) * ls + lpad(j)] = cscale(a.Z[p * N + base + e], sg);
     }
     __syncthreads();
     fft_lines<EPT, true>(lds, P * L, ls, a.px);
@@ -289,23 +306,29 @@ __global__ __launch_bounds__(kThreads) void k_fx_resid(IterArgs<R, S> a) {
       const int i = e / nx;
       const int x = e - i * nx;
       const T2 pw = pre[x];
-      R hv[kMaxV + 1];
+      double hv[VC + 1];
 #pragma unroll
-      for (int p = 0; p < (kMaxV + 1) / 2; ++p) {
+      for (int p = 0; p < (VC + 1) / 2; ++p) {
         if (p < P) {
-          const T2 z = cscale(cmulc(lds[(p * L + i) * ls + x], pw), ivol);
-          hv[2 * p] = z.x;
-          hv[2 * p + 1] = z.y;
+          const T2 z = cscale(cmulc(lds[(p * L + i) * ls + lpad(x)], pw), ivol);
+          hv[2 * p] = (double)z.x;
+          hv[2 * p + 1] = (double)z.y;
+          if (a.delta) {
+            const double2 hr = a.href[p * N + ip];
+            hv[2 * p] += hr.x;
+            hv[2 * p + 1] += hr.y;
+          }
           if (a.store_h) a.Z[p * N + ip] = z;
         }
       }
+      if (a.href_only) continue;
 #pragma unroll
-      for (int v = 0; v < kMaxV; ++v) {
+      for (int v = 0; v < VC; ++v) {
         if (v < V) {
           const long iv = (long)v * N + ip;
-          const S t = a.ts[iv];
-          const R g = closure_g<R>((R)t - a.u[iv], a.closure);
-          const S rv = (S)hv[v] + (S)1 - (S)g;  // tr: R = h + 1 - g
+          const double t = (double)a.ts[iv];
+          const double g = closure_g<double>(t - (double)a.u[iv], a.closure);
+          const S rv = (S)(hv[v] + 1.0 - g);  // tr: R = h + 1 - g
           a.res[iv] = rv;
           const double rd = (double)rv, wgt = a.rho_inv[v];
           acc[0] += rd * rd;
@@ -342,7 +365,7 @@ __global__ __launch_bounds__(kThreads) void k_fx_resid(IterArgs<R, S> a) {
 // ---------------------------------------------------------------------
 // 6. solve: reduction, rms, convergence, Anderson coefficients
 // ---------------------------------------------------------------------
-__device__ void leg_solve_dev(double *A, double *x, double *b, int n) {
+static __device__ void leg_solve_dev(double *A, double *x, double *b, int n) {
   // Gaussian elimination without pivoting (anderson.cu:87-112)
   for (int k = 0; k < n - 1; ++k) {
     const double p = A[k * n + k];
@@ -362,7 +385,7 @@ __device__ void leg_solve_dev(double *A, double *x, double *b, int n) {
   }
 }
 
-__global__ __launch_bounds__(kThreads) void k_anderson_solve(Ctl *ctl, const double *partials,
+static __global__ __launch_bounds__(kThreads) void k_anderson_solve(Ctl *ctl, const double *partials,
                                                               int nblk, int nv, int hm,
                                                               double *rms_hist, int iter) {
   if (ctl->run_iter != iter) return;
@@ -446,20 +469,20 @@ __global__ __launch_bounds__(kThreads) void k_fft_x_pure(typename C2<R>::T *Z, i
   using T2 = typename C2<R>::T;
   extern __shared__ unsigned char smem_raw[];
   T2 *lds = reinterpret_cast<T2 *>(smem_raw);
-  const int ls = nx + 1, nel = L * nx;
+  const int ls = line_stride(nx), nel = L * nx;
   const long base = (long)blockIdx.x * nel;
   const T2 *pre = (const T2 *)px.pre;
   for (int e = threadIdx.x; e < nel; e += kThreads) {
     const int i = e / nx, x = e - i * nx;
     T2 v = Z[base + e];
     v = INV ? cscale(v, (R)jsign(x, px.fixsign)) : cmul(v, pre[x]);
-    lds[i * ls + x] = v;
+    lds[i * ls + lpad(x)] = v;
   }
   __syncthreads();
   fft_lines<EPT, INV>(lds, L, ls, px);
   for (int e = threadIdx.x; e < nel; e += kThreads) {
     const int i = e / nx, x = e - i * nx;
-    T2 v = lds[i * ls + x];
+    T2 v = lds[i * ls + lpad(x)];
     v = INV ? cscale(cmulc(v, pre[x]), scale) : cscale(v, scale * (R)jsign(x, px.fixsign));
     Z[base + e] = v;
   }
@@ -471,7 +494,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_pure(typename C2<R>::T *Z, i
   using T2 = typename C2<R>::T;
   extern __shared__ unsigned char smem_raw[];
   T2 *lds = reinterpret_cast<T2 *>(smem_raw);
-  const int ls = nz + 1, nel = T * nz;
+  const int ls = line_stride(nz), nel = T * nz;
   const long nxy = (long)nx * ny;
   T2 *Zp = Z + (long)blockIdx.y * nx + blockIdx.x * T;
   const T2 *pre = (const T2 *)pz.pre;
@@ -479,13 +502,13 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_pure(typename C2<R>::T *Z, i
     const int z = e / T, i = e - z * T;
     T2 v = Zp[(long)z * nxy + i];
     v = INV ? cscale(v, (decltype(v.x))jsign(z, pz.fixsign)) : cmul(v, pre[z]);
-    lds[i * ls + z] = v;
+    lds[i * ls + lpad(z)] = v;
   }
   __syncthreads();
   fft_lines<EPT, INV>(lds, T, ls, pz);
   for (int e = threadIdx.x; e < nel; e += kThreads) {
     const int z = e / T, i = e - z * T;
-    T2 v = lds[i * ls + z];
+    T2 v = lds[i * ls + lpad(z)];
     v = INV ? cmulc(v, pre[z]) : cscale(v, (decltype(v.x))jsign(z, pz.fixsign));
     Zp[(long)z * nxy + i] = v;
   }
@@ -503,14 +526,21 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_pure(typename C2<R>::T *Z, i
     default: return hipErrorInvalidValue;         \
   }
 
+#define RISM_VC_SWITCH(V, ...)                                  \
+  switch ((V) <= 2 ? 2 : ((V) <= 4 ? 4 : 8)) {                 \
+    case 2: { constexpr int VC = 2; __VA_ARGS__; } break;       \
+    case 4: { constexpr int VC = 4; __VA_ARGS__; } break;       \
+    default: { constexpr int VC = 8; __VA_ARGS__; } break;      \
+  }
+
 template <typename R, typename S>
 hipError_t launch_closure_fx(const IterArgs<R, S> &a, const LaunchGeom &g, bool update, int,
                              hipStream_t st) {
   const dim3 grid(g.xgroups), block(kThreads);
   if (update) {
-    RISM_EPT_SWITCH(g.ept_x, hipLaunchKernelGGL((k_closure_fx<R, S, E, true>), grid, block, g.lds_x, st, a))
+    RISM_VC_SWITCH(a.V, RISM_EPT_SWITCH(g.ept_x, hipLaunchKernelGGL((k_closure_fx<R, S, E, true, VC>), grid, block, g.lds_x, st, a)))
   } else {
-    RISM_EPT_SWITCH(g.ept_x, hipLaunchKernelGGL((k_closure_fx<R, S, E, false>), grid, block, g.lds_x, st, a))
+    RISM_VC_SWITCH(a.V, RISM_EPT_SWITCH(g.ept_x, hipLaunchKernelGGL((k_closure_fx<R, S, E, false, VC>), grid, block, g.lds_x, st, a)))
   }
   return hipGetLastError();
 }
@@ -531,7 +561,7 @@ hipError_t launch_fft_y(const IterArgs<R, S> &a, const LaunchGeom &g, bool inver
 template <typename R, typename S>
 hipError_t launch_fft_z_oz(const IterArgs<R, S> &a, const LaunchGeom &g, hipStream_t st) {
   const dim3 grid(a.nx / g.Tz, a.ny / 2), block(kThreads);
-  RISM_EPT_SWITCH(g.ept_z, hipLaunchKernelGGL((k_fft_z_oz<R, S, E>), grid, block, g.lds_z, st, a))
+  RISM_VC_SWITCH(a.V, RISM_EPT_SWITCH(g.ept_z, hipLaunchKernelGGL((k_fft_z_oz<R, S, E, VC>), grid, block, g.lds_z, st, a)))
   return hipGetLastError();
 }
 
@@ -539,7 +569,7 @@ template <typename R, typename S>
 hipError_t launch_fx_resid(const IterArgs<R, S> &a, const LaunchGeom &g, int hm, hipStream_t st) {
   const dim3 grid(g.xblocks), block(kThreads);
 #define RISM_RESID(HMV)                                                                       \
-  RISM_EPT_SWITCH(g.ept_x, hipLaunchKernelGGL((k_fx_resid<R, S, E, HMV>), grid, block, g.lds_x, st, a))
+  RISM_VC_SWITCH(a.V, RISM_EPT_SWITCH(g.ept_x, hipLaunchKernelGGL((k_fx_resid<R, S, E, HMV, VC>), grid, block, g.lds_x, st, a)))
   switch (hm) {
     case 2: RISM_RESID(2); break;
     case 5: RISM_RESID(5); break;
@@ -550,7 +580,7 @@ hipError_t launch_fx_resid(const IterArgs<R, S> &a, const LaunchGeom &g, int hm,
   return hipGetLastError();
 }
 
-hipError_t launch_anderson_solve(Ctl *ctl, const double *partials, int nblk, int nv, int hm,
+inline hipError_t launch_anderson_solve_impl(Ctl *ctl, const double *partials, int nblk, int nv, int hm,
                                  double *rms_hist, int iter, hipStream_t st) {
   hipLaunchKernelGGL(k_anderson_solve, dim3(1), dim3(kThreads), 0, st, ctl, partials, nblk, nv, hm,
                      rms_hist, iter);
@@ -566,8 +596,8 @@ hipError_t launch_fft3d(typename C2<R>::T *data, const AxisPlan &px, const AxisP
   const dim3 block(kThreads);
   const dim3 gx((unsigned)((long)ny * nz / L)), gy(nx / g.Ty, nz, 1), gz(nx / g.Tz, ny);
   const size_t sz = sizeof(typename C2<R>::T);
-  const size_t ldsx = (size_t)L * (nx + 1) * sz, ldsy = (size_t)g.Ty * (ny + 1) * sz,
-               ldsz = (size_t)g.Tz * (nz + 1) * sz;
+  const size_t ldsx = (size_t)L * line_stride(nx) * sz, ldsy = (size_t)g.Ty * line_stride(ny) * sz,
+               ldsz = (size_t)g.Tz * line_stride(nz) * sz;
   // pure z tiles hold T lines: EPT class from T*nz
   const int ez = (g.Tz * nz <= 4 * kThreads) ? 4 : (g.Tz * nz <= 8 * kThreads) ? 8 : (g.Tz * nz <= 16 * kThreads) ? 16 : 32;
   if (dir < 0) {
@@ -581,22 +611,5 @@ hipError_t launch_fft3d(typename C2<R>::T *data, const AxisPlan &px, const AxisP
   }
   return hipGetLastError();
 }
-
-#define RISM_INST(R, S)                                                                            \
-  template hipError_t launch_closure_fx<R, S>(const IterArgs<R, S> &, const LaunchGeom &, bool, int, \
-                                              hipStream_t);                                        \
-  template hipError_t launch_fft_y<R, S>(const IterArgs<R, S> &, const LaunchGeom &, bool,          \
-                                         hipStream_t);                                             \
-  template hipError_t launch_fft_z_oz<R, S>(const IterArgs<R, S> &, const LaunchGeom &, hipStream_t); \
-  template hipError_t launch_fx_resid<R, S>(const IterArgs<R, S> &, const LaunchGeom &, int,        \
-                                            hipStream_t);
-RISM_INST(double, double)
-RISM_INST(float, double)
-#undef RISM_INST
-
-template hipError_t launch_fft3d<double>(double2 *, const AxisPlan &, const AxisPlan &, const AxisPlan &,
-                                         const LaunchGeom &, int, int, int, double, double, int, hipStream_t);
-template hipError_t launch_fft3d<float>(float2 *, const AxisPlan &, const AxisPlan &, const AxisPlan &,
-                                        const LaunchGeom &, int, int, int, double, double, int, hipStream_t);
 
 }  // namespace rism

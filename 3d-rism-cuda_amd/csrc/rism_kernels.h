@@ -77,6 +77,19 @@ struct IterArgs {
   int xblocks;      // workgroups of the x passes (grid-stride over line groups)
   int store_h;      // residual pass also writes real-space h into Z
   int iter;
+  // ---- refinement of the FP32 convolution (mixed precision) ----
+  // The OZ step is linear in c_s.  With a reference c_ref (FP32 values)
+  // whose real-space response h_ref was computed once in FP64, the FP32
+  // passes transform only c_s - c_ref and add h_ref back, so their rounding
+  // scales with |c_s - c_ref|, which shrinks as the solve converges.
+  const float *cref;    // [V][N] reference c_s
+  float *cref_out;      // refresh: first pass stores c_s here
+  const double2 *href;  // [P][N] FP64 real-space response to cref (incl. long range)
+  int delta;            // FP32 passes: subtract cref / add href, no fk term
+  int refresh;          // FP32 first pass: c_s -> cref_out, emit a zero spectrum
+  int src;              // FP64 first pass: transform cref instead of the closure
+  int href_only;        // FP64 residual pass: store h only
+  int use_fk;           // z pass subtracts q_v cf fk(k)
 };
 
 // launch configuration chosen on the host for a grid

@@ -72,6 +72,18 @@ int largest_divisor_le(long n, long cap) {
 
 }  // namespace
 
+// Everything the iteration kernels need in one field precision R: FFT plans
+// with twiddle tables, launch geometry, the closure potential u, the
+// long-range terms, the chi spline table and the spectrum work array.
+struct PrecSet {
+  bool ready = false;
+  AxisPlan plan[3];
+  LaunchGeom geom{};
+  void *u = nullptr, *fr = nullptr, *fkh = nullptr, *chi = nullptr, *Z = nullptr, *tables = nullptr;
+};
+
+template <typename R> constexpr int pidx() { return sizeof(R) == sizeof(double) ? 0 : 1; }
+
 struct rism_handle {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -87,14 +99,12 @@ struct rism_handle {
   std::vector<double> usig, ueps, uq, uxyz;
   rism_opts opts{};
   int hm = 2;               // compiled history class >= nhist
-  size_t rsz = 8;           // sizeof(R)
-  AxisPlan plan[3];
-  LaunchGeom geom{};
+  PrecSet ps[2];            // [0] FP64 (always), [1] FP32 (mixed precision)
   // device memory
   std::vector<DevBuf> bufs;
   double *ulj = nullptr, *de = nullptr, *fr = nullptr, *sr = nullptr;
   double2 *fkh = nullptr, *phase = nullptr;
-  void *uR = nullptr, *frR = nullptr, *fkhR = nullptr, *chiR = nullptr, *Z = nullptr;
+  float *cref = nullptr;    // mixed: reference c_s of the refinement
   double *ts = nullptr, *res = nullptr, *hist_t = nullptr, *hist_r = nullptr;
   double *partials = nullptr, *rms_hist = nullptr, *gfield = nullptr;
   Ctl *ctl = nullptr;
@@ -102,15 +112,23 @@ struct rism_handle {
   double4 *atoms = nullptr;
   double2 *lj = nullptr;
   double *sigh = nullptr, *d_qv = nullptr, *d_rho = nullptr;
-  void *tables = nullptr;
   // state
   bool potentials = false, solved = false, evaluated = false;
   double cf = 1.0;
+  bool delta = false;       // mixed: FP32 passes work on c_s - cref
+  int refreshes = 0;
   int iter = 0;
   int steps = 0;
   double rms = 0.0;
   double t_setup = 0, t_solve = 0, t_thermo = 0;
   std::vector<double> rms_log;
+  // per-kernel HIP-event timing (rism_profile)
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_next = 0;
+  std::vector<std::pair<int, size_t>> ev_marks;  // (kernel id, index of start event)
+  double kt_ms[RISM_NKERNEL] = {0};
+  long kt_n[RISM_NKERNEL] = {0};
 };
 
 #define HIP_OK(...)                                                                        \
@@ -123,6 +141,47 @@ struct rism_handle {
   } while (0)
 
 namespace {
+
+// ---- per-kernel timing: an event pair around each launch on the stream ----
+int prof_mark(rism_handle *h, int kernel) {
+  if (!h->prof) return RISM_OK;
+  while (h->ev_pool.size() < h->ev_next + 2) {
+    hipEvent_t e;
+    HIP_OK(hipEventCreate(&e));
+    h->ev_pool.push_back(e);
+  }
+  HIP_OK(hipEventRecord(h->ev_pool[h->ev_next], h->stream));
+  h->ev_marks.push_back({kernel, h->ev_next});
+  h->ev_next += 2;
+  return RISM_OK;
+}
+
+int prof_done(rism_handle *h) {
+  if (!h->prof) return RISM_OK;
+  HIP_OK(hipEventRecord(h->ev_pool[h->ev_marks.back().second + 1], h->stream));
+  return RISM_OK;
+}
+
+int prof_collect(rism_handle *h) {
+  if (!h->prof || h->ev_marks.empty()) return RISM_OK;
+  HIP_OK(hipStreamSynchronize(h->stream));
+  for (auto &m : h->ev_marks) {
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, h->ev_pool[m.second], h->ev_pool[m.second + 1]));
+    h->kt_ms[m.first] += ms;
+    h->kt_n[m.first] += 1;
+  }
+  h->ev_marks.clear();
+  h->ev_next = 0;
+  return RISM_OK;
+}
+
+#define PROF(h, id, ...)                                  \
+  do {                                                    \
+    if (int _rc = prof_mark((h), (id))) return _rc;       \
+    HIP_OK(__VA_ARGS__);                                  \
+    if (int _rc = prof_done(h)) return _rc;               \
+  } while (0)
 
 int dalloc(rism_handle *h, void **p, size_t bytes) {
   if (bytes == 0) bytes = 16;
@@ -146,6 +205,7 @@ template <typename T> int dalloc_t(rism_handle *h, T **p, size_t count) {
 // per-axis plan + twiddle tables in the iteration precision
 template <typename R> int build_plans(rism_handle *h) {
   using T2 = typename C2<R>::T;
+  PrecSet &ps = h->ps[pidx<R>()];
   const int ns[3] = {h->nx, h->ny, h->nz};
   size_t tot = 0;
   for (int a = 0; a < 3; ++a) tot += 2 * (size_t)ns[a];
@@ -160,7 +220,7 @@ template <typename R> int build_plans(rism_handle *h) {
       set_error("unsupported grid length " + std::to_string(n));
       return RISM_ERR_ARG;
     }
-    AxisPlan &p = h->plan[a];
+    AxisPlan &p = ps.plan[a];
     std::memset(&p, 0, sizeof p);
     p.n = n;
     p.nst = (int)rad.size();
@@ -170,6 +230,9 @@ template <typename R> int build_plans(rism_handle *h) {
       p.s[k] = s;
       p.m[k] = nn / rad[k];
       p.tstep[k] = n / nn;
+      p.ls2[k] = (s & (s - 1)) == 0 ? __builtin_ctz(s) : -1;
+      const int nb = n / rad[k];
+      p.lb2[k] = (nb & (nb - 1)) == 0 ? __builtin_ctz(nb) : -1;
       nn /= rad[k];
       s *= rad[k];
     }
@@ -188,7 +251,7 @@ template <typename R> int build_plans(rism_handle *h) {
     off += 2 * (size_t)n;
   }
   HIP_OK(hipMemcpy(dt, host.data(), tot * sizeof(T2), hipMemcpyHostToDevice));
-  h->tables = dt;
+  ps.tables = dt;
   return RISM_OK;
 }
 
@@ -202,27 +265,27 @@ long env_long(const char *name, long dflt) {
 // per pack.  Every workgroup's LDS (padded lines) stays within RISM_LDS_CAP
 // bytes (64 KiB by default).  RISM_LX / RISM_TY / RISM_TZ override the
 // choices (tuning sweeps).
-int choose_geometry(rism_handle *h) {
-  LaunchGeom &g = h->geom;
+int choose_geometry(rism_handle *h, LaunchGeom &g, size_t rsz) {
   const int nx = h->nx, ny = h->ny, nz = h->nz, P = h->P;
-  const long c2 = 2 * (long)h->rsz;
+  const long c2 = 2 * (long)rsz;
   const long cap = env_long("RISM_LDS_CAP", 65536);
   const long xe = env_long("RISM_X_ELEMS", 1024);
-  g.Lx = largest_divisor_le((long)ny * nz, std::max(1L, std::min(xe / ((long)P * nx), cap / ((long)P * (nx + 1) * c2))));
-  g.Ty = largest_divisor_le(nx, std::max(1L, std::min(8192L / ny, cap / ((ny + 1) * c2))));
-  g.Tz = largest_divisor_le(nx, std::max(1L, std::min(8192L / (2L * P * nz), cap / (2L * P * (nz + 1) * c2))));
+  const long lsx = line_stride(nx), lsy = line_stride(ny), lsz = line_stride(nz);
+  g.Lx = largest_divisor_le((long)ny * nz, std::max(1L, std::min(xe / ((long)P * nx), cap / ((long)P * lsx * c2))));
+  g.Ty = largest_divisor_le(nx, std::max(1L, std::min(8192L / ny, cap / (lsy * c2))));
+  g.Tz = largest_divisor_le(nx, std::max(1L, std::min(8192L / (2L * P * nz), cap / (2L * P * lsz * c2))));
   const long lx = env_long("RISM_LX", 0), ty = env_long("RISM_TY", 0), tz = env_long("RISM_TZ", 0);
   if (lx > 0 && ((long)ny * nz) % lx == 0) g.Lx = (int)lx;
   if (ty > 0 && nx % ty == 0) g.Ty = (int)ty;
   if (tz > 0 && nx % tz == 0) g.Tz = (int)tz;
   g.ept_x = ept_class((long)P * g.Lx * nx);
   g.xgroups = (int)((long)ny * nz / g.Lx);
-  g.xblocks = std::min(g.xgroups, 1024);
-  g.lds_x = (size_t)P * g.Lx * (nx + 1) * c2;
+  g.xblocks = std::min<int>(g.xgroups, (int)env_long("RISM_XBLOCKS", 2048));
+  g.lds_x = (size_t)P * g.Lx * lsx * c2;
   g.ept_y = ept_class((long)g.Ty * ny);
-  g.lds_y = (size_t)g.Ty * (ny + 1) * c2;
+  g.lds_y = (size_t)g.Ty * lsy * c2;
   g.ept_z = ept_class(2L * P * g.Tz * nz);
-  g.lds_z = (size_t)2 * P * g.Tz * (nz + 1) * c2;
+  g.lds_z = (size_t)2 * P * g.Tz * lsz * c2;
   if (g.ept_x < 0 || g.ept_y < 0 || g.ept_z < 0 || (long)g.lds_x > cap || (long)g.lds_y > cap ||
       (long)g.lds_z > cap) {
     set_error("grid too large for the single-workgroup line transforms");
@@ -234,14 +297,15 @@ int choose_geometry(rism_handle *h) {
 template <typename R> IterArgs<R, double> make_args(rism_handle *h, int iter) {
   IterArgs<R, double> a;
   std::memset(&a, 0, sizeof a);
+  const PrecSet &ps = h->ps[pidx<R>()];
   a.nx = h->nx; a.ny = h->ny; a.nz = h->nz; a.N = h->N;
   a.V = h->V; a.P = h->P;
   a.ts = h->ts; a.res = h->res; a.hist_t = h->hist_t; a.hist_r = h->hist_r;
-  a.u = (const R *)h->uR;
-  a.fr = (const R *)h->frR;
-  a.Z = (typename C2<R>::T *)h->Z;
-  a.fkh = (const typename C2<R>::T *)h->fkhR;
-  a.chi = (const typename R4T<R>::T *)h->chiR;
+  a.u = (const R *)ps.u;
+  a.fr = (const R *)ps.fr;
+  a.Z = (typename C2<R>::T *)ps.Z;
+  a.fkh = (const typename C2<R>::T *)ps.fkh;
+  a.chi = (const typename R4T<R>::T *)ps.chi;
   a.ntab = h->ntab;
   a.dk = h->dk;
   for (int k = 0; k < 3; ++k) a.box[k] = h->box[k];
@@ -255,25 +319,67 @@ template <typename R> IterArgs<R, double> make_args(rism_handle *h, int iter) {
   a.ctl = h->ctl;
   a.partials = h->partials;
   a.nv = 1 + h->hm + h->hm * (h->hm + 1) / 2;
-  a.px = h->plan[0]; a.py = h->plan[1]; a.pz = h->plan[2];
-  a.Lx = h->geom.Lx; a.Ty = h->geom.Ty; a.Tz = h->geom.Tz;
-  a.xblocks = h->geom.xblocks;
+  a.px = ps.plan[0]; a.py = ps.plan[1]; a.pz = ps.plan[2];
+  a.Lx = ps.geom.Lx; a.Ty = ps.geom.Ty; a.Tz = ps.geom.Tz;
+  a.xblocks = ps.geom.xblocks;
   a.store_h = 0;
   a.iter = iter;
+  a.use_fk = 1;
+  if (pidx<R>() == 1) {
+    a.cref = h->cref;
+    a.cref_out = h->cref;
+    a.href = (const double2 *)h->ps[0].Z;
+    a.delta = h->delta ? 1 : 0;
+    a.use_fk = h->delta ? 0 : 1;
+  }
   return a;
 }
 
 template <typename R> int launch_iteration(rism_handle *h, int it, bool store_h, bool solve) {
   IterArgs<R, double> a = make_args<R>(h, it);
+  const LaunchGeom &g = h->ps[pidx<R>()].geom;
   a.store_h = store_h ? 1 : 0;
-  HIP_OK(launch_closure_fx<R, double>(a, h->geom, true, h->hm, h->stream));
-  HIP_OK(launch_fft_y<R, double>(a, h->geom, false, h->stream));
-  HIP_OK(launch_fft_z_oz<R, double>(a, h->geom, h->stream));
-  HIP_OK(launch_fft_y<R, double>(a, h->geom, true, h->stream));
-  HIP_OK(launch_fx_resid<R, double>(a, h->geom, h->hm, h->stream));
+  PROF(h, 0, launch_closure_fx<R, double>(a, g, true, h->hm, h->stream));
+  PROF(h, 1, launch_fft_y<R, double>(a, g, false, h->stream));
+  PROF(h, 2, launch_fft_z_oz<R, double>(a, g, h->stream));
+  PROF(h, 3, launch_fft_y<R, double>(a, g, true, h->stream));
+  PROF(h, 4, launch_fx_resid<R, double>(a, g, h->hm, h->stream));
   if (solve)
-    HIP_OK(launch_anderson_solve(h->ctl, h->partials, h->geom.xblocks, a.nv, h->hm, h->rms_hist, it,
-                                 h->stream));
+    PROF(h, 5, launch_anderson_solve(h->ctl, h->partials, g.xblocks, a.nv, h->hm, h->rms_hist, it,
+                                     h->stream));
+  return RISM_OK;
+}
+
+// Mixed precision, refinement step: the FP32 first pass (with the pending
+// Anderson update) stores c_s, rounded to FP32, as the new reference and
+// sends only the rounding remainder through the FP32 passes; the FP64 passes
+// transform the reference into h_ref (long-range term included) and the
+// FP32 residual pass adds it back.  From here on the FP32 passes run in
+// delta mode.
+int launch_refresh_iteration(rism_handle *h, int it) {
+  h->delta = true;
+  ++h->refreshes;
+  IterArgs<float, double> a = make_args<float>(h, it);
+  const LaunchGeom &g = h->ps[1].geom;
+  a.refresh = 1;
+  PROF(h, 0, launch_closure_fx<float, double>(a, g, true, h->hm, h->stream));
+  IterArgs<double, double> d = make_args<double>(h, it);
+  const LaunchGeom &gd = h->ps[0].geom;
+  d.src = 1;
+  d.cref = h->cref;
+  d.href_only = 1;
+  d.store_h = 1;
+  PROF(h, 10, launch_closure_fx<double, double>(d, gd, false, h->hm, h->stream));
+  PROF(h, 10, launch_fft_y<double, double>(d, gd, false, h->stream));
+  PROF(h, 10, launch_fft_z_oz<double, double>(d, gd, h->stream));
+  PROF(h, 10, launch_fft_y<double, double>(d, gd, true, h->stream));
+  PROF(h, 10, launch_fx_resid<double, double>(d, gd, h->hm, h->stream));
+  a.refresh = 0;
+  PROF(h, 1, launch_fft_y<float, double>(a, g, false, h->stream));
+  PROF(h, 2, launch_fft_z_oz<float, double>(a, g, h->stream));
+  PROF(h, 3, launch_fft_y<float, double>(a, g, true, h->stream));
+  PROF(h, 4, launch_fx_resid<float, double>(a, g, h->hm, h->stream));
+  PROF(h, 5, launch_anderson_solve(h->ctl, h->partials, g.xblocks, a.nv, h->hm, h->rms_hist, it, h->stream));
   return RISM_OK;
 }
 
@@ -281,7 +387,14 @@ template <typename R> int launch_iteration(rism_handle *h, int it, bool store_h,
 template <typename R> int stage_potential(rism_handle *h) {
   std::vector<double> q(h->V);
   for (int v = 0; v < h->V; ++v) q[v] = h->qv[v] * h->cf;
-  HIP_OK(launch_combine_u<R>(h->ulj, h->sr, q.data(), h->V, h->N, (R *)h->uR, h->stream));
+  HIP_OK(launch_combine_u<R>(h->ulj, h->sr, q.data(), h->V, h->N, (R *)h->ps[pidx<R>()].u, h->stream));
+  return RISM_OK;
+}
+
+int stage_potentials(rism_handle *h) {
+  if (int rc = stage_potential<double>(h)) return rc;
+  if (h->opts.precision == RISM_PREC_MIXED)
+    if (int rc = stage_potential<float>(h)) return rc;
   return RISM_OK;
 }
 
@@ -408,7 +521,9 @@ template <typename R> int solve_impl(rism_handle *h, const char *tuv_path, int *
   int status = RISM_SOLVE_CONVERGED;
   h->rms_log.clear();
   int it = h->iter;
-  const int chunk = (h->opts.verbose ? 1 : (N >= (1L << 21) ? 4 : 16));
+  // iterations enqueued per control-block read; 1 when timing kernels so no
+  // gated (empty) launches enter the averages
+  const int chunk = (h->opts.verbose || h->prof) ? 1 : (N >= (1L << 21) ? 4 : 16);
   int printed = 0;
   for (int stage = 0; stage <= cu; ++stage) {
     if (stage > 0) {  // add_tuv.cu: t += q_v cuf fr; ts absorbs any clamp remainder
@@ -425,7 +540,11 @@ template <typename R> int solve_impl(rism_handle *h, const char *tuv_path, int *
       if (rem != 0.0) HIP_OK(launch_axpy_fr<double>(h->ts, h->fr, coef.data(), V, N, 1, h->stream));
       HIP_OK(launch_axpy_fr<double>(h->hist_t, h->fr, hcoef.data(), V, N, h->opts.nhist, h->stream));
     }
-    if (int rc = stage_potential<R>(h)) return rc;
+    if (int rc = stage_potentials(h)) return rc;
+    h->delta = false;  // a new stage (new cf) starts without a reference
+    double ref_rms = -1.0, best_rms = HUGE_VAL;
+    int best_at = 0, since_ref = 0;
+    bool want_refresh = false;
     if (verbose) std::printf("relaxing 3D UV RISM: Charge Up Factor = %g\n", h->cf);
     c.status = -1;
     c.step = 0;
@@ -434,9 +553,26 @@ template <typename R> int solve_impl(rism_handle *h, const char *tuv_path, int *
     if (int rc = put_ctl(h)) return rc;
     while (true) {
       const int n_launch = (h->opts.ksave > 0) ? 1 : chunk;
-      for (int k = 0; k < n_launch; ++k, ++it)
-        if (int rc = launch_iteration<R>(h, it, false, true)) return rc;
+      for (int k = 0; k < n_launch; ++k, ++it) {
+        if (pidx<R>() == 1 && want_refresh && k == 0) {
+          if (int rc = launch_refresh_iteration(h, it)) return rc;
+          want_refresh = false;
+          since_ref = 0;
+          ref_rms = -2.0;  // set from the next rms read
+        } else {
+          if (int rc = launch_iteration<R>(h, it, false, true)) return rc;
+        }
+      }
       if (int rc = get_ctl(h)) return rc;
+      if (pidx<R>() == 1 && c.status < 0) {
+        // refinement schedule: a new reference each time the residual has
+        // fallen 10x below the one at the last reference, or when it has
+        // not improved for 24 steps (the FP32 rounding floor)
+        if (ref_rms < 0.0) ref_rms = c.rms;
+        since_ref += n_launch;
+        if (c.rms < best_rms) { best_rms = c.rms; best_at = c.total; }
+        if (c.rms < 0.1 * ref_rms || (since_ref >= 24 && c.total - best_at >= 24)) want_refresh = true;
+      }
       // progress lines (iterate.cu:63)
       if (c.total > printed) {
         std::vector<double> tail(c.total - printed);
@@ -456,7 +592,7 @@ template <typename R> int solve_impl(rism_handle *h, const char *tuv_path, int *
         // the next iteration does not apply it again
         if (c.upd_iter == it) {
           IterArgs<R, double> a = make_args<R>(h, it);
-          HIP_OK(launch_closure_fx<R, double>(a, h->geom, true, h->hm, h->stream));
+          HIP_OK(launch_closure_fx<R, double>(a, h->ps[pidx<R>()].geom, true, h->hm, h->stream));
           c.upd_iter = -1;
           if (int rc = put_ctl(h)) return rc;
         }
@@ -471,7 +607,7 @@ template <typename R> int solve_impl(rism_handle *h, const char *tuv_path, int *
       c.run_iter = -1;  // update only, no closure/FFT
       if (int rc = put_ctl(h)) return rc;
       IterArgs<R, double> a = make_args<R>(h, it);
-      HIP_OK(launch_closure_fx<R, double>(a, h->geom, true, h->hm, h->stream));
+      HIP_OK(launch_closure_fx<R, double>(a, h->ps[pidx<R>()].geom, true, h->hm, h->stream));
       ++it;
     }
     status = c.status;
@@ -486,6 +622,7 @@ template <typename R> int solve_impl(rism_handle *h, const char *tuv_path, int *
     }
   }
   HIP_OK(hipStreamSynchronize(h->stream));
+  if (int rc = prof_collect(h)) return rc;
   h->iter = it;
   h->steps = c.total;
   h->rms = c.rms;
@@ -515,15 +652,16 @@ template <typename R> int evaluate_impl(rism_handle *h) {
   return RISM_OK;
 }
 
-int evaluate(rism_handle *h) {
-  return h->opts.precision == RISM_PREC_MIXED ? evaluate_impl<float>(h) : evaluate_impl<double>(h);
-}
+// The final state is always re-evaluated with the FP64 passes (the mixed
+// path keeps an FP64 field set for its refinement), so h, g and every
+// thermodynamic sum come from FP64 fields.
+int evaluate(rism_handle *h) { return evaluate_impl<double>(h); }
 
 template <typename R> int thermo_impl(rism_handle *h, rism_xmu *x) {
   const int V = h->V;
   const int nblk = 1024;
   IterArgs<R, double> a = make_args<R>(h, h->iter);
-  HIP_OK(launch_exchem<R, double>(a, h->fr, h->partials, nblk, h->stream));
+  PROF(h, 8, launch_exchem<R, double>(a, h->fr, h->partials, nblk, h->stream));
   std::vector<double> part((size_t)nblk * 4 * V);
   HIP_OK(hipMemcpyAsync(part.data(), h->partials, part.size() * sizeof(double), hipMemcpyDeviceToHost,
                         h->stream));
@@ -531,8 +669,8 @@ template <typename R> int thermo_impl(rism_handle *h, rism_xmu *x) {
   std::vector<double> tot(4 * V, 0.0);
   for (int b = 0; b < nblk; ++b)
     for (int k = 0; k < 4 * V; ++k) tot[k] += part[(size_t)b * 4 * V + k];
-  HIP_OK(launch_euv(h->nx, h->ny, h->nz, h->dr, V, h->natu, h->atoms, h->lj, h->sigh, h->d_qv, h->d_rho,
-                    h->gfield, h->partials, nblk, h->stream));
+  PROF(h, 9, launch_euv(h->nx, h->ny, h->nz, h->dr, V, h->natu, h->atoms, h->lj, h->sigh, h->d_qv, h->d_rho,
+                        h->gfield, h->partials, nblk, h->stream));
   std::vector<double> pe((size_t)nblk * 2);
   HIP_OK(hipMemcpyAsync(pe.data(), h->partials, pe.size() * sizeof(double), hipMemcpyDeviceToHost,
                         h->stream));
@@ -579,18 +717,19 @@ template <typename R> int thermo_impl(rism_handle *h, rism_xmu *x) {
 template <typename R> int alloc_precision(rism_handle *h) {
   using T2 = typename C2<R>::T;
   using T4 = typename R4T<R>::T;
+  PrecSet &ps = h->ps[pidx<R>()];
   const size_t VN = (size_t)h->V * h->N;
   if (int rc = build_plans<R>(h)) return rc;
-  if (int rc = choose_geometry(h)) return rc;
-  if (int rc = dalloc(h, &h->uR, VN * sizeof(R))) return rc;
+  if (int rc = choose_geometry(h, ps.geom, sizeof(R))) return rc;
+  if (int rc = dalloc(h, &ps.u, VN * sizeof(R))) return rc;
   if (sizeof(R) == sizeof(double)) {
-    h->frR = h->fr;
-    h->fkhR = h->fkh;
+    ps.fr = h->fr;
+    ps.fkh = h->fkh;
   } else {
-    if (int rc = dalloc(h, &h->frR, h->N * sizeof(R))) return rc;
-    if (int rc = dalloc(h, &h->fkhR, (size_t)h->nx * (h->ny / 2) * h->nz * sizeof(T2))) return rc;
+    if (int rc = dalloc(h, &ps.fr, h->N * sizeof(R))) return rc;
+    if (int rc = dalloc(h, &ps.fkh, (size_t)h->nx * (h->ny / 2) * h->nz * sizeof(T2))) return rc;
   }
-  if (int rc = dalloc(h, &h->Z, (size_t)h->P * h->N * sizeof(T2))) return rc;
+  if (int rc = dalloc(h, &ps.Z, (size_t)h->P * h->N * sizeof(T2))) return rc;
   // chi spline table [V*V][ntab-1] of (y_i, c0, c1, c2)
   const int nt = h->ntab, V = h->V;
   std::vector<double> kt(nt), y(nt), c0(nt), c1(nt), c2(nt);
@@ -608,8 +747,9 @@ template <typename R> int alloc_precision(rism_handle *h) {
         e.w = (R)c2[i];
       }
     }
-  if (int rc = dalloc(h, &h->chiR, tab.size() * sizeof(T4))) return rc;
-  HIP_OK(hipMemcpy(h->chiR, tab.data(), tab.size() * sizeof(T4), hipMemcpyHostToDevice));
+  if (int rc = dalloc(h, &ps.chi, tab.size() * sizeof(T4))) return rc;
+  HIP_OK(hipMemcpy(ps.chi, tab.data(), tab.size() * sizeof(T4), hipMemcpyHostToDevice));
+  ps.ready = true;
   return RISM_OK;
 }
 
@@ -685,7 +825,6 @@ int create_impl(const rism_grid *grid, const rism_solvent *sv, const rism_solute
   h->uxyz.assign(su->xyz, su->xyz + 3 * (size_t)su->natom);
   h->opts = *o;
   h->hm = o->nhist <= 2 ? 2 : (o->nhist <= 5 ? 5 : 10);
-  h->rsz = o->precision == RISM_PREC_MIXED ? sizeof(float) : sizeof(double);
   // k range check (solvent_spline.cu:18-21)
   double kmax2 = 0.0;
   for (int k = 0; k < 3; ++k) {
@@ -710,12 +849,19 @@ int create_impl(const rism_grid *grid, const rism_solvent *sv, const rism_solute
       (rc = dalloc_t(h, &h->sigh, (size_t)h->V * std::max(1, h->natu))) || (rc = dalloc_t(h, &h->d_qv, h->V)) ||
       (rc = dalloc_t(h, &h->d_rho, h->V)))
     return rc;
-  const size_t npart = std::max<size_t>((size_t)1024 * 4 * kMaxV, (size_t)1024 * 66);
-  if ((rc = dalloc_t(h, &h->partials, npart))) return rc;
   HIP_OK(hipHostMalloc((void **)&h->ctl_host, sizeof(Ctl), hipHostMallocDefault));
   std::memset(h->ctl_host, 0, sizeof(Ctl));
-  rc = (o->precision == RISM_PREC_MIXED) ? alloc_precision<float>(h) : alloc_precision<double>(h);
-  if (rc) return rc;
+  if ((rc = alloc_precision<double>(h))) return rc;
+  if (o->precision == RISM_PREC_MIXED) {
+    if ((rc = alloc_precision<float>(h))) return rc;
+    if ((rc = dalloc_t(h, &h->cref, VN))) return rc;
+  }
+  {  // per-workgroup partial sums: residual pass and thermodynamic reductions
+    const size_t nv = 1 + h->hm + h->hm * (h->hm + 1) / 2;
+    const size_t nb = std::max(h->ps[0].geom.xblocks, h->ps[1].ready ? h->ps[1].geom.xblocks : 0);
+    const size_t npart = std::max<size_t>((size_t)1024 * 4 * kMaxV, nb * nv);
+    if ((rc = dalloc_t(h, &h->partials, npart))) return rc;
+  }
   // solute records and Lorentz-Berthelot pair parameters (cal_LJ.cu:24-31)
   std::vector<double4> at(std::max(1, h->natu));
   std::vector<double2> lj((size_t)h->V * std::max(1, h->natu));
@@ -786,13 +932,14 @@ int rism_build_potentials(rism_handle *h) {
   a.ikbt = 1.0 / (kAvogadro * kBoltzmann * h->temper);
   a.ubeta = kHartree * kBohr / (kBoltzmann * h->temper);
   a.ulj = h->ulj; a.de = h->de; a.fr = h->fr; a.sr = h->sr;
-  HIP_OK(launch_potentials(a, h->stream));
-  HIP_OK(launch_fk_half(h->nx, h->ny, h->nz, h->natu, h->atoms, h->box, a.ubeta, h->phase, h->fkh, h->stream));
+  PROF(h, 6, launch_potentials(a, h->stream));
+  PROF(h, 7, launch_fk_half(h->nx, h->ny, h->nz, h->natu, h->atoms, h->box, a.ubeta, h->phase, h->fkh, h->stream));
   if (h->opts.precision == RISM_PREC_MIXED) {
-    HIP_OK(launch_convert<float>(h->fr, (float *)h->frR, h->N, h->stream));
-    HIP_OK(launch_convert_c<float>(h->fkh, (float2 *)h->fkhR, (long)h->nx * (h->ny / 2) * h->nz, h->stream));
+    HIP_OK(launch_convert<float>(h->fr, (float *)h->ps[1].fr, h->N, h->stream));
+    HIP_OK(launch_convert_c<float>(h->fkh, (float2 *)h->ps[1].fkh, (long)h->nx * (h->ny / 2) * h->nz, h->stream));
   }
   HIP_OK(hipStreamSynchronize(h->stream));
+  if (int rc = prof_collect(h)) return rc;
   h->potentials = true;
   h->solved = h->evaluated = false;
   h->t_setup += now_s() - t0;
@@ -814,7 +961,8 @@ int rism_thermo(rism_handle *h, rism_xmu *out) {
   const double t0 = now_s();
   if (!h->evaluated)
     if (int rc = evaluate(h)) return rc;
-  int rc = h->opts.precision == RISM_PREC_MIXED ? thermo_impl<float>(h, out) : thermo_impl<double>(h, out);
+  int rc = thermo_impl<double>(h, out);
+  if (rc == RISM_OK) rc = prof_collect(h);
   h->t_thermo = now_s() - t0;
   return rc;
 }
@@ -913,14 +1061,9 @@ int rism_get_field(rism_handle *h, int which, double *out) {
         return RISM_OK;
       }
       std::vector<double> hh(VN);
-      if (h->opts.precision == RISM_PREC_MIXED) {
-        std::vector<float2> z((size_t)h->P * N);
-        HIP_OK(hipMemcpy(z.data(), h->Z, z.size() * sizeof(float2), hipMemcpyDeviceToHost));
-        for (int v = 0; v < V; ++v)
-          for (long i = 0; i < N; ++i) hh[(size_t)v * N + i] = (v & 1) ? z[(size_t)(v >> 1) * N + i].y : z[(size_t)(v >> 1) * N + i].x;
-      } else {
-        std::vector<double2> z((size_t)h->P * N);
-        HIP_OK(hipMemcpy(z.data(), h->Z, z.size() * sizeof(double2), hipMemcpyDeviceToHost));
+      {
+        std::vector<double2> z((size_t)h->P * N);  // FP64 evaluation result
+        HIP_OK(hipMemcpy(z.data(), h->ps[0].Z, z.size() * sizeof(double2), hipMemcpyDeviceToHost));
         for (int v = 0; v < V; ++v)
           for (long i = 0; i < N; ++i) hh[(size_t)v * N + i] = (v & 1) ? z[(size_t)(v >> 1) * N + i].y : z[(size_t)(v >> 1) * N + i].x;
       }
@@ -983,9 +1126,37 @@ void rism_destroy(rism_handle *h) {
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   for (auto &b : h->bufs) (void)hipFree(b.p);
+  for (auto &e : h->ev_pool) (void)hipEventDestroy(e);
   if (h->ctl_host) (void)hipHostFree(h->ctl_host);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
+}
+
+int rism_profile(rism_handle *h, int enable) {
+  if (!h) { set_error("null handle"); return RISM_ERR_ARG; }
+  HIP_OK(hipSetDevice(h->device));
+  if (int rc = prof_collect(h)) return rc;
+  h->prof = enable != 0;
+  for (int k = 0; k < RISM_NKERNEL; ++k) { h->kt_ms[k] = 0.0; h->kt_n[k] = 0; }
+  return RISM_OK;
+}
+
+int rism_kernel_times(rism_handle *h, double *ms, long *count) {
+  if (!h) { set_error("null handle"); return RISM_ERR_ARG; }
+  if (int rc = prof_collect(h)) return rc;
+  for (int k = 0; k < RISM_NKERNEL; ++k) {
+    if (ms) ms[k] = h->kt_ms[k];
+    if (count) count[k] = h->kt_n[k];
+  }
+  return RISM_OK;
+}
+
+int rism_geometry(rism_handle *h, int *out) {
+  if (!h || !out) { set_error("null argument"); return RISM_ERR_ARG; }
+  const LaunchGeom &g = h->ps[h->opts.precision == RISM_PREC_MIXED ? 1 : 0].geom;
+  const int v[10] = {g.Lx, g.Ty, g.Tz, g.ept_x, g.ept_y, g.ept_z, g.xgroups, g.xblocks, h->P, h->hm};
+  for (int k = 0; k < 10; ++k) out[k] = v[k];
+  return RISM_OK;
 }
 
 int rism_fft3d(rism_handle *h, void *data, int dir) {
@@ -993,10 +1164,10 @@ int rism_fft3d(rism_handle *h, void *data, int dir) {
   HIP_OK(hipSetDevice(h->device));
   hipError_t e;
   if (h->opts.precision == RISM_PREC_MIXED)
-    e = launch_fft3d<float>((float2 *)data, h->plan[0], h->plan[1], h->plan[2], h->geom, h->nx, h->ny, h->nz,
+    e = launch_fft3d<float>((float2 *)data, h->ps[1].plan[0], h->ps[1].plan[1], h->ps[1].plan[2], h->ps[1].geom, h->nx, h->ny, h->nz,
                             h->dv, 1.0 / h->volume, dir, h->stream);
   else
-    e = launch_fft3d<double>((double2 *)data, h->plan[0], h->plan[1], h->plan[2], h->geom, h->nx, h->ny, h->nz,
+    e = launch_fft3d<double>((double2 *)data, h->ps[0].plan[0], h->ps[0].plan[1], h->ps[0].plan[2], h->ps[0].geom, h->nx, h->ny, h->nz,
                              h->dv, 1.0 / h->volume, dir, h->stream);
   HIP_OK(e);
   HIP_OK(hipStreamSynchronize(h->stream));
@@ -1014,7 +1185,8 @@ int rism_calculate(rism_handle *h, const double *t_dev, double *res_dev) {
   std::vector<double> coef(h->V);
   for (int v = 0; v < h->V; ++v) coef[v] = -h->qv[v];
   HIP_OK(launch_axpy_fr<double>(h->ts, h->fr, coef.data(), h->V, h->N, 1, h->stream));
-  int rc = (h->opts.precision == RISM_PREC_MIXED) ? stage_potential<float>(h) : stage_potential<double>(h);
+  h->delta = false;
+  int rc = stage_potentials(h);
   if (rc) return rc;
   Ctl &c = *h->ctl_host;
   std::memset(&c, 0, sizeof c);

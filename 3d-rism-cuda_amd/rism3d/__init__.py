@@ -33,8 +33,11 @@ EXPORTED = [
     "rism_deck_views", "rism_xvk_read", "rism_create", "rism_build_potentials", "rism_solve",
     "rism_thermo", "rism_grad", "rism_get_field", "rism_set_t", "rism_save_tuv", "rism_load_tuv",
     "rism_write_xmu", "rism_write_gra", "rism_stats", "rism_rms_history", "rism_destroy",
-    "rism_fft3d", "rism_calculate",
+    "rism_fft3d", "rism_calculate", "rism_profile", "rism_kernel_times", "rism_geometry",
 ]
+
+KERNELS = ["closure_fx", "fft_y_fwd", "fft_z_oz", "fft_y_inv", "resid", "solve", "potentials", "fk",
+           "exchem", "euv", "refresh_fp64"]
 
 
 class Grid(C.Structure):
@@ -94,6 +97,13 @@ def lib():
     """Load librism3d_amd.so (raises if it has not been built)."""
     global _lib
     if _lib is None:
+        # torch bundles its own libamdhip64.so.7 with the same SONAME as the
+        # system one this library links; loading torch first makes both share
+        # one HIP runtime (loading ours first leaves torch without devices)
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} not built: run __graft_entry__.build() or make -C 3d-rism-cuda_amd")
         L = C.CDLL(LIB_PATH)
@@ -123,6 +133,9 @@ def lib():
         L.rism_destroy.restype = None
         L.rism_fft3d.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
         L.rism_calculate.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.rism_profile.argtypes = [C.c_void_p, C.c_int]
+        L.rism_kernel_times.argtypes = [C.c_void_p, dp, P(C.c_long)]
+        L.rism_geometry.argtypes = [C.c_void_p, P(C.c_int)]
         _lib = L
     return _lib
 
@@ -335,6 +348,23 @@ class Solver:
         out = np.zeros(max(n, 1))
         lib().rism_rms_history(self.h, _dptr(out), n)
         return out[:n]
+
+    def profile(self, enable: bool = True):
+        """Start (and reset) or stop HIP-event timing of every kernel launch."""
+        _check(lib().rism_profile(self.h, int(enable)), "rism_profile")
+
+    def kernel_times(self) -> dict:
+        """{kernel: (total_ms, launches)} since profile(True)."""
+        ms = (C.c_double * len(KERNELS))()
+        n = (C.c_long * len(KERNELS))()
+        _check(lib().rism_kernel_times(self.h, ms, n), "rism_kernel_times")
+        return {k: (ms[i], n[i]) for i, k in enumerate(KERNELS)}
+
+    def geometry(self) -> dict:
+        g = (C.c_int * 10)()
+        _check(lib().rism_geometry(self.h, g), "rism_geometry")
+        keys = ["Lx", "Ty", "Tz", "ept_x", "ept_y", "ept_z", "xgroups", "xblocks", "packs", "hist_class"]
+        return dict(zip(keys, list(g)))
 
     def fft3d(self, data_ptr: int, direction: int):
         """In-place transform of caller device memory (FFT3D::execute seam)."""

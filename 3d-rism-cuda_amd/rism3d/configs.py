@@ -1,0 +1,142 @@
+"""Seeded synthetic workloads (solutes, grids) for the benchmark configurations.
+
+Every generator uses numpy.random.default_rng(20220301 + index), so the same
+index always yields the same molecule.  LJ epsilon is drawn in kcal/mol and
+converted to J/mol (x 4184, src-cuda/physical.h:17); coordinates are centred
+with the reference's integer-rounded shift (src-cuda/solute.cu:13-43).
+
+  c0  64^3 / 32 A box, 16 atoms: random walk of 1.5 A steps kept within 6 A
+      of the centre; sigma U[3.0,3.8] A, eps U[0.05,0.2] kcal/mol,
+      q ~ N(0, 0.4) shifted to sum 0; 5 Anderson histories.
+  c1  256^3 / 128 A box, 846 atoms: compact self-avoiding chain of 3.8 A
+      steps confined to a sphere (radius of gyration ~14 A), same parameter
+      distributions, net charge -4.
+  c2  128^3 / 64 A box, 200 atoms (same chain model, net charge 0).
+  c4  512^3 / 256 A box, 5000 atoms, net charge -12.
+"""
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import Solute
+
+SEED0 = 20220301
+KCAL = 4184.0
+
+
+@dataclass
+class Workload:
+    name: str
+    n: tuple
+    box: tuple
+    solute: Solute
+    nhist: int = 2
+
+
+def centre(xyz: np.ndarray) -> np.ndarray:
+    lo, hi = xyz.min(0), xyz.max(0)
+    shift = np.round(-(hi - lo) / 2 - lo)
+    return xyz + shift
+
+
+def _params(rng, n, net):
+    sig = rng.uniform(3.0, 3.8, n)
+    eps = rng.uniform(0.05, 0.2, n) * KCAL
+    q = rng.normal(0.0, 0.4, n)
+    q = q - (q.sum() - net) / n
+    return sig, eps, q
+
+
+def random_walk(rng, n, step, radius):
+    """Random walk of fixed step length, each step redrawn until it stays
+    within `radius` of the origin."""
+    xyz = np.zeros((n, 3))
+    for i in range(1, n):
+        while True:
+            d = rng.normal(size=3)
+            p = xyz[i - 1] + step * d / np.linalg.norm(d)
+            if np.linalg.norm(p) <= radius:
+                xyz[i] = p
+                break
+    return xyz
+
+
+def compact_chain(rng, n, step=3.8, radius=18.0, min_dist=2.2, tries=200, max_rounds=200000):
+    """Self-avoiding chain of fixed bond length confined to a sphere: each new
+    bead is drawn on the sphere of radius `step` around the previous one and
+    accepted if inside the confining radius and at least `min_dist` from all
+    earlier beads; when no draw succeeds the chain backs up a few beads."""
+    xyz = np.zeros((n, 3))
+    i = 1
+    fails = 0
+    rounds = 0
+    while i < n:
+        rounds += 1
+        if rounds > max_rounds:
+            raise RuntimeError("compact_chain: packing too dense for these parameters")
+        ok = False
+        for _ in range(tries):
+            d = rng.normal(size=3)
+            p = xyz[i - 1] + step * d / np.linalg.norm(d)
+            if np.linalg.norm(p) > radius:
+                continue
+            if i > 1 and np.min(np.linalg.norm(xyz[: i - 1] - p, axis=1)) < min_dist:
+                continue
+            xyz[i] = p
+            ok = True
+            break
+        if ok:
+            i += 1
+            fails = 0
+        else:
+            fails += 1
+            i = max(1, i - min(4 * fails, 20))
+    return xyz
+
+
+def radius_of_gyration(xyz):
+    c = xyz - xyz.mean(0)
+    return float(np.sqrt((c * c).sum(1).mean()))
+
+
+def config(index: int) -> Workload:
+    rng = np.random.default_rng(SEED0 + index)
+    if index == 0:
+        xyz = random_walk(rng, 16, 1.5, 6.0)
+        sig, eps, q = _params(rng, 16, 0.0)
+        return Workload("c0: 64^3 / 32 A, 16-atom random walk, KH", (64, 64, 64), (32.0, 32.0, 32.0),
+                        Solute(sig, eps, q, centre(xyz)), nhist=5)
+    if index == 1:
+        xyz = compact_chain(rng, 846, radius=18.0)
+        sig, eps, q = _params(rng, 846, -4.0)
+        return Workload("c1: 256^3 / 128 A, 846-atom compact chain (net -4), KH", (256, 256, 256),
+                        (128.0, 128.0, 128.0), Solute(sig, eps, q, centre(xyz)))
+    if index == 2:
+        xyz = compact_chain(rng, 200, radius=11.0)
+        sig, eps, q = _params(rng, 200, 0.0)
+        return Workload("c2: 128^3 / 64 A, 200-atom chain, KH", (128, 128, 128), (64.0, 64.0, 64.0),
+                        Solute(sig, eps, q, centre(xyz)))
+    if index == 4:
+        xyz = compact_chain(rng, 5000, radius=33.0)
+        sig, eps, q = _params(rng, 5000, -12.0)
+        return Workload("c4: 512^3 / 256 A, 5000-atom chain (net -12), KH", (512, 512, 512),
+                        (256.0, 256.0, 256.0), Solute(sig, eps, q, centre(xyz)), nhist=10)
+    raise ValueError(f"no single-solute workload {index}")
+
+
+def write_inp(path: str, w: Workload, solvent_file: str, tol=1e-6, maxstep=10000, count=2, m=0.6,
+              mp=0.2, closure="KH"):
+    """Write a workload as a reference-format .inp deck (read_input.cu:21-89)."""
+    with open(path, "w") as f:
+        f.write(" m    0\t 20220301\n")
+        f.write(f" {closure}\n {solvent_file}\n {tol:g}   {maxstep}\n {count}   {m}   {mp}\n")
+        f.write("  {:.4f} {:.4f} {:.4f}\n".format(*w.box))
+        f.write("  {} {} {}\n".format(*w.n))
+        s = w.solute
+        f.write(f" {len(s.q)}\n")
+        for i in range(len(s.q)):
+            f.write("{:.10f} {:.10f} {:.10f} {:.10f} {:.10f} {:.10f}\n".format(
+                s.sig[i], s.eps[i], s.q[i], *s.xyz[i]))

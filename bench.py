@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""bench.py -- time-to-converge of the 256^3 / 128 A / 846-atom KH 3D-RISM solve
+on one MI355X, mixed precision (headline `value`) against the FP64 path.
+
+A step is one complete job on inputs already resident in HBM: grid potentials,
+the fixed-point solve from the initial estimate to RMS residual <= 1e-6, and
+the thermodynamic reductions (Delta-mu, PMV, SE).  Warm-up steps run first,
+then `--steps` timed steps bracketed by device synchronisation.
+
+The JSON line also carries:
+  roofline      the dominant iteration kernel of the mixed path: algorithmic
+                bytes per launch / average launch time (HIP events around every
+                launch on the solver's stream, collected during one extra
+                profiled step), against the 8 TB/s HBM peak;
+  cpu_baseline  the FP64 host oracle (OpenMP) on a bounded sample of the same
+                workload: potentials plus a few iterations, scaled to the
+                FP64 step count of a full solve.
+Solvent: the reference's example_rmdft/h2o_vv.xvk (16384-point TIP3P-like
+water chi, 298.15 K) committed under tests/golden.
+"""
+import argparse
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(REPO, "3d-rism-cuda_amd"), REPO]
+
+import numpy as np  # noqa: E402
+import torch  # noqa: E402,F401  (first: its HIP runtime then serves the solver library too)
+
+METRIC = "s-to-converge 256^3 KH, mixed vs FP64 at Δμ≤0.01 kcal/mol; % HBM roofline"
+SOLVENT = os.path.join(REPO, "tests", "golden", "reference_examples", "h2o_vv_16384.xvk")
+HBM_PEAK_GBS = 8000.0
+
+
+def kernel_bytes(V, P, H, s_t, s_f):
+    """Algorithmic HBM bytes per grid point for each iteration kernel
+    (each operand read once, each result written once, at stored precision;
+    steady-state Anderson step with H filled history slots)."""
+    c = 2 * s_f
+    return {
+        # t, R, H (t_i, R_i), u in; t, slot (t, R), spectrum out
+        "closure_fx": V * (s_t + s_t + 2 * H * s_t + s_f) + V * (3 * s_t) + P * c,
+        "fft_y_fwd": 2 * P * c,
+        # spectrum in/out + the half-grid long-range term
+        "fft_z_oz": 2 * P * c + s_f,
+        "fft_y_inv": 2 * P * c,
+        # spectrum, t, u, H history R_i in; R out
+        "resid": P * c + V * (s_t + s_f + H * s_t) + V * s_t,
+    }
+
+
+def run_job(s):
+    s.build_potentials()
+    st, steps, rms = s.solve()
+    x = s.thermo()
+    return st, steps, rms, x
+
+
+def time_path(rism3d, w, solvent, prec, steps, warmup):
+    import torch
+    opts = rism3d.Options(precision=prec, nhist=w.nhist)
+    s = rism3d.Solver(w.n, w.box, solvent, w.solute, opts)
+    for _ in range(warmup):
+        run_job(s)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = None
+    for _ in range(steps):
+        res = run_job(s)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    st, nit, rms, x = res
+    stats = s.stats()
+    s.profile(True)
+    run_job(s)
+    kt = s.kernel_times()
+    s.profile(False)
+    geo = s.geometry()
+    s.close()
+    return {"s": dt, "status": st, "iterations": nit, "rms": rms, "xmu": x, "stats": stats,
+            "kernels": kt, "geometry": geo}
+
+
+def cpu_baseline(w, fp64_iterations, iters=3):
+    """FP64 oracle on the same deck: set-up + `iters` iterations, scaled."""
+    sys.path.insert(0, REPO)
+    from rism3d import configs
+    from oracle import oracle as orc
+    if not os.path.exists(orc.EXE):
+        return None
+    d = tempfile.mkdtemp()
+    try:
+        shutil.copy(SOLVENT, os.path.join(d, "h2o_vv.xvk"))
+        configs.write_inp(os.path.join(d, "c1.inp"), w, "h2o_vv.xvk")
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        r = subprocess.run([orc.EXE, "-q", "-j", "-f", "-n", str(iters), "-T", str(threads), "c1.inp"],
+                           cwd=d, capture_output=True, text=True, timeout=900)
+        rec = json.loads(r.stdout.strip().splitlines()[-1])
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    per_it = rec["t_iter"] / rec["steps"]
+    total = rec["t_setup"] + per_it * fp64_iterations + rec["t_thermo"]
+    return {"value": round(total, 3), "unit": "s", "cores": rec["threads"], "kind": "port",
+            "sample": (f"oracle on the same 256^3 deck: potentials ({rec['t_setup']:.1f} s) + "
+                       f"{rec['steps']} iterations ({per_it:.2f} s each) + thermo; scaled to the "
+                       f"{fp64_iterations} iterations of the FP64 solve")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline sample")
+    ap.add_argument("--mixed-only", action="store_true", help="time only the mixed path")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU: this build has no multi-GPU path "
+              f"(--gpus {args.gpus} requested).", file=sys.stderr)
+        sys.exit(2)
+    import rism3d
+    from rism3d import configs
+    lib = rism3d.lib()
+    if lib.rism_device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    w = configs.config(1)
+    solvent = rism3d.read_xvk(SOLVENT)
+    V = solvent.nsite
+    P = (V + 1) // 2
+    N = int(np.prod(w.n))
+    mixed = time_path(rism3d, w, solvent, rism3d.MIXED, args.steps, args.warmup)
+    fp64 = None if args.mixed_only else time_path(rism3d, w, solvent, rism3d.FP64, args.steps, args.warmup)
+    # roofline: dominant iteration kernel of the mixed path
+    kb = kernel_bytes(V, P, w.nhist, 8, 4)
+    it_kernels = {k: v for k, v in mixed["kernels"].items() if k in kb and v[1] > 0}
+    dom = max(it_kernels, key=lambda k: it_kernels[k][0])
+    ms, n = it_kernels[dom]
+    avg_s = ms / n / 1e3
+    bytes_launch = kb[dom] * N
+    achieved = bytes_launch / avg_s / 1e9
+    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_launch": bytes_launch, "avg_launch_us": round(avg_s * 1e6, 2)}
+    per_kernel = {}
+    for k, (kms, kn) in mixed["kernels"].items():
+        if kn:
+            e = {"avg_us": round(kms / kn * 1e3, 2), "launches": kn}
+            if k in kb:
+                e["GBps"] = round(kb[k] * N / (kms / kn / 1e3) / 1e9, 1)
+            per_kernel[k] = e
+    cpu = None
+    if not args.no_cpu and fp64 is not None:
+        try:
+            cpu = cpu_baseline(w, fp64["iterations"])
+        except Exception as e:  # the baseline is reported, never required
+            cpu = {"value": None, "error": str(e)[:200]}
+    out = {
+        "metric": METRIC,
+        "value": round(mixed["s"], 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed["s"] * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 fields / f64 state",
+        "data": "synthetic",
+        "config": {"workload": w.name, "grid": list(w.n), "box_A": list(w.box), "atoms": len(w.solute.q),
+                   "solvent": "water (reference example_rmdft/h2o_vv.xvk chi)", "closure": "KH",
+                   "tol": 1e-6, "anderson": {"nhist": w.nhist, "count": 2, "m": 0.6, "mp": 0.2},
+                   "parallelism": "1 GPU"},
+        "iterations_mixed": mixed["iterations"],
+        "rms_mixed": mixed["rms"],
+        "roofline": roof,
+        "kernels_mixed": per_kernel,
+        "cpu_baseline": cpu,
+    }
+    if fp64 is not None:
+        out.update({
+            "fp64_s": round(fp64["s"], 4),
+            "iterations_fp64": fp64["iterations"],
+            "speedup_mixed_vs_fp64": round(fp64["s"] / mixed["s"], 3),
+            "dmu_mixed_minus_fp64_J_mol": round(mixed["xmu"]["SFE_SC"] - fp64["xmu"]["SFE_SC"], 4),
+            "sfe_sc_fp64_J_mol": fp64["xmu"]["SFE_SC"],
+            "pmv_fp64_L_mol": fp64["xmu"]["PMV"],
+            "kernels_fp64": {k: {"avg_us": round(v[0] / v[1] * 1e3, 2), "launches": v[1]}
+                             for k, v in fp64["kernels"].items() if v[1]},
+        })
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -191,6 +191,29 @@ int rism_stats(rism_handle *h, int *steps, double *rms, double *t_setup,
 int rism_rms_history(rism_handle *h, double *out, int n);
 void rism_destroy(rism_handle *h);
 
+/* ---- measurement (no counterpart in the reference, which has no timers) */
+/* kernel ids for rism_kernel_times */
+#define RISM_K_CLOSURE_FX 0   /* Anderson update + closure + forward x FFT */
+#define RISM_K_FFT_Y_FWD 1    /* forward y FFT */
+#define RISM_K_FFT_Z_OZ 2     /* z FFT + k-space OZ mixing + inverse z FFT */
+#define RISM_K_FFT_Y_INV 3    /* inverse y FFT */
+#define RISM_K_RESID 4        /* inverse x FFT + residual + Anderson dots */
+#define RISM_K_SOLVE 5        /* single-workgroup reduction + Anderson solve */
+#define RISM_K_POT 6          /* LJ + Coulomb grid potentials */
+#define RISM_K_FK 7           /* long-range k-space term */
+#define RISM_K_EXCHEM 8       /* free-energy / PMV sums */
+#define RISM_K_EUV 9          /* solute-solvent energy */
+#define RISM_K_REFRESH 10     /* mixed precision: FP64 passes of a refinement step */
+#define RISM_NKERNEL 11
+/* enable (1) or disable (0) HIP-event timing of every launch; resets the
+   accumulators.  While enabled the solver reads its control block after
+   every iteration. */
+int rism_profile(rism_handle *h, int enable);
+/* accumulated milliseconds and launch counts per kernel id */
+int rism_kernel_times(rism_handle *h, double *ms, long *count);
+/* launch geometry: Lx, Ty, Tz, ept_x, ept_y, ept_z, xgroups, xblocks, packs, history class */
+int rism_geometry(rism_handle *h, int *out /* [10] */);
+
 /* ---- hot-path seams on caller device memory --------------------------- */
 /* FFT3D::execute (fft3d.cu:22-38): in-place transform of one x-fastest
    complex field with the reference's convention (dir -1: x dv with the
