@@ -184,65 +184,64 @@ __host__ __device__ constexpr int line_stride(int n) { return n + (n >> 4) + 1; 
 // ---------------------------------------------------------------------
 template <int F, int KMAX, bool INV, typename T2>
 __device__ __forceinline__ void fft_stage(T2 *lds, int nlines, int ls, int n, int s, int m,
-                                          int tstep, int ls2, int lb2, const T2 *__restrict__ tw) {
+                                          int tstep, int ls2, int lb2, const T2 *__restrict__ tw, int tid) {
   const int nbl = n / F;
   const int total = nlines * nbl;
   T2 v[KMAX][F];
+  T2 w[KMAX][F];
   int lbase[KMAX], q0[KMAX], ppv[KMAX];
+  // twiddles first (independent of the data), then the LDS reads
 #pragma unroll
   for (int k = 0; k < KMAX; ++k) {
-    const int bf = threadIdx.x + k * kThreads;
+    const int bf = tid + k * kThreads;
     lbase[k] = -1;
     if (bf < total) {
       const int line = lb2 >= 0 ? (bf >> lb2) : bf / nbl;
       const int j = bf - line * nbl;
       const int pp = ls2 >= 0 ? (j >> ls2) : j / s;
-      const int q = j - pp * s;
       lbase[k] = line * ls;
-      q0[k] = q;
+      q0[k] = j - pp * s;
       ppv[k] = pp;
 #pragma unroll
-      for (int a = 0; a < F; ++a) v[k][a] = lds[lbase[k] + lpad(q + s * (pp + a * m))];
+      for (int b = 1; b < F; ++b) w[k][b] = tw[pp * b * tstep];
     }
   }
+#pragma unroll
+  for (int k = 0; k < KMAX; ++k)
+    if (lbase[k] >= 0)
+#pragma unroll
+      for (int a = 0; a < F; ++a) v[k][a] = lds[lbase[k] + lpad(q0[k] + s * (ppv[k] + a * m))];
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < KMAX; ++k) {
     if (lbase[k] >= 0) {
       Dft<F, INV, T2>::run(v[k]);
-      const int pp = ppv[k];
-      const int o = q0[k] + s * (F * pp);
+      const int o = q0[k] + s * (F * ppv[k]);
       lds[lbase[k] + lpad(o)] = v[k][0];
-      if (pp == 0) {
 #pragma unroll
-        for (int b = 1; b < F; ++b) lds[lbase[k] + lpad(o + s * b)] = v[k][b];
-      } else {
-#pragma unroll
-        for (int b = 1; b < F; ++b) {
-          T2 w = tw[pp * b * tstep];
-          lds[lbase[k] + lpad(o + s * b)] = INV ? cmulc(v[k][b], w) : cmul(v[k][b], w);
-        }
-      }
+      for (int b = 1; b < F; ++b)
+        lds[lbase[k] + lpad(o + s * b)] = INV ? cmulc(v[k][b], w[k][b]) : cmul(v[k][b], w[k][b]);
     }
   }
   __syncthreads();
 }
 
 // Run the full plan on nlines lines of length p.n.  EPT = upper bound of
-// elements per thread (nlines * n <= kThreads * EPT).
+// elements per thread (nlines * n <= kThreads * EPT).  tid is the lane's
+// thread index (a caller may pass an opaque copy of threadIdx.x).
 template <int EPT, bool INV, typename T2>
-__device__ __forceinline__ void fft_lines(T2 *lds, int nlines, int ls, const AxisPlan &p) {
+__device__ __forceinline__ void fft_lines(T2 *lds, int nlines, int ls, const AxisPlan &p, int tid = threadIdx.x) {
   const T2 *__restrict__ tw = (const T2 *)p.tw;
   for (int st = 0; st < p.nst; ++st) {
     const int F = p.rad[st], s = p.s[st], m = p.m[st], ts = p.tstep[st];
     const int a2 = p.ls2[st], b2 = p.lb2[st];
     switch (F) {
-      case 16: fft_stage<16, (EPT + 15) / 16, INV>(lds, nlines, ls, p.n, s, m, ts, a2, b2, tw); break;
-      case 8: fft_stage<8, (EPT + 7) / 8, INV>(lds, nlines, ls, p.n, s, m, ts, a2, b2, tw); break;
-      case 4: fft_stage<4, (EPT + 3) / 4, INV>(lds, nlines, ls, p.n, s, m, ts, a2, b2, tw); break;
-      case 2: fft_stage<2, (EPT + 1) / 2, INV>(lds, nlines, ls, p.n, s, m, ts, a2, b2, tw); break;
-      case 3: fft_stage<3, (EPT + 2) / 3, INV>(lds, nlines, ls, p.n, s, m, ts, a2, b2, tw); break;
-      case 5: fft_stage<5, (EPT + 4) / 5, INV>(lds, nlines, ls, p.n, s, m, ts, a2, b2, tw); break;
+      case 16: fft_stage<16, (EPT + 15) / 16, INV>(lds, nlines, ls, p.n, s, m, ts, a2, b2, tw, tid); break;
+      case 8: fft_stage<8, (EPT + 7) / 8, INV>(lds, nlines, ls, p.n, s, m, ts, a2, b2, tw, tid); break;
+      case 4: fft_stage<4, (EPT + 3) / 4, INV>(lds, nlines, ls, p.n, s, m, ts, a2, b2, tw, tid); break;
+      case 2: fft_stage<2, (EPT + 1) / 2, INV>(lds, nlines, ls, p.n, s, m, ts, a2, b2, tw, tid); break;
+      case 3: fft_stage<3, (EPT + 2) / 3, INV>(lds, nlines, ls, p.n, s, m, ts, a2, b2, tw, tid); break;
+      case 5: fft_stage<5, (EPT + 4) / 5, INV>(lds, nlines, ls, p.n, s, m, ts, a2, b2, tw, tid); break;
       default: break;
     }
   }

@@ -28,6 +28,17 @@ template <typename R> __device__ __forceinline__ R closure_g(R d, int clos) {
   return exp(d);                                         // hnc, calculate_cuda.cu:14-20
 }
 
+// Element access relative to a wave-uniform base with a 32-bit lane offset:
+// the compiler keeps the base in scalar registers and issues the
+// base+offset addressing form (one VGPR per offset instead of a 64-bit
+// address per load).  e * sizeof(T) must stay below 4 GiB.
+template <typename T> __device__ __forceinline__ const T &lane_at(const T *base, unsigned e) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + e * (unsigned)sizeof(T));
+}
+template <typename T> __device__ __forceinline__ T &lane_at(T *base, unsigned e) {
+  return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + e * (unsigned)sizeof(T));
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -35,92 +46,198 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // ---------------------------------------------------------------------
-// 1. update + closure + forward x
+// 1. update + closure + forward x.  The points of a thread are processed
+// in batches of CH; every global load of a batch is issued (at a clamped
+// index, so without branches) before the first use, and only the stores
+// are guarded by the true index.
 // ---------------------------------------------------------------------
-template <typename R, typename S, int EPT, bool UPDATE, int VC>
+#ifndef RISM_CLOS_CH
+#define RISM_CLOS_CH 2
+#endif
+template <typename R, typename S, int EPT, int VC, int HM, bool UPD, bool CREF>
+__device__ __forceinline__ void closure_points(const IterArgs<R, S> &a, typename C2<R>::T *lds, bool do_run,
+                                               const double *th, const long *off, double mix, int w, int nf) {
+  using T2 = typename C2<R>::T;
+  using Q = typename HistT<R>::T;
+  constexpr bool kIncr = sizeof(Q) < sizeof(S);
+  constexpr int CH = EPT < RISM_CLOS_CH ? EPT : RISM_CLOS_CH;
+  const int nx = a.nx, L = a.Lx, ls = line_stride(nx), V = a.V;
+  const long N = a.N, VN = (long)V * N;
+  const long base = (long)blockIdx.x * L * nx;
+  const int nel = L * nx;
+  const T2 *__restrict__ pre = (const T2 *)a.px.pre;
+  S *__restrict__ ts = a.ts;
+  const Q *__restrict__ res = a.res;
+  Q *__restrict__ ht = a.hist_t;
+  Q *__restrict__ hr = a.hist_r;
+  const R *__restrict__ uu = a.u;
+  const float *__restrict__ crf = a.cref;
+#pragma unroll
+  for (int k0 = 0; k0 < EPT; k0 += CH) {
+    double t[CH][VC], u[CH][VC];
+    Q r[CH][VC], hti[CH][VC][UPD ? HM : 1], hri[CH][VC][UPD ? HM : 1];
+    float cr[CH][VC];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const unsigned e = (unsigned)min((int)threadIdx.x + (k0 + c) * kThreads, nel - 1);
+#pragma unroll
+      for (int v = 0; v < VC; ++v) {
+        if (v < V) {
+          const long vb = (long)v * N + base;
+          t[c][v] = (double)lane_at(ts + vb, e);
+          u[c][v] = (double)lane_at(uu + vb, e);
+          if (UPD) {
+            r[c][v] = lane_at(res + vb, e);
+#pragma unroll
+            for (int s = 0; s < (UPD ? HM : 0); ++s) {
+              hti[c][v][s] = lane_at(ht + off[s] + vb, e);
+              hri[c][v][s] = lane_at(hr + off[s] + vb, e);
+            }
+          }
+          if (CREF) cr[c][v] = lane_at(crf + vb, e);
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const unsigned e = threadIdx.x + (k0 + c) * kThreads;
+      const bool valid = e < (unsigned)nel;
+#pragma unroll
+      for (int v = 0; v < VC; ++v) {
+        if (v < V) {
+          const long vb = (long)v * N + base;
+          if (UPD) {  // newdt / newdt0 (anderson_cuda.cu:1-23)
+            const double rv = (double)r[c][v];
+            double tb = t[c][v], rb = rv, back = 0.0;
+#pragma unroll
+            for (int s = 0; s < (UPD ? HM : 0); ++s) {
+              // slots beyond the filled ones: valid loads, values discarded
+              const double hts = s < nf ? (double)hti[c][v][UPD ? s : 0] : 0.0;
+              const double hrs = s < nf ? (double)hri[c][v][UPD ? s : 0] : rv;
+              if (kIncr) {  // age s+1: t_(s+1) - t = -(sum of the increments of ages 1..s+1)
+                back -= hts;
+                tb += th[s] * back;
+              } else {
+                tb += th[s] * (hts - t[c][v]);
+              }
+              rb += th[s] * (hrs - rv);
+            }
+            const double tn = tb + mix * rb;
+            if (valid) {
+              lane_at(ht + (long)w * VN + vb, e) = kIncr ? (Q)(tn - t[c][v]) : (Q)t[c][v];
+              lane_at(hr + (long)w * VN + vb, e) = r[c][v];
+              lane_at(ts + vb, e) = (S)tn;
+            }
+            t[c][v] = tn;
+          }
+        }
+      }
+      if (do_run && valid) {
+        const int i = e / nx, x = e - i * nx;
+        R cs[VC + 1];
+#pragma unroll
+        for (int v = 0; v < VC; ++v) {
+          cs[v] = (R)0;
+          if (v < V) {
+            const long vb = (long)v * N + base;
+            if (a.src) {
+              cs[v] = (R)cr[c][v];
+            } else {
+              // closure in FP64 in both precisions: near a charged solute its
+              // argument ts - u is the small difference of large terms
+              const double g = closure_g<double>(t[c][v] - u[c][v], a.closure);
+              double cv = g - 1.0 - t[c][v];
+              if (a.refresh) {  // new reference; the rounding remainder goes to the FP32 passes
+                const float cf = (float)cv;
+                lane_at(a.cref_out + vb, e) = cf;
+                cv -= (double)cf;
+              } else if (a.delta) {
+                cv -= (double)cr[c][v];
+              }
+              cs[v] = (R)cv;
+            }
+          }
+        }
+        cs[VC] = (R)0;
+        const T2 pw = pre[x];
+#pragma unroll
+        for (int p = 0; p < (VC + 1) / 2; ++p)
+          if (2 * p < V) lds[(p * L + i) * ls + lpad(x)] = cmul(mk<T2>(cs[2 * p], cs[2 * p + 1]), pw);
+      }
+    }
+    // keep the next batch's loads out of this one (register pressure)
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+template <typename R, typename S, int EPT, bool UPDATE, int VC, int HM>
 __global__ __launch_bounds__(kThreads) void k_closure_fx(IterArgs<R, S> a) {
   using T2 = typename C2<R>::T;
+  using Q = typename HistT<R>::T;
+  constexpr bool kIncr = sizeof(Q) < sizeof(S);
   const Ctl *ctl = a.ctl;
   const bool do_upd = UPDATE && (ctl->upd_iter == a.iter);
   const bool do_run = (ctl->run_iter == a.iter);
   if (!do_upd && !do_run) return;
   extern __shared__ unsigned char smem_raw[];
   T2 *lds = reinterpret_cast<T2 *>(smem_raw);
-  const int nx = a.nx, L = a.Lx, ls = line_stride(nx), V = a.V, P = a.P;
-  const long N = a.N;
+  const int nx = a.nx, L = a.Lx, ls = line_stride(nx), P = a.P;
+  const long N = a.N, VN = (long)a.V * N;
   const long base = (long)blockIdx.x * L * nx;
   const int nel = L * nx;
-  const T2 *pre = (const T2 *)a.px.pre;
-  int nf = 0, w = 0;
+  // update coefficients by age (incremental form) or by slot (absolute form);
+  // slots beyond the filled ones get a zero coefficient and a valid address
+  int w = 0, nf = 0;
   double mix = 0.0;
+  double th[HM];
+  long off[HM];
+#pragma unroll
+  for (int k = 0; k < HM; ++k) {
+    th[k] = 0.0;
+    off[k] = 0;
+  }
   if (do_upd) {
     nf = ctl->upd_nf;
     w = ctl->upd_w;
     mix = ctl->upd_mix;
+    const int nh = ctl->nhist;
+#pragma unroll
+    for (int k = 0; k < HM; ++k) {
+      const int slot = kIncr ? (((w - 1 - k) % nh) + nh) % nh : (k < nh ? k : 0);
+      th[k] = (k < nf) ? ctl->upd_theta[slot] : 0.0;
+      off[k] = (long)slot * VN;
+    }
   }
-  const long VN = (long)V * N;
-  for (int e = threadIdx.x; e < nel; e += kThreads) {
-    const long ip = base + e;
-    const int i = e / nx;
-    const int x = e - i * nx;
-    R cs[VC];
-#pragma unroll
-    for (int v = 0; v < VC; ++v) {
-      if (v < V) {
-        const long iv = (long)v * N + ip;
-        S t = a.ts[iv];
-        if (do_upd) {  // newdt / newdt0 (anderson_cuda.cu:1-23)
-          const S r = a.res[iv];
-          S tb = t, rb = r;
-          for (int s = 0; s < nf; ++s) {
-            const double th = ctl->upd_theta[s];
-            tb += (S)th * (a.hist_t[s * VN + iv] - t);
-            rb += (S)th * (a.hist_r[s * VN + iv] - r);
-          }
-          a.hist_t[w * VN + iv] = t;
-          a.hist_r[w * VN + iv] = r;
-          t = tb + (S)mix * rb;
-          a.ts[iv] = t;
-        }
-        if (do_run) {
-          if (a.src) {
-            cs[v] = (R)a.cref[iv];
-          } else {
-            // closure in FP64 in both precisions: near a charged solute its
-            // argument ts - u is the small difference of large terms
-            const double td = (double)t;
-            const double g = closure_g<double>(td - (double)a.u[iv], a.closure);
-            double c = g - 1.0 - td;
-            if (a.refresh) {  // new reference; the rounding remainder goes to the FP32 passes
-              const float cr = (float)c;
-              a.cref_out[iv] = cr;
-              c -= (double)cr;
-            } else if (a.delta) {
-              c -= (double)a.cref[iv];
-            }
-            cs[v] = (R)c;
-          }
-        }
-      } else {
-        cs[v] = (R)0;
-      }
-    }
-    if (do_run) {
-      const T2 pw = pre[x];
-#pragma unroll
-      for (int p = 0; p < (VC + 1) / 2; ++p)
-        if (p < P) lds[(p * L + i) * ls + lpad(x)] = cmul(mk<T2>(cs[2 * p], 2 * p + 1 < VC ? cs[2 * p + 1] : (R)0), pw);
-    }
+  // c_ref is an input of the delta-mode FP32 pass and of the FP64 source pass
+  // (the FP64 update pass never reads it: delta mode is FP32-only)
+  constexpr bool kCrefUpd = sizeof(R) < sizeof(double);
+  const bool cref_in = do_run && a.cref && (a.src || (a.delta && !a.refresh));
+  if (UPDATE && do_upd) {
+    if (kCrefUpd && cref_in)
+      closure_points<R, S, EPT, VC, HM, true, kCrefUpd>(a, lds, do_run, th, off, mix, w, nf);
+    else
+      closure_points<R, S, EPT, VC, HM, true, false>(a, lds, do_run, th, off, mix, w, nf);
+  } else {
+    constexpr bool kCrefRun = kCrefUpd || !UPDATE;  // FP64: only the source pass reads c_ref
+    if (kCrefRun && cref_in)
+      closure_points<R, S, EPT, VC, HM, false, kCrefRun>(a, lds, do_run, th, off, mix, w, nf);
+    else
+      closure_points<R, S, EPT, VC, HM, false, false>(a, lds, do_run, th, off, mix, w, nf);
   }
   if (!do_run) return;
   __syncthreads();
   fft_lines<EPT, false>(lds, P * L, ls, a.px);
   const R dv = (R)a.dv;
-  for (int e = threadIdx.x; e < nel; e += kThreads) {
-    const int i = e / nx;
-    const int j = e - i * nx;
-    const R sc = dv * (R)jsign(j, a.px.fixsign);
-    for (int p = 0; p < P; ++p) a.Z[p * N + base + e] = cscale(lds[(p * L + i) * ls + lpad(j)], sc);
+  T2 *__restrict__ Z = a.Z;
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int f = threadIdx.x + k * kThreads;  // (pack, element) flattened
+    if (f < P * nel) {
+      const int p = f / nel, e = f - p * nel;
+      const int i = e / nx, j = e - i * nx;
+      const R sc = dv * (R)jsign(j, a.px.fixsign);
+      Z[p * N + base + e] = cscale(lds[(p * L + i) * ls + lpad(j)], sc);
+    }
   }
 }
 
@@ -137,24 +254,36 @@ __global__ __launch_bounds__(kThreads) void k_fft_y(typename C2<R>::T *Z, const 
   const int ls = line_stride(ny);
   const int x0 = blockIdx.x * T;
   const long zoff = (long)blockIdx.y * nx * ny;
-  T2 *Zp = Z + (long)blockIdx.z * N + zoff + x0;
-  const T2 *pre = (const T2 *)py.pre;
+  T2 *__restrict__ Zp = Z + (long)blockIdx.z * N + zoff + x0;
+  const T2 *__restrict__ pre = (const T2 *)py.pre;
   const int nel = T * ny;
-  for (int e = threadIdx.x; e < nel; e += kThreads) {
-    const int y = e / T;
-    const int i = e - y * T;
-    T2 v = Zp[(long)y * nx + i];
-    v = INV ? cscale(v, (decltype(v.x))jsign(y, py.fixsign)) : cmul(v, pre[y]);
-    lds[i * ls + lpad(y)] = v;
+  // loads use a clamped index so they form one straight-line batch
+  T2 buf[EPT];
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int e = min((int)threadIdx.x + k * kThreads, nel - 1);
+    const int y = e / T, i = e - y * T;
+    buf[k] = Zp[(long)y * nx + i];
+  }
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int e = threadIdx.x + k * kThreads;
+    if (e < nel) {
+      const int y = e / T, i = e - y * T;
+      lds[i * ls + lpad(y)] = INV ? cscale(buf[k], (decltype(buf[k].x))jsign(y, py.fixsign)) : cmul(buf[k], pre[y]);
+    }
   }
   __syncthreads();
   fft_lines<EPT, INV>(lds, T, ls, py);
-  for (int e = threadIdx.x; e < nel; e += kThreads) {
-    const int y = e / T;
-    const int i = e - y * T;
-    T2 v = lds[i * ls + lpad(y)];
-    v = INV ? cmulc(v, pre[y]) : cscale(v, (decltype(v.x))jsign(y, py.fixsign));
-    Zp[(long)y * nx + i] = v;
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int e = threadIdx.x + k * kThreads;
+    if (e < nel) {
+      const int y = e / T, i = e - y * T;
+      T2 v = lds[i * ls + lpad(y)];
+      v = INV ? cmulc(v, pre[y]) : cscale(v, (decltype(v.x))jsign(y, py.fixsign));
+      Zp[(long)y * nx + i] = v;
+    }
   }
 }
 
@@ -175,195 +304,133 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_oz(IterArgs<R, S> a) {
   const int ls = line_stride(nz);
   const int xa = blockIdx.x * T, ya = blockIdx.y;
   const int xb = nx - xa - T, yb = ny - 1 - ya;
-  const T2 *pre = (const T2 *)a.pz.pre;
+  const T2 *__restrict__ pre = (const T2 *)a.pz.pre;
+  T2 *__restrict__ Z = a.Z;
   const int nel = 2 * P * T * nz;
   // load both tiles of every pack; line = (p*2 + tile)*T + i
-  for (int e = threadIdx.x; e < nel; e += kThreads) {
-    int r = e;
+  T2 buf[EPT];
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    int r = min((int)threadIdx.x + k * kThreads, nel - 1);
     const int i = r % T; r /= T;
     const int z = r % nz; r /= nz;
     const int tile = r & 1, p = r >> 1;
-    const long off = (long)p * N + (long)z * nxy + (tile ? (long)yb * nx + xb : (long)ya * nx + xa) + i;
-    lds[((p * 2 + tile) * T + i) * ls + lpad(z)] = cmul(a.Z[off], pre[z]);
+    buf[k] = Z[(long)p * N + (long)z * nxy + (tile ? (long)yb * nx + xb : (long)ya * nx + xa) + i];
+  }
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int e = threadIdx.x + k * kThreads;
+    if (e < nel) {
+      int r = e;
+      const int i = r % T; r /= T;
+      const int z = r % nz; r /= nz;
+      const int tile = r & 1, p = r >> 1;
+      lds[((p * 2 + tile) * T + i) * ls + lpad(z)] = cmul(buf[k], pre[z]);
+    }
   }
   __syncthreads();
   fft_lines<EPT, false>(lds, 2 * P * T, ls, a.pz);
-  // k-space: pair (i, j) of tile A with (T-1-i, nz-1-j) of tile B
-  const double tpx = 2.0 * M_PI / a.box[0], tpy = 2.0 * M_PI / a.box[1], tpz = 2.0 * M_PI / a.box[2];
-  const double ky = tpy * ((double)ya - ny / 2.0 + 0.5);
+  // k-space: pair (i, j) of tile A with (T-1-i, nz-1-j) of tile B.  The
+  // wave-vector arithmetic runs in the field precision R.
+  const R tpx = (R)(2.0 * M_PI / a.box[0]), tpz = (R)(2.0 * M_PI / a.box[2]);
+  const R ky = (R)(2.0 * M_PI / a.box[1] * ((double)ya - ny / 2.0 + 0.5));
+  const R dk = (R)a.dk, idk = (R)(1.0 / a.dk);
+  const R hx = (R)(0.5 - nx / 2), hz = (R)(0.5 - nz / 2);
   const int nseg = a.ntab - 1;
-  for (int e = threadIdx.x; e < T * nz; e += kThreads) {
-    const int j = e / T;
-    const int i = e - j * T;
-    const int ib = T - 1 - i, jb = nz - 1 - j;
-    const int x = xa + i;
-    const double kx = tpx * ((double)x - nx / 2.0 + 0.5);
-    const double kz = tpz * ((double)j - nz / 2.0 + 0.5);
-    const double kk = sqrt(kx * kx + ky * ky + kz * kz);
-    // spline interval on the uniform table k_i = dk (i + 1) (splint.cc:4-20)
-    int lo = (int)floor(kk / a.dk) - 1;
-    if (lo < 0) lo = 0;
-    if (lo > nseg - 1) lo = nseg - 1;
-    if (lo < nseg - 1 && a.dk * (lo + 2) <= kk) ++lo;
-    if (lo > 0 && a.dk * (lo + 1) > kk) --lo;
-    const R tt = (R)(kk - a.dk * (lo + 1));
-    const R sa = (R)jsign(j, a.pz.fixsign), sb = (R)jsign(jb, a.pz.fixsign);
-    const T2 fk = a.fkh[((long)j * (ny / 2) + ya) * nx + x];
-    T2 c[VC + 1];
+  const T4 *__restrict__ chi = a.chi;
+  const T2 *__restrict__ fkh = a.fkh;
+  const int npair = T * nz;
 #pragma unroll
-    for (int p = 0; p < (VC + 1) / 2; ++p) {
-      if (p < P) {
-        const T2 za = cscale(lds[((p * 2 + 0) * T + i) * ls + lpad(j)], sa);
-        const T2 zb = cscale(lds[((p * 2 + 1) * T + ib) * ls + lpad(jb)], sb);
-        // C_{2p} = (za + conj zb)/2 ; C_{2p+1} = -i (za - conj zb)/2
-        c[2 * p] = mk<T2>((R)0.5 * (za.x + zb.x), (R)0.5 * (za.y - zb.y));
-        c[2 * p + 1] = mk<T2>((R)0.5 * (za.y + zb.y), (R)-0.5 * (za.x - zb.x));
+  for (int k = 0; k < (EPT + 1) / 2; ++k) {
+    // clamped index: loads and arithmetic are unconditional, the LDS stores
+    // below are guarded by the true index
+    const int e0 = threadIdx.x + k * kThreads;
+    const int e = min(e0, npair - 1);
+    {
+      const int j = e / T;
+      const int i = e - j * T;
+      const int ib = T - 1 - i, jb = nz - 1 - j;
+      const int x = xa + i;
+      const R kx = tpx * ((R)x + hx);
+      const R kz = tpz * ((R)j + hz);
+      const R kk = sqrt(kx * kx + ky * ky + kz * kz);
+      // spline interval on the uniform table k_i = dk (i + 1) (splint.cc:4-20)
+      int lo = (int)(kk * idk) - 1;
+      lo = max(0, min(lo, nseg - 1));
+      if (lo < nseg - 1 && dk * (R)(lo + 2) <= kk) ++lo;
+      if (lo > 0 && dk * (R)(lo + 1) > kk) --lo;
+      const R tt = kk - dk * (R)(lo + 1);
+      T4 cf[VC][VC];
+#pragma unroll
+      for (int v = 0; v < VC; ++v)
+#pragma unroll
+        for (int v1 = 0; v1 < VC; ++v1)
+          if (v < V && v1 < V) cf[v][v1] = chi[(long)(v * V + v1) * nseg + lo];
+      const T2 fk = a.use_fk ? fkh[((long)j * (ny / 2) + ya) * nx + x] : mk<T2>((R)0, (R)0);
+      const R sa = (R)jsign(j, a.pz.fixsign), sb = (R)jsign(jb, a.pz.fixsign);
+      T2 c[VC + 1];
+#pragma unroll
+      for (int p = 0; p < (VC + 1) / 2; ++p) {
+        if (p < P) {
+          const T2 za = cscale(lds[((p * 2 + 0) * T + i) * ls + lpad(j)], sa);
+          const T2 zb = cscale(lds[((p * 2 + 1) * T + ib) * ls + lpad(jb)], sb);
+          // C_{2p} = (za + conj zb)/2 ; C_{2p+1} = -i (za - conj zb)/2
+          c[2 * p] = mk<T2>((R)0.5 * (za.x + zb.x), (R)0.5 * (za.y - zb.y));
+          c[2 * p + 1] = mk<T2>((R)0.5 * (za.y + zb.y), (R)-0.5 * (za.x - zb.x));
+        }
       }
-    }
-    if (a.use_fk) {
 #pragma unroll
       for (int v = 0; v < VC; ++v)
         if (v < V) {
           const R q = (R)a.qcf[v];
-          c[v] = mk<T2>(c[v].x - q * fk.x, c[v].y - q * fk.y);  // mqvfk
+          c[v] = mk<T2>(c[v].x - q * fk.x, c[v].y - q * fk.y);  // mqvfk (fk = 0 in delta mode)
         }
-    }
-    T2 h[VC + 1];
+      T2 h[VC + 1];
 #pragma unroll
-    for (int v = 0; v < VC + 1; ++v) {
-      h[v] = mk<T2>((R)0, (R)0);
-      if (v < V) {
-        R hr = 0, hi = 0;
+      for (int v = 0; v < VC + 1; ++v) {
+        h[v] = mk<T2>((R)0, (R)0);
+        if (v < V && v < VC) {
+          R hr_ = 0, hi_ = 0;
 #pragma unroll
-        for (int v1 = 0; v1 < VC; ++v1)
-          if (v1 < V) {  // oz: h_v = sum_v' chi[v][v'] c_v'
-            const T4 cf = a.chi[(long)(v * V + v1) * nseg + lo];
-            const R xv = ((cf.w * tt + cf.z) * tt + cf.y) * tt + cf.x;
-            hr += c[v1].x * xv;
-            hi += c[v1].y * xv;
-          }
-        h[v] = mk<T2>(hr, hi);
+          for (int v1 = 0; v1 < VC; ++v1)
+            if (v1 < V) {  // oz: h_v = sum_v' chi[v][v'] c_v'
+              const T4 q4 = cf[v][v1];
+              const R xv = ((q4.w * tt + q4.z) * tt + q4.y) * tt + q4.x;
+              hr_ += c[v1].x * xv;
+              hi_ += c[v1].y * xv;
+            }
+          h[v] = mk<T2>(hr_, hi_);
+        }
       }
-    }
 #pragma unroll
-    for (int p = 0; p < (VC + 1) / 2; ++p) {
-      if (p < P) {
-        const T2 h0 = h[2 * p], h1 = h[2 * p + 1];
-        // H(k) = h0 + i h1 ; H(-k) = conj(h0) + i conj(h1)
-        const T2 ha = mk<T2>(h0.x - h1.y, h0.y + h1.x);
-        const T2 hb = mk<T2>(h0.x + h1.y, h1.x - h0.y);
-        lds[((p * 2 + 0) * T + i) * ls + lpad(j)] = cscale(ha, sa);
-        lds[((p * 2 + 1) * T + ib) * ls + lpad(jb)] = cscale(hb, sb);
+      for (int p = 0; p < (VC + 1) / 2; ++p) {
+        if (p < P && e0 < npair) {
+          const T2 h0 = h[2 * p], h1 = h[2 * p + 1];
+          // H(k) = h0 + i h1 ; H(-k) = conj(h0) + i conj(h1)
+          const T2 ha = mk<T2>(h0.x - h1.y, h0.y + h1.x);
+          const T2 hb = mk<T2>(h0.x + h1.y, h1.x - h0.y);
+          lds[((p * 2 + 0) * T + i) * ls + lpad(j)] = cscale(ha, sa);
+          lds[((p * 2 + 1) * T + ib) * ls + lpad(jb)] = cscale(hb, sb);
+        }
       }
     }
   }
   __syncthreads();
   fft_lines<EPT, true>(lds, 2 * P * T, ls, a.pz);
-  for (int e = threadIdx.x; e < nel; e += kThreads) {
-    int r = e;
-    const int i = r % T; r /= T;
-    const int z = r % nz; r /= nz;
-    const int tile = r & 1, p = r >> 1;
-    const long off = (long)p * N + (long)z * nxy + (tile ? (long)yb * nx + xb : (long)ya * nx + xa) + i;
-    a.Z[off] = cmulc(lds[((p * 2 + tile) * T + i) * ls + lpad(z)], pre[z]);
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int e = threadIdx.x + k * kThreads;
+    if (e < nel) {
+      int r = e;
+      const int i = r % T; r /= T;
+      const int z = r % nz; r /= nz;
+      const int tile = r & 1, p = r >> 1;
+      Z[(long)p * N + (long)z * nxy + (tile ? (long)yb * nx + xb : (long)ya * nx + xa) + i] =
+          cmulc(lds[((p * 2 + tile) * T + i) * ls + lpad(z)], pre[z]);
+    }
   }
 }
 
-// ---------------------------------------------------------------------
-// 5. inverse x + residual + Anderson inner products
-// ---------------------------------------------------------------------
-template <typename R, typename S, int EPT, int HM, int VC>
-__global__ __launch_bounds__(kThreads) void k_fx_resid(IterArgs<R, S> a) {
-  using T2 = typename C2<R>::T;
-  constexpr int NV = 1 + HM + HM * (HM + 1) / 2;
-  const Ctl *ctl = a.ctl;
-  if (ctl->run_iter != a.iter) return;
-  extern __shared__ unsigned char smem_raw[];
-  T2 *lds = reinterpret_cast<T2 *>(smem_raw);
-  const int nx = a.nx, L = a.Lx, ls = line_stride(nx), V = a.V, P = a.P;
-  const long N = a.N, VN = (long)V * N;
-  const int nel = L * nx;
-  const int groups = (int)((long)a.ny * a.nz / L);
-  const int nf = ctl->filled < HM ? ctl->filled : HM;
-  const T2 *pre = (const T2 *)a.px.pre;
-  const R ivol = (R)a.ivol;
-  double acc[NV];
-#pragma unroll
-  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-  for (int grp = blockIdx.x; grp < groups; grp += gridDim.x) {
-    const long base = (long)grp * nel;
-    __syncthreads();
-    for (int e = threadIdx.x; e < nel; e += kThreads) {
-      const int i = e / nx;
-      const int j = e - i * nx;
-      const R sg = (R)jsign(j, a.px.fixsign);
-      for (int p = 0; p < P; ++p) lds[(p * L + i) * ls + lpad(j)] = cscale(a.Z[p * N + base + e], sg);
-    }
-    __syncthreads();
-    fft_lines<EPT, true>(lds, P * L, ls, a.px);
-    for (int e = threadIdx.x; e < nel; e += kThreads) {
-      const long ip = base + e;
-      const int i = e / nx;
-      const int x = e - i * nx;
-      const T2 pw = pre[x];
-      double hv[VC + 1];
-#pragma unroll
-      for (int p = 0; p < (VC + 1) / 2; ++p) {
-        if (p < P) {
-          const T2 z = cscale(cmulc(lds[(p * L + i) * ls + lpad(x)], pw), ivol);
-          hv[2 * p] = (double)z.x;
-          hv[2 * p + 1] = (double)z.y;
-          if (a.delta) {
-            const double2 hr = a.href[p * N + ip];
-            hv[2 * p] += hr.x;
-            hv[2 * p + 1] += hr.y;
-          }
-          if (a.store_h) a.Z[p * N + ip] = z;
-        }
-      }
-      if (a.href_only) continue;
-#pragma unroll
-      for (int v = 0; v < VC; ++v) {
-        if (v < V) {
-          const long iv = (long)v * N + ip;
-          const double t = (double)a.ts[iv];
-          const double g = closure_g<double>(t - (double)a.u[iv], a.closure);
-          const S rv = (S)(hv[v] + 1.0 - g);  // tr: R = h + 1 - g
-          a.res[iv] = rv;
-          const double rd = (double)rv, wgt = a.rho_inv[v];
-          acc[0] += rd * rd;
-          double d[HM];
-#pragma unroll
-          for (int s = 0; s < HM; ++s) d[s] = (s < nf) ? rd - (double)a.hist_r[s * VN + iv] : 0.0;
-#pragma unroll
-          for (int s = 0; s < HM; ++s) acc[1 + s] += wgt * rd * d[s];
-          int k = 1 + HM;
-#pragma unroll
-          for (int s = 0; s < HM; ++s)
-#pragma unroll
-            for (int s2 = s; s2 < HM; ++s2) acc[k++] += wgt * d[s] * d[s2];
-        }
-      }
-    }
-  }
-  // workgroup reduction of the accumulators
-  __shared__ double red[kThreads / 64][NV];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-#pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    const double s = wave_sum(acc[k]);
-    if (lane == 0) red[wave][k] = s;
-  }
-  __syncthreads();
-  for (int k = threadIdx.x; k < NV; k += kThreads) {
-    double s = 0.0;
-    for (int wv = 0; wv < kThreads / 64; ++wv) s += red[wv][k];
-    a.partials[(long)blockIdx.x * NV + k] = s;
-  }
-}
-
-// ---------------------------------------------------------------------
-// 6. solve: reduction, rms, convergence, Anderson coefficients
 // ---------------------------------------------------------------------
 static __device__ void leg_solve_dev(double *A, double *x, double *b, int n) {
   // Gaussian elimination without pivoting (anderson.cu:87-112)
@@ -385,26 +452,47 @@ static __device__ void leg_solve_dev(double *A, double *x, double *b, int n) {
   }
 }
 
-static __global__ __launch_bounds__(kThreads) void k_anderson_solve(Ctl *ctl, const double *partials,
-                                                              int nblk, int nv, int hm,
-                                                              double *rms_hist, int iter) {
-  if (ctl->run_iter != iter) return;
-  __shared__ double tot[1 + kHMax + kHMax * (kHMax + 1) / 2];
-  __shared__ double wred[kThreads / 64];
+// One workgroup: reduce the per-workgroup partials (fixed order), rms,
+// convergence test, Picard / Anderson coefficients (leg), control block.
+template <int HM>
+static __device__ void anderson_solve_block(Ctl *ctl, const double *partials, int nblk, double *rms_hist,
+                                            int iter) {
+  constexpr int NV = 1 + HM + HM * (HM + 1) / 2;
+  const int hm = HM;
+  __shared__ double tot[NV];
+  __shared__ double wred[kThreads / 64][NV];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int k = 0; k < nv; ++k) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += kThreads) s += partials[(long)b * nv + k];
-    s = wave_sum(s);
-    if (lane == 0) wred[wave] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double t = 0.0;
-      for (int wv = 0; wv < kThreads / 64; ++wv) t += wred[wv];
-      tot[k] = t;
-    }
-    __syncthreads();
+  // thread t sums rows t, t + 256, ... (fixed order); four rows in flight
+  double s[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) s[k] = 0.0;
+  int b = threadIdx.x;
+  for (; b + 3 * kThreads < nblk; b += 4 * kThreads) {
+    double r[4][NV];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int k = 0; k < NV; ++k) r[j][k] = partials[(long)(b + j * kThreads) * NV + k];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int k = 0; k < NV; ++k) s[k] += r[j][k];
   }
+  for (; b < nblk; b += kThreads)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) s[k] += partials[(long)b * NV + k];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const double sw = wave_sum(s[k]);
+    if (lane == 0) wred[wave][k] = sw;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < NV; k += kThreads) {
+    double t = 0.0;
+    for (int wv = 0; wv < kThreads / 64; ++wv) t += wred[wv][k];
+    tot[k] = t;
+  }
+  __syncthreads();
   if (threadIdx.x != 0) return;
   const double rms = sqrt(tot[0] / ctl->npts);  // cal_rms.cu:18-20
   ctl->step += 1;
@@ -457,6 +545,167 @@ static __global__ __launch_bounds__(kThreads) void k_anderson_solve(Ctl *ctl, co
     ctl->run_iter = -1;
   } else {
     ctl->run_iter = iter + 1;
+  }
+}
+
+template <int HM>
+__global__ __launch_bounds__(kThreads) void k_anderson_solve(Ctl *ctl, const double *partials, int nblk,
+                                                             double *rms_hist, int iter) {
+  if (ctl->run_iter != iter) return;
+  anderson_solve_block<HM>(ctl, partials, nblk, rms_hist, iter);
+}
+
+// ---------------------------------------------------------------------
+// 5. inverse x + residual + Anderson inner products.  After the transform
+// the points are processed in batches whose loads are all issued first.
+// ---------------------------------------------------------------------
+template <typename R, typename S, int EPT, int HM, int VC>
+__global__ __launch_bounds__(kThreads) void k_fx_resid(IterArgs<R, S> a) {
+  using T2 = typename C2<R>::T;
+  using Q = typename HistT<R>::T;
+  constexpr int NV = 1 + HM + HM * (HM + 1) / 2;
+  const Ctl *ctl = a.ctl;
+  if (ctl->run_iter != a.iter) return;
+  extern __shared__ unsigned char smem_raw[];
+  T2 *lds = reinterpret_cast<T2 *>(smem_raw);
+  const int nx = a.nx, L = a.Lx, ls = line_stride(nx), V = a.V, P = a.P;
+  const long N = a.N, VN = (long)V * N;
+  const int nel = L * nx;
+  const int groups = (int)((long)a.ny * a.nz / L);
+  const int nf = ctl->filled < HM ? ctl->filled : HM;
+  const T2 *__restrict__ pre = (const T2 *)a.px.pre;
+  const R ivol = (R)a.ivol;
+  T2 *__restrict__ Z = a.Z;
+  const S *__restrict__ ts = a.ts;
+  const R *__restrict__ uu = a.u;
+  Q *__restrict__ res = a.res;
+  const Q *__restrict__ hr = a.hist_r;
+  const double2 *__restrict__ href = a.href;
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+  for (int grp = blockIdx.x; grp < groups; grp += gridDim.x) {
+    const long base = (long)grp * nel;
+    // the lane's index is made opaque per group: otherwise the compiler
+    // hoists every per-element index and twiddle out of this loop and keeps
+    // them live, which costs more registers than recomputing them
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    __syncthreads();
+    T2 zb[EPT];
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const int f = min(tid + k * kThreads, P * nel - 1);
+      const int p = f / nel, e = f - p * nel;
+      zb[k] = Z[p * N + base + e];
+    }
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const int f = tid + k * kThreads;
+      if (f < P * nel) {
+        const int p = f / nel, e = f - p * nel;
+        const int i = e / nx, j = e - i * nx;
+        lds[(p * L + i) * ls + lpad(j)] = cscale(zb[k], (R)jsign(j, a.px.fixsign));
+      }
+    }
+    __syncthreads();
+    fft_lines<EPT, true>(lds, P * L, ls, a.px, tid);
+    // points per thread: nel <= EPT * kThreads / P and P >= PMIN in this site class
+    constexpr int PMIN = VC <= 2 ? 1 : (VC <= 4 ? 2 : 3);
+    constexpr int KPT = (EPT + PMIN - 1) / PMIN;
+#ifndef RISM_RESID_CH
+#define RISM_RESID_CH 1
+#endif
+    constexpr int CH = KPT < RISM_RESID_CH ? KPT : RISM_RESID_CH;
+    constexpr bool PREH = CH * VC * HM <= 16;  // history residuals join the batch
+#pragma unroll
+    for (int k0 = 0; k0 < KPT; k0 += CH) {
+      // batch of CH points: every load first (clamped index), then the arithmetic
+      double hv[CH][VC + 1];
+      double tv[CH][VC], uv[CH][VC];
+      Q hq[CH][VC][PREH ? HM : 1];
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const unsigned e = (unsigned)min(tid + (k0 + c) * kThreads, nel - 1);
+        const int i = e / nx, x = e - i * nx;
+        const T2 pw = pre[x];
+#pragma unroll
+        for (int p = 0; p < (VC + 1) / 2; ++p) {
+          if (p < P) {
+            const T2 z = cscale(cmulc(lds[(p * L + i) * ls + lpad(x)], pw), ivol);
+            hv[c][2 * p] = (double)z.x;
+            hv[c][2 * p + 1] = (double)z.y;
+            if (a.delta && !a.href_only) {
+              const double2 hb = lane_at(href + p * N + base, e);
+              hv[c][2 * p] += hb.x;
+              hv[c][2 * p + 1] += hb.y;
+            }
+          }
+        }
+        if (!a.href_only) {
+#pragma unroll
+          for (int v = 0; v < VC; ++v)
+            if (v < V) {
+              const long vb = (long)v * N + base;
+              tv[c][v] = (double)lane_at(ts + vb, e);
+              uv[c][v] = (double)lane_at(uu + vb, e);
+#pragma unroll
+              for (int s = 0; s < (PREH ? HM : 0); ++s) hq[c][v][s] = lane_at(hr + (s < nf ? s : 0) * VN + vb, e);
+            }
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const unsigned e = tid + (k0 + c) * kThreads;
+        if (k0 + c < KPT && e < (unsigned)nel) {
+          if (a.store_h) {
+            const int i = e / nx, x = e - i * nx;
+#pragma unroll
+            for (int p = 0; p < (VC + 1) / 2; ++p)
+              if (p < P) lane_at(Z + p * N + base, e) = cscale(cmulc(lds[(p * L + i) * ls + lpad(x)], pre[x]), ivol);
+          }
+          if (a.href_only) continue;
+#pragma unroll
+          for (int v = 0; v < VC; ++v) {
+            if (v < V) {
+              const long vb = (long)v * N + base;
+              const double g = closure_g<double>(tv[c][v] - uv[c][v], a.closure);
+              const double rd = hv[c][v] + 1.0 - g;  // tr: R = h + 1 - g
+              lane_at(res + vb, e) = (Q)rd;
+              const double wgt = a.rho_inv[v];
+              acc[0] += rd * rd;
+              double d[HM];
+#pragma unroll
+              for (int s = 0; s < HM; ++s)
+                d[s] = (s < nf) ? rd - (double)(PREH ? hq[c][v][PREH ? s : 0] : lane_at(hr + s * VN + vb, e)) : 0.0;
+#pragma unroll
+              for (int s = 0; s < HM; ++s) acc[1 + s] += wgt * rd * d[s];
+              int kk = 1 + HM;
+#pragma unroll
+              for (int s = 0; s < HM; ++s)
+#pragma unroll
+                for (int s2 = s; s2 < HM; ++s2) acc[kk++] += wgt * d[s] * d[s2];
+            }
+          }
+        }
+      }
+      // keep the next batch's loads from being hoisted into this one
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  // workgroup reduction of the accumulators
+  __shared__ double red[kThreads / 64][NV];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const double sw = wave_sum(acc[k]);
+    if (lane == 0) red[wave][k] = sw;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < NV; k += kThreads) {
+    double sm = 0.0;
+    for (int wv = 0; wv < kThreads / 64; ++wv) sm += red[wv][k];
+    a.partials[(long)blockIdx.x * NV + k] = sm;
   }
 }
 
@@ -525,6 +774,14 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_pure(typename C2<R>::T *Z, i
     case 32: { constexpr int E = 32; __VA_ARGS__; } break; \
     default: return hipErrorInvalidValue;         \
   }
+// x-line kernels: at most 16 elements per thread (P * Lx * nx <= 4096)
+#define RISM_EPTX_SWITCH(ept, ...)                \
+  switch (ept) {                                  \
+    case 4: { constexpr int E = 4; __VA_ARGS__; } break;   \
+    case 8: { constexpr int E = 8; __VA_ARGS__; } break;   \
+    case 16: { constexpr int E = 16; __VA_ARGS__; } break; \
+    default: return hipErrorInvalidValue;         \
+  }
 
 #define RISM_VC_SWITCH(V, ...)                                  \
   switch ((V) <= 2 ? 2 : ((V) <= 4 ? 4 : 8)) {                 \
@@ -534,14 +791,26 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_pure(typename C2<R>::T *Z, i
   }
 
 template <typename R, typename S>
-hipError_t launch_closure_fx(const IterArgs<R, S> &a, const LaunchGeom &g, bool update, int,
+hipError_t launch_closure_fx(const IterArgs<R, S> &a, const LaunchGeom &g, bool update, int hm,
                              hipStream_t st) {
   const dim3 grid(g.xgroups), block(kThreads);
-  if (update) {
-    RISM_VC_SWITCH(a.V, RISM_EPT_SWITCH(g.ept_x, hipLaunchKernelGGL((k_closure_fx<R, S, E, true, VC>), grid, block, g.lds_x, st, a)))
-  } else {
-    RISM_VC_SWITCH(a.V, RISM_EPT_SWITCH(g.ept_x, hipLaunchKernelGGL((k_closure_fx<R, S, E, false, VC>), grid, block, g.lds_x, st, a)))
+  if (!update) {  // closure + transform only (the FP64 source pass): no history class
+    if constexpr (sizeof(R) == sizeof(double)) {
+      RISM_VC_SWITCH(a.V, RISM_EPTX_SWITCH(g.ept_x, hipLaunchKernelGGL((k_closure_fx<R, S, E, false, VC, 2>), grid, block, g.lds_x, st, a)))
+      return hipGetLastError();
+    } else {
+      return hipErrorInvalidValue;
+    }
   }
+#define RISM_CLOS(HMV) \
+  RISM_VC_SWITCH(a.V, RISM_EPTX_SWITCH(g.ept_x, hipLaunchKernelGGL((k_closure_fx<R, S, E, true, VC, HMV>), grid, block, g.lds_x, st, a)))
+  switch (hm) {
+    case 2: RISM_CLOS(2); break;
+    case 5: RISM_CLOS(5); break;
+    case 10: RISM_CLOS(10); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef RISM_CLOS
   return hipGetLastError();
 }
 
@@ -569,7 +838,7 @@ template <typename R, typename S>
 hipError_t launch_fx_resid(const IterArgs<R, S> &a, const LaunchGeom &g, int hm, hipStream_t st) {
   const dim3 grid(g.xblocks), block(kThreads);
 #define RISM_RESID(HMV)                                                                       \
-  RISM_VC_SWITCH(a.V, RISM_EPT_SWITCH(g.ept_x, hipLaunchKernelGGL((k_fx_resid<R, S, E, HMV, VC>), grid, block, g.lds_x, st, a)))
+  RISM_VC_SWITCH(a.V, RISM_EPTX_SWITCH(g.ept_x, hipLaunchKernelGGL((k_fx_resid<R, S, E, HMV, VC>), grid, block, g.lds_x, st, a)))
   switch (hm) {
     case 2: RISM_RESID(2); break;
     case 5: RISM_RESID(5); break;
@@ -582,8 +851,12 @@ hipError_t launch_fx_resid(const IterArgs<R, S> &a, const LaunchGeom &g, int hm,
 
 inline hipError_t launch_anderson_solve_impl(Ctl *ctl, const double *partials, int nblk, int nv, int hm,
                                  double *rms_hist, int iter, hipStream_t st) {
-  hipLaunchKernelGGL(k_anderson_solve, dim3(1), dim3(kThreads), 0, st, ctl, partials, nblk, nv, hm,
-                     rms_hist, iter);
+  switch (hm) {
+    case 2: hipLaunchKernelGGL(k_anderson_solve<2>, dim3(1), dim3(kThreads), 0, st, ctl, partials, nblk, rms_hist, iter); break;
+    case 5: hipLaunchKernelGGL(k_anderson_solve<5>, dim3(1), dim3(kThreads), 0, st, ctl, partials, nblk, rms_hist, iter); break;
+    case 10: hipLaunchKernelGGL(k_anderson_solve<10>, dim3(1), dim3(kThreads), 0, st, ctl, partials, nblk, rms_hist, iter); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
@@ -601,13 +874,13 @@ hipError_t launch_fft3d(typename C2<R>::T *data, const AxisPlan &px, const AxisP
   // pure z tiles hold T lines: EPT class from T*nz
   const int ez = (g.Tz * nz <= 4 * kThreads) ? 4 : (g.Tz * nz <= 8 * kThreads) ? 8 : (g.Tz * nz <= 16 * kThreads) ? 16 : 32;
   if (dir < 0) {
-    RISM_EPT_SWITCH(g.ept_x, hipLaunchKernelGGL((k_fft_x_pure<R, E, false>), gx, block, ldsx, st, data, nx, L, px, (R)dv))
+    RISM_EPTX_SWITCH(g.ept_x, hipLaunchKernelGGL((k_fft_x_pure<R, E, false>), gx, block, ldsx, st, data, nx, L, px, (R)dv))
     RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, false>), gy, block, ldsy, st, data, (const Ctl *)nullptr, 0, nx, ny, N, g.Ty, py))
     RISM_EPT_SWITCH(ez, hipLaunchKernelGGL((k_fft_z_pure<R, E, false>), gz, block, ldsz, st, data, nx, ny, nz, g.Tz, pz))
   } else {
     RISM_EPT_SWITCH(ez, hipLaunchKernelGGL((k_fft_z_pure<R, E, true>), gz, block, ldsz, st, data, nx, ny, nz, g.Tz, pz))
     RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, true>), gy, block, ldsy, st, data, (const Ctl *)nullptr, 0, nx, ny, N, g.Ty, py))
-    RISM_EPT_SWITCH(g.ept_x, hipLaunchKernelGGL((k_fft_x_pure<R, E, true>), gx, block, ldsx, st, data, nx, L, px, (R)ivol))
+    RISM_EPTX_SWITCH(g.ept_x, hipLaunchKernelGGL((k_fft_x_pure<R, E, true>), gx, block, ldsx, st, data, nx, L, px, (R)ivol))
   }
   return hipGetLastError();
 }

@@ -235,5 +235,17 @@ template hipError_t launch_convert_c<double>(const double2 *, double2 *, long, h
 template hipError_t launch_convert_c<float>(const double2 *, float2 *, long, hipStream_t);
 template hipError_t launch_axpy_fr<double>(double *, const double *, const double *, int, long, int,
                                            hipStream_t);
+template hipError_t launch_axpy_fr<float>(float *, const double *, const double *, int, long, int,
+                                          hipStream_t);
+
+__global__ __launch_bounds__(kThreads) void k_widen(const float *src, double *dst, long n) {
+  const long i = (long)blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) dst[i] = (double)src[i];
+}
+
+hipError_t launch_widen(const float *src, double *dst, long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_widen, grid1(n), dim3(kThreads), 0, st, src, dst, n);
+  return hipGetLastError();
+}
 
 }  // namespace rism

@@ -47,6 +47,13 @@ struct Ctl {
   double npts;    // V*N
 };
 
+// Residual / history storage type: FP64 in the FP64 path; FP32 in the mixed
+// path, where the t history is kept as FP32 step increments t_{k+1} - t_k
+// (rounding then scales with the step, not with |t|) and the state ts stays FP64.
+template <typename R> struct HistT;
+template <> struct HistT<double> { using T = double; };
+template <> struct HistT<float> { using T = float; };
+
 template <typename R> struct R4T;
 template <> struct R4T<double> { using T = double4; };
 template <> struct R4T<float> { using T = float4; };
@@ -56,7 +63,8 @@ struct IterArgs {
   int nx, ny, nz;
   long N;
   int V, P;
-  S *ts, *res, *hist_t, *hist_r;
+  S *ts;
+  typename HistT<R>::T *res, *hist_t, *hist_r;
   const R *u, *fr;
   typename C2<R>::T *Z;
   const typename C2<R>::T *fkh;
@@ -90,6 +98,7 @@ struct IterArgs {
   int src;              // FP64 first pass: transform cref instead of the closure
   int href_only;        // FP64 residual pass: store h only
   int use_fk;           // z pass subtracts q_v cf fk(k)
+  double *rms_hist;
 };
 
 // launch configuration chosen on the host for a grid
@@ -145,6 +154,7 @@ hipError_t launch_convert_c(const double2 *src, typename C2<R>::T *dst, long n, 
 template <typename S>
 hipError_t launch_axpy_fr(S *x, const double *fr, const double *coef, int V, long N, int nslots,
                           hipStream_t st);  // x[s][v] += coef[v] * fr
+hipError_t launch_widen(const float *src, double *dst, long n, hipStream_t st);
 
 // ---- thermodynamics (k_thermo.hip) ----
 template <typename R, typename S>

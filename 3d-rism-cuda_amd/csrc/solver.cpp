@@ -105,7 +105,9 @@ struct rism_handle {
   double *ulj = nullptr, *de = nullptr, *fr = nullptr, *sr = nullptr;
   double2 *fkh = nullptr, *phase = nullptr;
   float *cref = nullptr;    // mixed: reference c_s of the refinement
-  double *ts = nullptr, *res = nullptr, *hist_t = nullptr, *hist_r = nullptr;
+  double *ts = nullptr;
+  void *res = nullptr, *hist_t = nullptr, *hist_r = nullptr;  // FP64, or FP32 in the mixed path
+  size_t qsz = 8;           // bytes per residual / history element
   double *partials = nullptr, *rms_hist = nullptr, *gfield = nullptr;
   Ctl *ctl = nullptr;
   Ctl *ctl_host = nullptr;  // pinned
@@ -268,8 +270,13 @@ long env_long(const char *name, long dflt) {
 int choose_geometry(rism_handle *h, LaunchGeom &g, size_t rsz) {
   const int nx = h->nx, ny = h->ny, nz = h->nz, P = h->P;
   const long c2 = 2 * (long)rsz;
-  const long cap = env_long("RISM_LDS_CAP", 65536);
-  const long xe = env_long("RISM_X_ELEMS", 1024);
+  // LDS per workgroup: 64 KiB by default (RISM_LDS_CAP raises or lowers it),
+  // never above what the device grants one workgroup
+  int lds_max = 65536;
+  if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device) != hipSuccess)
+    lds_max = 65536;
+  const long cap = std::min<long>(env_long("RISM_LDS_CAP", 65536), lds_max);
+  const long xe = env_long("RISM_X_ELEMS", 2048);
   const long lsx = line_stride(nx), lsy = line_stride(ny), lsz = line_stride(nz);
   g.Lx = largest_divisor_le((long)ny * nz, std::max(1L, std::min(xe / ((long)P * nx), cap / ((long)P * lsx * c2))));
   g.Ty = largest_divisor_le(nx, std::max(1L, std::min(8192L / ny, cap / (lsy * c2))));
@@ -280,13 +287,13 @@ int choose_geometry(rism_handle *h, LaunchGeom &g, size_t rsz) {
   if (tz > 0 && nx % tz == 0) g.Tz = (int)tz;
   g.ept_x = ept_class((long)P * g.Lx * nx);
   g.xgroups = (int)((long)ny * nz / g.Lx);
-  g.xblocks = std::min<int>(g.xgroups, (int)env_long("RISM_XBLOCKS", 2048));
+  g.xblocks = std::min<int>(g.xgroups, (int)env_long("RISM_XBLOCKS", 4096));
   g.lds_x = (size_t)P * g.Lx * lsx * c2;
   g.ept_y = ept_class((long)g.Ty * ny);
   g.lds_y = (size_t)g.Ty * lsy * c2;
   g.ept_z = ept_class(2L * P * g.Tz * nz);
   g.lds_z = (size_t)2 * P * g.Tz * lsz * c2;
-  if (g.ept_x < 0 || g.ept_y < 0 || g.ept_z < 0 || (long)g.lds_x > cap || (long)g.lds_y > cap ||
+  if (g.ept_x < 0 || g.ept_x > 16 || g.ept_y < 0 || g.ept_z < 0 || (long)g.lds_x > cap || (long)g.lds_y > cap ||
       (long)g.lds_z > cap) {
     set_error("grid too large for the single-workgroup line transforms");
     return RISM_ERR_ARG;
@@ -300,7 +307,14 @@ template <typename R> IterArgs<R, double> make_args(rism_handle *h, int iter) {
   const PrecSet &ps = h->ps[pidx<R>()];
   a.nx = h->nx; a.ny = h->ny; a.nz = h->nz; a.N = h->N;
   a.V = h->V; a.P = h->P;
-  a.ts = h->ts; a.res = h->res; a.hist_t = h->hist_t; a.hist_r = h->hist_r;
+  a.ts = h->ts;
+  using Q = typename HistT<R>::T;
+  if (sizeof(Q) == h->qsz) {  // the FP64 passes of the mixed path never touch them
+    a.res = (Q *)h->res;
+    a.hist_t = (Q *)h->hist_t;
+    a.hist_r = (Q *)h->hist_r;
+  }
+  a.rms_hist = h->rms_hist;
   a.u = (const R *)ps.u;
   a.fr = (const R *)ps.fr;
   a.Z = (typename C2<R>::T *)ps.Z;
@@ -335,10 +349,12 @@ template <typename R> IterArgs<R, double> make_args(rism_handle *h, int iter) {
   return a;
 }
 
-template <typename R> int launch_iteration(rism_handle *h, int it, bool store_h, bool solve) {
+template <typename R> int launch_iteration(rism_handle *h, int it, bool store_h, bool solve,
+                                          bool h_only = false) {
   IterArgs<R, double> a = make_args<R>(h, it);
   const LaunchGeom &g = h->ps[pidx<R>()].geom;
   a.store_h = store_h ? 1 : 0;
+  a.href_only = h_only ? 1 : 0;
   PROF(h, 0, launch_closure_fx<R, double>(a, g, true, h->hm, h->stream));
   PROF(h, 1, launch_fft_y<R, double>(a, g, false, h->stream));
   PROF(h, 2, launch_fft_z_oz<R, double>(a, g, h->stream));
@@ -492,9 +508,9 @@ template <typename R> int solve_impl(rism_handle *h, const char *tuv_path, int *
   h->evaluated = false;
   // history and state start from zero (anderson.cu allocates without clearing;
   // unfilled slots are never read here)
-  HIP_OK(hipMemsetAsync(h->hist_t, 0, (size_t)h->opts.nhist * VN * sizeof(double), h->stream));
-  HIP_OK(hipMemsetAsync(h->hist_r, 0, (size_t)h->opts.nhist * VN * sizeof(double), h->stream));
-  HIP_OK(hipMemsetAsync(h->res, 0, VN * sizeof(double), h->stream));
+  HIP_OK(hipMemsetAsync(h->hist_t, 0, (size_t)h->opts.nhist * VN * h->qsz, h->stream));
+  HIP_OK(hipMemsetAsync(h->hist_r, 0, (size_t)h->opts.nhist * VN * h->qsz, h->stream));
+  HIP_OK(hipMemsetAsync(h->res, 0, VN * h->qsz, h->stream));
   if (file_exists(tuv_path)) {  // iterate.cu:25-33
     h->cf = 1.0;
     if (int rc = read_tuv_file(h, tuv_path)) return rc;
@@ -538,7 +554,17 @@ template <typename R> int solve_impl(rism_handle *h, const char *tuv_path, int *
         hcoef[v] = -h->qv[v] * dcf;  // history keeps the same t in the new frame
       }
       if (rem != 0.0) HIP_OK(launch_axpy_fr<double>(h->ts, h->fr, coef.data(), V, N, 1, h->stream));
-      HIP_OK(launch_axpy_fr<double>(h->hist_t, h->fr, hcoef.data(), V, N, h->opts.nhist, h->stream));
+      if (h->qsz == sizeof(double)) {
+        HIP_OK(launch_axpy_fr<double>((double *)h->hist_t, h->fr, hcoef.data(), V, N, h->opts.nhist, h->stream));
+      } else if (c.filled > 0) {
+        // increment form: t_(k) - t moves by -q cuf fr for every age k, i.e.
+        // the newest increment grows by q cuf fr
+        const int s1 = (c.write - 1 + h->opts.nhist) % h->opts.nhist;
+        std::vector<double> inc(V);
+        for (int v = 0; v < V; ++v) inc[v] = h->qv[v] * cuf;
+        HIP_OK(launch_axpy_fr<float>((float *)h->hist_t + (size_t)s1 * VN, h->fr, inc.data(), V, N, 1,
+                                     h->stream));
+      }
     }
     if (int rc = stage_potentials(h)) return rc;
     h->delta = false;  // a new stage (new cf) starts without a reference
@@ -642,7 +668,7 @@ template <typename R> int evaluate_impl(rism_handle *h) {
   c.upd_iter = -1;
   const int keep = c.status;
   if (int rc = put_ctl(h)) return rc;
-  if (int rc = launch_iteration<R>(h, it, true, false)) return rc;
+  if (int rc = launch_iteration<R>(h, it, true, false, true)) return rc;
   IterArgs<R, double> a = make_args<R>(h, it);
   HIP_OK(launch_gfield<R, double>(a, h->gfield, h->stream));
   c.run_iter = -1;
@@ -825,6 +851,7 @@ int create_impl(const rism_grid *grid, const rism_solvent *sv, const rism_solute
   h->uxyz.assign(su->xyz, su->xyz + 3 * (size_t)su->natom);
   h->opts = *o;
   h->hm = o->nhist <= 2 ? 2 : (o->nhist <= 5 ? 5 : 10);
+  h->qsz = o->precision == RISM_PREC_MIXED ? sizeof(float) : sizeof(double);
   // k range check (solvent_spline.cu:18-21)
   double kmax2 = 0.0;
   for (int k = 0; k < 3; ++k) {
@@ -841,8 +868,9 @@ int create_impl(const rism_grid *grid, const rism_solvent *sv, const rism_solute
   if ((rc = dalloc_t(h, &h->ulj, VN)) || (rc = dalloc_t(h, &h->de, h->N)) || (rc = dalloc_t(h, &h->fr, h->N)) ||
       (rc = dalloc_t(h, &h->sr, h->N)) || (rc = dalloc_t(h, &h->fkh, nh)) ||
       (rc = dalloc_t(h, &h->phase, (size_t)std::max(1, h->natu) * (h->nx + h->ny + h->nz))) ||
-      (rc = dalloc_t(h, &h->ts, VN)) || (rc = dalloc_t(h, &h->res, VN)) ||
-      (rc = dalloc_t(h, &h->hist_t, (size_t)o->nhist * VN)) || (rc = dalloc_t(h, &h->hist_r, (size_t)o->nhist * VN)) ||
+      (rc = dalloc_t(h, &h->ts, VN)) || (rc = dalloc(h, &h->res, VN * h->qsz)) ||
+      (rc = dalloc(h, &h->hist_t, (size_t)o->nhist * VN * h->qsz)) ||
+      (rc = dalloc(h, &h->hist_r, (size_t)o->nhist * VN * h->qsz)) ||
       (rc = dalloc_t(h, &h->gfield, VN)) || (rc = dalloc_t(h, &h->ctl, 1)) ||
       (rc = dalloc_t(h, &h->rms_hist, (size_t)(o->maxstep + 1) * (o->chargeup + 1) + 4)) ||
       (rc = dalloc_t(h, &h->atoms, std::max(1, h->natu))) || (rc = dalloc_t(h, &h->lj, (size_t)h->V * std::max(1, h->natu))) ||
@@ -1004,7 +1032,15 @@ int rism_get_field(rism_handle *h, int which, double *out) {
   const size_t VN = (size_t)V * N;
   switch (which) {
     case RISM_FIELD_T: return fetch_t(h, out);
-    case RISM_FIELD_RES: HIP_OK(hipMemcpy(out, h->res, VN * sizeof(double), hipMemcpyDeviceToHost)); return RISM_OK;
+    case RISM_FIELD_RES:
+      if (h->qsz == sizeof(double)) {
+        HIP_OK(hipMemcpy(out, h->res, VN * sizeof(double), hipMemcpyDeviceToHost));
+      } else {
+        std::vector<float> r(VN);
+        HIP_OK(hipMemcpy(r.data(), h->res, VN * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < VN; ++i) out[i] = r[i];
+      }
+      return RISM_OK;
     case RISM_FIELD_ULJ: HIP_OK(hipMemcpy(out, h->ulj, VN * sizeof(double), hipMemcpyDeviceToHost)); return RISM_OK;
     case RISM_FIELD_DE: HIP_OK(hipMemcpy(out, h->de, N * sizeof(double), hipMemcpyDeviceToHost)); return RISM_OK;
     case RISM_FIELD_FR: HIP_OK(hipMemcpy(out, h->fr, N * sizeof(double), hipMemcpyDeviceToHost)); return RISM_OK;
@@ -1201,7 +1237,10 @@ int rism_calculate(rism_handle *h, const double *t_dev, double *res_dev) {
   rc = (h->opts.precision == RISM_PREC_MIXED) ? launch_iteration<float>(h, it, true, false)
                                               : launch_iteration<double>(h, it, true, false);
   if (rc) return rc;
-  HIP_OK(hipMemcpyAsync(res_dev, h->res, VN * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+  if (h->qsz == sizeof(double))
+    HIP_OK(hipMemcpyAsync(res_dev, h->res, VN * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+  else
+    HIP_OK(launch_widen((const float *)h->res, res_dev, (long)VN, h->stream));
   HIP_OK(hipStreamSynchronize(h->stream));
   h->solved = true;
   h->evaluated = false;

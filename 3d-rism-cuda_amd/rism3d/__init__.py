@@ -17,7 +17,9 @@ from typing import Optional
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "_build", "librism3d_amd.so")
+# RISM3D_LIBRARY selects another build of the library (e.g. one compiled
+# with different tuning macros); the default is the in-tree build.
+LIB_PATH = os.environ.get("RISM3D_LIBRARY") or os.path.join(os.path.dirname(_HERE), "_build", "librism3d_amd.so")
 
 # status codes (include/rism3d_amd.h)
 OK = 0

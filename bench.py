@@ -38,21 +38,43 @@ SOLVENT = os.path.join(REPO, "tests", "golden", "reference_examples", "h2o_vv_16
 HBM_PEAK_GBS = 8000.0
 
 
-def kernel_bytes(V, P, H, s_t, s_f):
-    """Algorithmic HBM bytes per grid point for each iteration kernel
-    (each operand read once, each result written once, at stored precision;
-    steady-state Anderson step with H filled history slots)."""
+def kernel_bytes(V, P, H, mixed):
+    """Algorithmic HBM bytes per grid point of each iteration kernel: every
+    operand read once and every result written once at its stored width, for
+    a steady-state Anderson step with H filled history slots.
+
+    Stored widths: the state t_s is FP64 in both paths; u, the spectrum and,
+    in the mixed path, R and the history are FP32 (FP64 in the FP64 path).
+    The mixed path transforms the difference to an FP64 reference, so its
+    update step also reads c_ref (FP32) and its residual step reads h_ref
+    (complex FP64 per site pair) instead of the half-grid long-range term.
+    """
+    s_t = 8
+    s_f = 4 if mixed else 8
+    s_h = 4 if mixed else 8
     c = 2 * s_f
     return {
-        # t, R, H (t_i, R_i), u in; t, slot (t, R), spectrum out
-        "closure_fx": V * (s_t + s_t + 2 * H * s_t + s_f) + V * (3 * s_t) + P * c,
+        # t_s, u, R, H x (t_i, R_i) in; new slot (t, R) and t_s out; [c_ref in]; spectrum out
+        "closure_fx": V * (s_t + s_f + s_h + 2 * H * s_h + 2 * s_h + s_t + (4 if mixed else 0)) + P * c,
         "fft_y_fwd": 2 * P * c,
-        # spectrum in/out + the half-grid long-range term
-        "fft_z_oz": 2 * P * c + s_f,
+        # spectrum in/out + the half-grid long-range term (FP64 path only)
+        "fft_z_oz": 2 * P * c + (0 if mixed else c // 2),
         "fft_y_inv": 2 * P * c,
-        # spectrum, t, u, H history R_i in; R out
-        "resid": P * c + V * (s_t + s_f + H * s_t) + V * s_t,
+        # spectrum, [h_ref], t_s, u, H history R_i in; R out
+        "resid": P * c + (16 * P if mixed else 0) + V * (s_t + s_f + H * s_h + s_h),
     }
+
+
+def per_kernel_table(kt, kb, N):
+    out = {}
+    for k, (kms, kn) in kt.items():
+        if kn:
+            e = {"avg_us": round(kms / kn * 1e3, 2), "launches": kn}
+            if k in kb:
+                e["bytes_per_pt"] = kb[k]
+                e["GBps"] = round(kb[k] * N / (kms / kn / 1e3) / 1e9, 1)
+            out[k] = e
+    return out
 
 
 def run_job(s):
@@ -138,7 +160,7 @@ def main():
     mixed = time_path(rism3d, w, solvent, rism3d.MIXED, args.steps, args.warmup)
     fp64 = None if args.mixed_only else time_path(rism3d, w, solvent, rism3d.FP64, args.steps, args.warmup)
     # roofline: dominant iteration kernel of the mixed path
-    kb = kernel_bytes(V, P, w.nhist, 8, 4)
+    kb = kernel_bytes(V, P, w.nhist, True)
     it_kernels = {k: v for k, v in mixed["kernels"].items() if k in kb and v[1] > 0}
     dom = max(it_kernels, key=lambda k: it_kernels[k][0])
     ms, n = it_kernels[dom]
@@ -148,13 +170,7 @@ def main():
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "bytes_per_launch": bytes_launch, "avg_launch_us": round(avg_s * 1e6, 2)}
-    per_kernel = {}
-    for k, (kms, kn) in mixed["kernels"].items():
-        if kn:
-            e = {"avg_us": round(kms / kn * 1e3, 2), "launches": kn}
-            if k in kb:
-                e["GBps"] = round(kb[k] * N / (kms / kn / 1e3) / 1e9, 1)
-            per_kernel[k] = e
+    per_kernel = per_kernel_table(mixed["kernels"], kb, N)
     cpu = None
     if not args.no_cpu and fp64 is not None:
         try:
@@ -192,8 +208,7 @@ def main():
             "dmu_mixed_minus_fp64_J_mol": round(mixed["xmu"]["SFE_SC"] - fp64["xmu"]["SFE_SC"], 4),
             "sfe_sc_fp64_J_mol": fp64["xmu"]["SFE_SC"],
             "pmv_fp64_L_mol": fp64["xmu"]["PMV"],
-            "kernels_fp64": {k: {"avg_us": round(v[0] / v[1] * 1e3, 2), "launches": v[1]}
-                             for k, v in fp64["kernels"].items() if v[1]},
+            "kernels_fp64": per_kernel_table(fp64["kernels"], kernel_bytes(V, P, w.nhist, False), N),
         })
     print(json.dumps(out))
 

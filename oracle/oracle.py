@@ -73,6 +73,7 @@ def lib():
         L.orc_alloc_test.argtypes = [vp, C.c_int, C.c_long, C.c_int, dp]
         L.orc_set_tr.argtypes = [vp, dp]
         L.orc_copy_hist.argtypes = [vp, C.c_int, C.c_int, dp]
+        L.orc_set_atom_xyz.argtypes = [vp, C.c_int, dp]
         _lib = L
     return _lib
 
@@ -182,6 +183,11 @@ class Oracle:
     def set_tr(self, tr):
         tr = np.ascontiguousarray(tr, dtype=np.float64).reshape(-1)
         lib().orc_set_tr(self.s, _d(tr))
+
+    def set_atom_xyz(self, u, xyz):
+        """Move atom u without recomputing the potentials (test hook)."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(3)
+        lib().orc_set_atom_xyz(self.s, int(u), _d(xyz))
 
     def hist(self, which, slot):
         out = np.zeros(self.V * self.N)

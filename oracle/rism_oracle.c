@@ -1218,3 +1218,8 @@ void orc_copy_hist(const orc_sys *s, int which, int slot, double *out) {
   size_t VN = (size_t)s->natv * s->ng;
   memcpy(out, (which ? s->rp : s->tp) + (size_t)slot * VN, VN * sizeof(double));
 }
+
+void orc_set_atom_xyz(orc_sys *s, int u, const double *xyz) {
+  if (u < 0 || u >= s->natu) return;
+  for (int k = 0; k < 3; ++k) s->ru[3 * u + k] = xyz[k];
+}

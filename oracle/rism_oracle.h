@@ -150,6 +150,8 @@ int orc_alloc_test(orc_sys *s, int natv, long ng, int nhist, const double *rho);
 void orc_set_tr(orc_sys *s, const double *tr);
 void orc_copy_hist(const orc_sys *s, int which, int slot, double *out);
 void orc_set_iter(orc_sys *s, double tol, int maxstep, int count, double m, double mp);
+/* test hook: move solute atom u (its grid potentials are not recomputed) */
+void orc_set_atom_xyz(orc_sys *s, int u, const double *xyz);
 
 #ifdef __cplusplus
 }

@@ -158,6 +158,33 @@ def test_solve_test1_fp64_matches_oracle(gpu, deckdir):
     s.close()
 
 
+@pytest.mark.parametrize("closure,qscale", [(rism3d.KH, 1.0), (rism3d.HNC, 0.2)])
+def test_solve_multi_atom_fp64_thermo_and_grad(gpu, closure, qscale):
+    """Six charged LJ atoms on a 32^3 grid, both closures (HNC with the
+    charges scaled by 0.2, where it converges): converged thermodynamics, SE
+    and the cal_grad forces agree with the oracle."""
+    print(f"solve 6 atoms 32^3 closure={closure}", flush=True)
+    n, box = (32, 32, 32), (16.0, 16.0, 16.0)
+    sig, eps, q, xyz = _atoms(6, seed=21, spread=3.0)
+    q = q * qscale
+    o = orc.Oracle.from_arrays(n, box, example("h2o_vv.xvk"), sig, eps, q, xyz, closure=closure).setup()
+    ost, osteps, _ = o.iterate()
+    ox, og = o.thermo(), o.grad()
+    su = rism3d.Solute(sig, eps, q, xyz)
+    s = rism3d.Solver(n, box, _solvent(), su, rism3d.Options(closure=closure))
+    s.build_potentials()
+    st, steps, _ = s.solve()
+    x, g = s.thermo(), s.grad()
+    s.close()
+    print("oracle", ost, osteps, ox["SFE_SC"], "gpu", st, steps, x["SFE_SC"], flush=True)
+    assert st == rism3d.SOLVE_CONVERGED and ost == 0
+    assert abs(steps - osteps) <= 1
+    for k in ("SFE_SC", "SFE_GF", "SE", "SE_LJ", "SE_ES"):
+        assert abs(x[k] - ox[k]) < 0.05, k
+    assert abs(x["PMV"] - ox["PMV"]) < 1e-8
+    assert np.abs(g - og).max() < 1e-6 * np.abs(og).max()
+
+
 @pytest.mark.parametrize("tol", [1e-6, 1e-7])
 def test_solve_test1_mixed_against_converged_oracle(gpu, deckdir, tol):
     """Mixed precision takes its own Anderson trajectory, so it is judged
