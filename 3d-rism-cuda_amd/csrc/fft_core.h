@@ -188,9 +188,7 @@ __device__ __forceinline__ void fft_stage(T2 *lds, int nlines, int ls, int n, in
   const int nbl = n / F;
   const int total = nlines * nbl;
   T2 v[KMAX][F];
-  T2 w[KMAX][F];
   int lbase[KMAX], q0[KMAX], ppv[KMAX];
-  // twiddles first (independent of the data), then the LDS reads
 #pragma unroll
   for (int k = 0; k < KMAX; ++k) {
     const int bf = tid + k * kThreads;
@@ -202,8 +200,6 @@ __device__ __forceinline__ void fft_stage(T2 *lds, int nlines, int ls, int n, in
       lbase[k] = line * ls;
       q0[k] = j - pp * s;
       ppv[k] = pp;
-#pragma unroll
-      for (int b = 1; b < F; ++b) w[k][b] = tw[pp * b * tstep];
     }
   }
 #pragma unroll
@@ -219,19 +215,31 @@ __device__ __forceinline__ void fft_stage(T2 *lds, int nlines, int ls, int n, in
       const int o = q0[k] + s * (F * ppv[k]);
       lds[lbase[k] + lpad(o)] = v[k][0];
 #pragma unroll
-      for (int b = 1; b < F; ++b)
-        lds[lbase[k] + lpad(o + s * b)] = INV ? cmulc(v[k][b], w[k][b]) : cmul(v[k][b], w[k][b]);
+      for (int b = 1; b < F; ++b) {
+        const T2 w = tw[ppv[k] * b * tstep];  // tw lives in LDS
+        lds[lbase[k] + lpad(o + s * b)] = INV ? cmulc(v[k][b], w) : cmul(v[k][b], w);
+      }
     }
   }
   __syncthreads();
 }
 
-// Run the full plan on nlines lines of length p.n.  EPT = upper bound of
-// elements per thread (nlines * n <= kThreads * EPT).  tid is the lane's
-// thread index (a caller may pass an opaque copy of threadIdx.x).
+// stage an axis plan's twiddle and phase tables (n complex each) into LDS
+template <typename T2>
+__device__ __forceinline__ void stage_tables(T2 *ltw, const AxisPlan &p) {
+  for (int k = threadIdx.x; k < p.n; k += kThreads) {
+    ltw[k] = ((const T2 *)p.tw)[k];
+    ltw[p.n + k] = ((const T2 *)p.pre)[k];
+  }
+}
+
+// Run the full plan on nlines lines of length p.n held in LDS.  EPT = upper
+// bound of elements per thread (nlines * n <= kThreads * EPT); tw = the
+// plan's twiddle table staged in LDS (stage_tables); tid = the lane's
+// thread index (callers in persistent loops pass an opaque copy).
 template <int EPT, bool INV, typename T2>
-__device__ __forceinline__ void fft_lines(T2 *lds, int nlines, int ls, const AxisPlan &p, int tid = threadIdx.x) {
-  const T2 *__restrict__ tw = (const T2 *)p.tw;
+__device__ __forceinline__ void fft_lines(T2 *lds, int nlines, int ls, const AxisPlan &p, const T2 *tw,
+                                          int tid = threadIdx.x) {
   for (int st = 0; st < p.nst; ++st) {
     const int F = p.rad[st], s = p.s[st], m = p.m[st], ts = p.tstep[st];
     const int a2 = p.ls2[st], b2 = p.lb2[st];

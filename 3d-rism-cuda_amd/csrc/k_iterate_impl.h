@@ -225,8 +225,10 @@ __global__ __launch_bounds__(kThreads) void k_closure_fx(IterArgs<R, S> a) {
       closure_points<R, S, EPT, VC, HM, false, false>(a, lds, do_run, th, off, mix, w, nf);
   }
   if (!do_run) return;
+  T2 *ltw = lds + P * L * ls;
+  stage_tables(ltw, a.px);
   __syncthreads();
-  fft_lines<EPT, false>(lds, P * L, ls, a.px);
+  fft_lines<EPT, false>(lds, P * L, ls, a.px, ltw);
   const R dv = (R)a.dv;
   T2 *__restrict__ Z = a.Z;
 #pragma unroll
@@ -242,46 +244,49 @@ __global__ __launch_bounds__(kThreads) void k_closure_fx(IterArgs<R, S> a) {
 }
 
 // ---------------------------------------------------------------------
-// 2/4. FFT along y on tiles of Ty consecutive x
+// 2/4. FFT along y on tiles of T consecutive x (T x 8 bytes = one 128-byte
+// line for FP32 at T = 16).  One tile per workgroup; the twiddle and phase
+// tables are staged in LDS behind the tile.
 // ---------------------------------------------------------------------
 template <typename R, int EPT, bool INV>
-__global__ __launch_bounds__(kThreads) void k_fft_y(typename C2<R>::T *Z, const Ctl *ctl, int iter,
-                                                     int nx, int ny, long N, int T, AxisPlan py) {
+__global__ __launch_bounds__(kThreads) void k_fft_y(typename C2<R>::T *Z, const Ctl *ctl, int iter, int nx,
+                                                     int ny, int nz, long N, int T, int P, AxisPlan py) {
   using T2 = typename C2<R>::T;
   if (ctl && ctl->run_iter != iter) return;
   extern __shared__ unsigned char smem_raw[];
   T2 *lds = reinterpret_cast<T2 *>(smem_raw);
   const int ls = line_stride(ny);
-  const int x0 = blockIdx.x * T;
-  const long zoff = (long)blockIdx.y * nx * ny;
-  T2 *__restrict__ Zp = Z + (long)blockIdx.z * N + zoff + x0;
-  const T2 *__restrict__ pre = (const T2 *)py.pre;
+  const int nxt = nx / T;
   const int nel = T * ny;
-  // loads use a clamped index so they form one straight-line batch
+  const int t = blockIdx.x, xt = t % nxt, r = t / nxt;
+  T2 *__restrict__ Zp = Z + (long)(r / nz) * N + (long)(r % nz) * nx * ny + xt * T;
   T2 buf[EPT];
 #pragma unroll
-  for (int k = 0; k < EPT; ++k) {
+  for (int k = 0; k < EPT; ++k) {  // clamped index: one straight-line batch of loads
     const int e = min((int)threadIdx.x + k * kThreads, nel - 1);
     const int y = e / T, i = e - y * T;
     buf[k] = Zp[(long)y * nx + i];
   }
+  T2 *ltw = lds + T * ls, *lpre = ltw + ny;
+  stage_tables(ltw, py);
+  __syncthreads();
 #pragma unroll
   for (int k = 0; k < EPT; ++k) {
     const int e = threadIdx.x + k * kThreads;
     if (e < nel) {
       const int y = e / T, i = e - y * T;
-      lds[i * ls + lpad(y)] = INV ? cscale(buf[k], (decltype(buf[k].x))jsign(y, py.fixsign)) : cmul(buf[k], pre[y]);
+      lds[i * ls + lpad(y)] = INV ? cscale(buf[k], (decltype(buf[k].x))jsign(y, py.fixsign)) : cmul(buf[k], lpre[y]);
     }
   }
   __syncthreads();
-  fft_lines<EPT, INV>(lds, T, ls, py);
+  fft_lines<EPT, INV>(lds, T, ls, py, ltw);
 #pragma unroll
   for (int k = 0; k < EPT; ++k) {
     const int e = threadIdx.x + k * kThreads;
     if (e < nel) {
       const int y = e / T, i = e - y * T;
       T2 v = lds[i * ls + lpad(y)];
-      v = INV ? cmulc(v, pre[y]) : cscale(v, (decltype(v.x))jsign(y, py.fixsign));
+      v = INV ? cmulc(v, lpre[y]) : cscale(v, (decltype(v.x))jsign(y, py.fixsign));
       Zp[(long)y * nx + i] = v;
     }
   }
@@ -289,9 +294,17 @@ __global__ __launch_bounds__(kThreads) void k_fft_y(typename C2<R>::T *Z, const 
 
 // ---------------------------------------------------------------------
 // 3. FFT along z + k-space Ornstein-Zernike mixing + inverse FFT along z.
-// A workgroup owns tile A = (x in [xa, xa+T), y) and its mirror tile
+// A tile pair is A = (x in [xa, xa+T), y) and its mirror
 // B = (x in [nx-xa-T, nx-xa), ny-1-y), so every k and -k are local.
+// Tiles narrower than a 128-byte line share lines with their x neighbours:
+// workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2), so the tile order is remapped to give each XCD a contiguous run of
+// tiles, and neighbours meet in the same L2 instead of both reading HBM.
 // ---------------------------------------------------------------------
+#ifndef RISM_OZ_GROUP
+#define RISM_OZ_GROUP 2
+#endif
+constexpr int kXcds = 8;
 template <typename R, typename S, int EPT, int VC>
 __global__ __launch_bounds__(kThreads) void k_fft_z_oz(IterArgs<R, S> a) {
   using T2 = typename C2<R>::T;
@@ -302,131 +315,135 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_oz(IterArgs<R, S> a) {
   const int nx = a.nx, ny = a.ny, nz = a.nz, T = a.Tz, P = a.P, V = a.V;
   const long N = a.N, nxy = (long)nx * ny;
   const int ls = line_stride(nz);
-  const int xa = blockIdx.x * T, ya = blockIdx.y;
+  const int nxt = nx / T, ntile = nxt * (ny / 2);
+  const int b = blockIdx.x;
+  const int t = (ntile % kXcds) ? b : (b % kXcds) * (ntile / kXcds) + b / kXcds;
+  const int xa = (t % nxt) * T, ya = t / nxt;
   const int xb = nx - xa - T, yb = ny - 1 - ya;
-  const T2 *__restrict__ pre = (const T2 *)a.pz.pre;
   T2 *__restrict__ Z = a.Z;
   const int nel = 2 * P * T * nz;
-  // load both tiles of every pack; line = (p*2 + tile)*T + i
+  // element e of the tile pair: line (p*2 + tile)*T + i, position z
+  auto gofs = [&](int e) -> long {
+    const int i = e % T;
+    const int r = e / T;
+    const int z = r % nz, q = r / nz;
+    const int tile = q & 1, p = q >> 1;
+    return (long)p * N + (long)z * nxy + (tile ? (long)yb * nx + xb : (long)ya * nx + xa) + i;
+  };
   T2 buf[EPT];
 #pragma unroll
-  for (int k = 0; k < EPT; ++k) {
-    int r = min((int)threadIdx.x + k * kThreads, nel - 1);
-    const int i = r % T; r /= T;
-    const int z = r % nz; r /= nz;
-    const int tile = r & 1, p = r >> 1;
-    buf[k] = Z[(long)p * N + (long)z * nxy + (tile ? (long)yb * nx + xb : (long)ya * nx + xa) + i];
-  }
-#pragma unroll
-  for (int k = 0; k < EPT; ++k) {
-    const int e = threadIdx.x + k * kThreads;
-    if (e < nel) {
-      int r = e;
-      const int i = r % T; r /= T;
-      const int z = r % nz; r /= nz;
-      const int tile = r & 1, p = r >> 1;
-      lds[((p * 2 + tile) * T + i) * ls + lpad(z)] = cmul(buf[k], pre[z]);
-    }
-  }
+  for (int k = 0; k < EPT; ++k) buf[k] = Z[gofs(min((int)threadIdx.x + k * kThreads, nel - 1))];
+  T2 *ltw = lds + 2 * P * T * ls, *lpre = ltw + nz;
+  stage_tables(ltw, a.pz);
   __syncthreads();
-  fft_lines<EPT, false>(lds, 2 * P * T, ls, a.pz);
-  // k-space: pair (i, j) of tile A with (T-1-i, nz-1-j) of tile B.  The
-  // wave-vector arithmetic runs in the field precision R.
-  const R tpx = (R)(2.0 * M_PI / a.box[0]), tpz = (R)(2.0 * M_PI / a.box[2]);
-  const R ky = (R)(2.0 * M_PI / a.box[1] * ((double)ya - ny / 2.0 + 0.5));
-  const R dk = (R)a.dk, idk = (R)(1.0 / a.dk);
-  const R hx = (R)(0.5 - nx / 2), hz = (R)(0.5 - nz / 2);
-  const int nseg = a.ntab - 1;
-  const T4 *__restrict__ chi = a.chi;
-  const T2 *__restrict__ fkh = a.fkh;
-  const int npair = T * nz;
+  {
+    const int tid = threadIdx.x;
 #pragma unroll
-  for (int k = 0; k < (EPT + 1) / 2; ++k) {
-    // clamped index: loads and arithmetic are unconditional, the LDS stores
-    // below are guarded by the true index
-    const int e0 = threadIdx.x + k * kThreads;
-    const int e = min(e0, npair - 1);
-    {
-      const int j = e / T;
-      const int i = e - j * T;
-      const int ib = T - 1 - i, jb = nz - 1 - j;
-      const int x = xa + i;
-      const R kx = tpx * ((R)x + hx);
-      const R kz = tpz * ((R)j + hz);
-      const R kk = sqrt(kx * kx + ky * ky + kz * kz);
-      // spline interval on the uniform table k_i = dk (i + 1) (splint.cc:4-20)
-      int lo = (int)(kk * idk) - 1;
-      lo = max(0, min(lo, nseg - 1));
-      if (lo < nseg - 1 && dk * (R)(lo + 2) <= kk) ++lo;
-      if (lo > 0 && dk * (R)(lo + 1) > kk) --lo;
-      const R tt = kk - dk * (R)(lo + 1);
-      T4 cf[VC][VC];
-#pragma unroll
-      for (int v = 0; v < VC; ++v)
-#pragma unroll
-        for (int v1 = 0; v1 < VC; ++v1)
-          if (v < V && v1 < V) cf[v][v1] = chi[(long)(v * V + v1) * nseg + lo];
-      const T2 fk = a.use_fk ? fkh[((long)j * (ny / 2) + ya) * nx + x] : mk<T2>((R)0, (R)0);
-      const R sa = (R)jsign(j, a.pz.fixsign), sb = (R)jsign(jb, a.pz.fixsign);
-      T2 c[VC + 1];
-#pragma unroll
-      for (int p = 0; p < (VC + 1) / 2; ++p) {
-        if (p < P) {
-          const T2 za = cscale(lds[((p * 2 + 0) * T + i) * ls + lpad(j)], sa);
-          const T2 zb = cscale(lds[((p * 2 + 1) * T + ib) * ls + lpad(jb)], sb);
-          // C_{2p} = (za + conj zb)/2 ; C_{2p+1} = -i (za - conj zb)/2
-          c[2 * p] = mk<T2>((R)0.5 * (za.x + zb.x), (R)0.5 * (za.y - zb.y));
-          c[2 * p + 1] = mk<T2>((R)0.5 * (za.y + zb.y), (R)-0.5 * (za.x - zb.x));
-        }
+    for (int k = 0; k < EPT; ++k) {
+      const int e = tid + k * kThreads;
+      if (e < nel) {
+        const int i = e % T, z = (e / T) % nz, line = (e / T / nz) * T + i;
+        lds[line * ls + lpad(z)] = cmul(buf[k], lpre[z]);
       }
+    }
+    __syncthreads();
+    fft_lines<EPT, false>(lds, 2 * P * T, ls, a.pz, ltw);
+    // k-space: pair (i, j) of tile A with (T-1-i, nz-1-j) of tile B.  |k|
+    // and the spline interval are evaluated in FP64 in both precisions.
+    const double tpx = 2.0 * M_PI / a.box[0], tpz = 2.0 * M_PI / a.box[2];
+    const double ky = 2.0 * M_PI / a.box[1] * ((double)ya - ny / 2.0 + 0.5);
+    const double dk = a.dk, idk = 1.0 / a.dk;
+    const double hx = 0.5 - nx / 2, hz = 0.5 - nz / 2;
+    const int nseg = a.ntab - 1;
+    const T4 *__restrict__ chi = a.chi;
+    const T2 *__restrict__ fkh = a.fkh;
+    const int npair = T * nz;
 #pragma unroll
-      for (int v = 0; v < VC; ++v)
-        if (v < V) {
-          const R q = (R)a.qcf[v];
-          c[v] = mk<T2>(c[v].x - q * fk.x, c[v].y - q * fk.y);  // mqvfk (fk = 0 in delta mode)
-        }
-      T2 h[VC + 1];
+    for (int k = 0; k < (EPT + 1) / 2; ++k) {
+      // clamped index: loads and arithmetic are unconditional, the LDS stores
+      // below are guarded by the true index
+      const int e0 = tid + k * kThreads;
+      const int e = min(e0, npair - 1);
+      {
+        const int j = e / T;
+        const int i = e - j * T;
+        const int ib = T - 1 - i, jb = nz - 1 - j;
+        const int x = xa + i;
+        const double kx = tpx * ((double)x + hx);
+        const double kz = tpz * ((double)j + hz);
+        const double kk = sqrt(kx * kx + ky * ky + kz * kz);
+        // spline interval on the uniform table k_i = dk (i + 1) (splint.cc:4-20)
+        int lo = (int)(kk * idk) - 1;
+        lo = max(0, min(lo, nseg - 1));
+        if (lo < nseg - 1 && dk * (lo + 2) <= kk) ++lo;
+        if (lo > 0 && dk * (lo + 1) > kk) --lo;
+        const R tt = (R)(kk - dk * (lo + 1));
+        T4 cf[VC][VC];
 #pragma unroll
-      for (int v = 0; v < VC + 1; ++v) {
-        h[v] = mk<T2>((R)0, (R)0);
-        if (v < V && v < VC) {
-          R hr_ = 0, hi_ = 0;
+        for (int v = 0; v < VC; ++v)
 #pragma unroll
           for (int v1 = 0; v1 < VC; ++v1)
-            if (v1 < V) {  // oz: h_v = sum_v' chi[v][v'] c_v'
-              const T4 q4 = cf[v][v1];
-              const R xv = ((q4.w * tt + q4.z) * tt + q4.y) * tt + q4.x;
-              hr_ += c[v1].x * xv;
-              hi_ += c[v1].y * xv;
-            }
-          h[v] = mk<T2>(hr_, hi_);
-        }
-      }
+            if (v < V && v1 < V) cf[v][v1] = chi[(long)(v * V + v1) * nseg + lo];
+        const T2 fk = a.use_fk ? fkh[((long)j * (ny / 2) + ya) * nx + x] : mk<T2>((R)0, (R)0);
+        const R sa = (R)jsign(j, a.pz.fixsign), sb = (R)jsign(jb, a.pz.fixsign);
+        T2 c[VC + 1];
 #pragma unroll
-      for (int p = 0; p < (VC + 1) / 2; ++p) {
-        if (p < P && e0 < npair) {
-          const T2 h0 = h[2 * p], h1 = h[2 * p + 1];
-          // H(k) = h0 + i h1 ; H(-k) = conj(h0) + i conj(h1)
-          const T2 ha = mk<T2>(h0.x - h1.y, h0.y + h1.x);
-          const T2 hb = mk<T2>(h0.x + h1.y, h1.x - h0.y);
-          lds[((p * 2 + 0) * T + i) * ls + lpad(j)] = cscale(ha, sa);
-          lds[((p * 2 + 1) * T + ib) * ls + lpad(jb)] = cscale(hb, sb);
+        for (int p = 0; p < (VC + 1) / 2; ++p) {
+          if (p < P) {
+            const T2 za = cscale(lds[((p * 2 + 0) * T + i) * ls + lpad(j)], sa);
+            const T2 zb = cscale(lds[((p * 2 + 1) * T + ib) * ls + lpad(jb)], sb);
+            // C_{2p} = (za + conj zb)/2 ; C_{2p+1} = -i (za - conj zb)/2
+            c[2 * p] = mk<T2>((R)0.5 * (za.x + zb.x), (R)0.5 * (za.y - zb.y));
+            c[2 * p + 1] = mk<T2>((R)0.5 * (za.y + zb.y), (R)-0.5 * (za.x - zb.x));
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < VC; ++v)
+          if (v < V) {
+            const R q = (R)a.qcf[v];
+            c[v] = mk<T2>(c[v].x - q * fk.x, c[v].y - q * fk.y);  // mqvfk (fk = 0 in delta mode)
+          }
+        T2 h[VC + 1];
+#pragma unroll
+        for (int v = 0; v < VC + 1; ++v) {
+          h[v] = mk<T2>((R)0, (R)0);
+          if (v < V && v < VC) {
+            R hr_ = 0, hi_ = 0;
+#pragma unroll
+            for (int v1 = 0; v1 < VC; ++v1)
+              if (v1 < V) {  // oz: h_v = sum_v' chi[v][v'] c_v'
+                const T4 q4 = cf[v][v1];
+                const R xv = ((q4.w * tt + q4.z) * tt + q4.y) * tt + q4.x;
+                hr_ += c[v1].x * xv;
+                hi_ += c[v1].y * xv;
+              }
+            h[v] = mk<T2>(hr_, hi_);
+          }
+        }
+#pragma unroll
+        for (int p = 0; p < (VC + 1) / 2; ++p) {
+          if (p < P && e0 < npair) {
+            const T2 h0 = h[2 * p], h1 = h[2 * p + 1];
+            // H(k) = h0 + i h1 ; H(-k) = conj(h0) + i conj(h1)
+            const T2 ha = mk<T2>(h0.x - h1.y, h0.y + h1.x);
+            const T2 hb = mk<T2>(h0.x + h1.y, h1.x - h0.y);
+            lds[((p * 2 + 0) * T + i) * ls + lpad(j)] = cscale(ha, sa);
+            lds[((p * 2 + 1) * T + ib) * ls + lpad(jb)] = cscale(hb, sb);
+          }
         }
       }
+      // bound the pairs in flight (table loads are hoisted within a group)
+      if ((k + 1) % RISM_OZ_GROUP == 0) __builtin_amdgcn_sched_barrier(0);
     }
-  }
-  __syncthreads();
-  fft_lines<EPT, true>(lds, 2 * P * T, ls, a.pz);
+    __syncthreads();
+    fft_lines<EPT, true>(lds, 2 * P * T, ls, a.pz, ltw);
 #pragma unroll
-  for (int k = 0; k < EPT; ++k) {
-    const int e = threadIdx.x + k * kThreads;
-    if (e < nel) {
-      int r = e;
-      const int i = r % T; r /= T;
-      const int z = r % nz; r /= nz;
-      const int tile = r & 1, p = r >> 1;
-      Z[(long)p * N + (long)z * nxy + (tile ? (long)yb * nx + xb : (long)ya * nx + xa) + i] =
-          cmulc(lds[((p * 2 + tile) * T + i) * ls + lpad(z)], pre[z]);
+    for (int k = 0; k < EPT; ++k) {
+      const int e = tid + k * kThreads;
+      if (e < nel) {
+        const int i = e % T, z = (e / T) % nz, line = (e / T / nz) * T + i;
+        Z[gofs(e)] = cmulc(lds[line * ls + lpad(z)], lpre[z]);
+      }
     }
   }
 }
@@ -573,7 +590,9 @@ __global__ __launch_bounds__(kThreads) void k_fx_resid(IterArgs<R, S> a) {
   const int nel = L * nx;
   const int groups = (int)((long)a.ny * a.nz / L);
   const int nf = ctl->filled < HM ? ctl->filled : HM;
-  const T2 *__restrict__ pre = (const T2 *)a.px.pre;
+  T2 *ltw = lds + P * L * ls;
+  const T2 *pre = ltw + nx;
+  stage_tables(ltw, a.px);  // visible after the first barrier of the group loop
   const R ivol = (R)a.ivol;
   T2 *__restrict__ Z = a.Z;
   const S *__restrict__ ts = a.ts;
@@ -609,7 +628,7 @@ __global__ __launch_bounds__(kThreads) void k_fx_resid(IterArgs<R, S> a) {
       }
     }
     __syncthreads();
-    fft_lines<EPT, true>(lds, P * L, ls, a.px, tid);
+    fft_lines<EPT, true>(lds, P * L, ls, a.px, ltw, tid);
     // points per thread: nel <= EPT * kThreads / P and P >= PMIN in this site class
     constexpr int PMIN = VC <= 2 ? 1 : (VC <= 4 ? 2 : 3);
     constexpr int KPT = (EPT + PMIN - 1) / PMIN;
@@ -720,7 +739,10 @@ __global__ __launch_bounds__(kThreads) void k_fft_x_pure(typename C2<R>::T *Z, i
   T2 *lds = reinterpret_cast<T2 *>(smem_raw);
   const int ls = line_stride(nx), nel = L * nx;
   const long base = (long)blockIdx.x * nel;
-  const T2 *pre = (const T2 *)px.pre;
+  T2 *ltw = lds + L * ls;
+  const T2 *pre = ltw + nx;
+  stage_tables(ltw, px);
+  __syncthreads();
   for (int e = threadIdx.x; e < nel; e += kThreads) {
     const int i = e / nx, x = e - i * nx;
     T2 v = Z[base + e];
@@ -728,7 +750,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_x_pure(typename C2<R>::T *Z, i
     lds[i * ls + lpad(x)] = v;
   }
   __syncthreads();
-  fft_lines<EPT, INV>(lds, L, ls, px);
+  fft_lines<EPT, INV>(lds, L, ls, px, ltw);
   for (int e = threadIdx.x; e < nel; e += kThreads) {
     const int i = e / nx, x = e - i * nx;
     T2 v = lds[i * ls + lpad(x)];
@@ -746,7 +768,10 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_pure(typename C2<R>::T *Z, i
   const int ls = line_stride(nz), nel = T * nz;
   const long nxy = (long)nx * ny;
   T2 *Zp = Z + (long)blockIdx.y * nx + blockIdx.x * T;
-  const T2 *pre = (const T2 *)pz.pre;
+  T2 *ltw = lds + T * ls;
+  const T2 *pre = ltw + nz;
+  stage_tables(ltw, pz);
+  __syncthreads();
   for (int e = threadIdx.x; e < nel; e += kThreads) {
     const int z = e / T, i = e - z * T;
     T2 v = Zp[(long)z * nxy + i];
@@ -754,7 +779,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_pure(typename C2<R>::T *Z, i
     lds[i * ls + lpad(z)] = v;
   }
   __syncthreads();
-  fft_lines<EPT, INV>(lds, T, ls, pz);
+  fft_lines<EPT, INV>(lds, T, ls, pz, ltw);
   for (int e = threadIdx.x; e < nel; e += kThreads) {
     const int z = e / T, i = e - z * T;
     T2 v = lds[i * ls + lpad(z)];
@@ -816,20 +841,20 @@ hipError_t launch_closure_fx(const IterArgs<R, S> &a, const LaunchGeom &g, bool 
 
 template <typename R, typename S>
 hipError_t launch_fft_y(const IterArgs<R, S> &a, const LaunchGeom &g, bool inverse, hipStream_t st) {
-  const dim3 grid(a.nx / g.Ty, a.nz, a.P), block(kThreads);
+  const dim3 grid(a.nx / g.Ty * a.nz * a.P), block(kThreads);
   if (inverse) {
     RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, true>), grid, block, g.lds_y, st, a.Z,
-                                                a.ctl, a.iter, a.nx, a.ny, a.N, g.Ty, a.py))
+                                                a.ctl, a.iter, a.nx, a.ny, a.nz, a.N, g.Ty, a.P, a.py))
   } else {
     RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, false>), grid, block, g.lds_y, st, a.Z,
-                                                a.ctl, a.iter, a.nx, a.ny, a.N, g.Ty, a.py))
+                                                a.ctl, a.iter, a.nx, a.ny, a.nz, a.N, g.Ty, a.P, a.py))
   }
   return hipGetLastError();
 }
 
 template <typename R, typename S>
 hipError_t launch_fft_z_oz(const IterArgs<R, S> &a, const LaunchGeom &g, hipStream_t st) {
-  const dim3 grid(a.nx / g.Tz, a.ny / 2), block(kThreads);
+  const dim3 grid(a.nx / g.Tz * (a.ny / 2)), block(kThreads);
   RISM_VC_SWITCH(a.V, RISM_EPT_SWITCH(g.ept_z, hipLaunchKernelGGL((k_fft_z_oz<R, S, E, VC>), grid, block, g.lds_z, st, a)))
   return hipGetLastError();
 }
@@ -867,19 +892,19 @@ hipError_t launch_fft3d(typename C2<R>::T *data, const AxisPlan &px, const AxisP
   const long N = (long)nx * ny * nz;
   const int L = g.Lx;
   const dim3 block(kThreads);
-  const dim3 gx((unsigned)((long)ny * nz / L)), gy(nx / g.Ty, nz, 1), gz(nx / g.Tz, ny);
+  const dim3 gx((unsigned)((long)ny * nz / L)), gy(nx / g.Ty * nz), gz(nx / g.Tz, ny);
   const size_t sz = sizeof(typename C2<R>::T);
-  const size_t ldsx = (size_t)L * line_stride(nx) * sz, ldsy = (size_t)g.Ty * line_stride(ny) * sz,
-               ldsz = (size_t)g.Tz * line_stride(nz) * sz;
+  const size_t ldsx = ((size_t)L * line_stride(nx) + 2 * nx) * sz, ldsy = ((size_t)g.Ty * line_stride(ny) + 2 * ny) * sz,
+               ldsz = ((size_t)g.Tz * line_stride(nz) + 2 * nz) * sz;
   // pure z tiles hold T lines: EPT class from T*nz
   const int ez = (g.Tz * nz <= 4 * kThreads) ? 4 : (g.Tz * nz <= 8 * kThreads) ? 8 : (g.Tz * nz <= 16 * kThreads) ? 16 : 32;
   if (dir < 0) {
     RISM_EPTX_SWITCH(g.ept_x, hipLaunchKernelGGL((k_fft_x_pure<R, E, false>), gx, block, ldsx, st, data, nx, L, px, (R)dv))
-    RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, false>), gy, block, ldsy, st, data, (const Ctl *)nullptr, 0, nx, ny, N, g.Ty, py))
+    RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, false>), gy, block, ldsy, st, data, (const Ctl *)nullptr, 0, nx, ny, nz, N, g.Ty, 1, py))
     RISM_EPT_SWITCH(ez, hipLaunchKernelGGL((k_fft_z_pure<R, E, false>), gz, block, ldsz, st, data, nx, ny, nz, g.Tz, pz))
   } else {
     RISM_EPT_SWITCH(ez, hipLaunchKernelGGL((k_fft_z_pure<R, E, true>), gz, block, ldsz, st, data, nx, ny, nz, g.Tz, pz))
-    RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, true>), gy, block, ldsy, st, data, (const Ctl *)nullptr, 0, nx, ny, N, g.Ty, py))
+    RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, true>), gy, block, ldsy, st, data, (const Ctl *)nullptr, 0, nx, ny, nz, N, g.Ty, 1, py))
     RISM_EPTX_SWITCH(g.ept_x, hipLaunchKernelGGL((k_fft_x_pure<R, E, true>), gx, block, ldsx, st, data, nx, L, px, (R)ivol))
   }
   return hipGetLastError();

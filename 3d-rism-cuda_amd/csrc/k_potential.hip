@@ -19,6 +19,8 @@
 
 namespace rism {
 
+// One grid point per thread; 1/r comes from rsqrt_f64 and gives r = r^2/r
+// and 1/r^2 without a division or a square root.
 __global__ __launch_bounds__(kThreads) void k_potentials(PotArgs a) {
   const long ip = (long)blockIdx.x * kThreads + threadIdx.x;
   if (ip >= a.N) return;
@@ -29,7 +31,7 @@ __global__ __launch_bounds__(kThreads) void k_potentials(PotArgs a) {
   const double rx = (x - nx / 2) * a.dr[0];
   const double ry = (y - ny / 2) * a.dr[1];
   const double rz = (z - nz / 2) * a.dr[2];
-  const double cut2 = 1.0e-2 * 1.0e-2;
+  const double cut2 = 1.0e-2 * 1.0e-2;  // LJ distance floor 0.01 A (cal_LJ.cu:63)
   double lj[kMaxV];
 #pragma unroll
   for (int v = 0; v < kMaxV; ++v) lj[v] = 0.0;
@@ -39,8 +41,8 @@ __global__ __launch_bounds__(kThreads) void k_potentials(PotArgs a) {
     const double4 at = a.atoms[u];
     const double dx = rx - at.x, dy = ry - at.y, dz = rz - at.z;
     const double r2 = dx * dx + dy * dy + dz * dz;
-    const double r2c = r2 < cut2 ? cut2 : r2;
-    const double ir2 = 1.0 / r2c;
+    const double rinv = rsqrt_f64(fmax(r2, 1.0e-300));
+    const double ir2 = r2 < cut2 ? 1.0 / cut2 : rinv * rinv;
 #pragma unroll
     for (int v = 0; v < kMaxV; ++v)
       if (v < V) {
@@ -49,9 +51,9 @@ __global__ __launch_bounds__(kThreads) void k_potentials(PotArgs a) {
         const double s6 = s2 * s2 * s2;
         lj[v] += p.y * s6 * (s6 - 1.0);
       }
-    const double ra = sqrt(r2);
+    const double ra = r2 * rinv;
     if (ra >= 1.0e-5) {
-      const double qr = at.w / ra;
+      const double qr = at.w * rinv;
       e += qr;
       if (ra < 38.0) {
         const double ex = exp(-ra);

@@ -85,24 +85,34 @@ __global__ __launch_bounds__(kThreads) void k_euv(int nx, int ny, int nz, double
   for (long ip = (long)blockIdx.x * kThreads + threadIdx.x; ip < N; ip += (long)gridDim.x * kThreads) {
     const int x = (int)(ip % nx), y = (int)((ip / nx) % ny), z = (int)(ip / ((long)nx * ny));
     const double rx = (x - nx / 2) * drx, ry = (y - ny / 2) * dry, rz = (z - nz / 2) * drz;
-    double gv[kMaxV];
+    // per-site sums over atoms; the point's factors rho_v g_v (q_v cc) are applied once
+    double alj[kMaxV], aes[kMaxV];
 #pragma unroll
-    for (int v = 0; v < kMaxV; ++v) gv[v] = (v < V) ? g[(long)v * N + ip] : 0.0;
+    for (int v = 0; v < kMaxV; ++v) alj[v] = aes[v] = 0.0;
     for (int u = 0; u < natu; ++u) {
       const double4 at = atoms[u];
       const double dx = rx - at.x, dy = ry - at.y, dz = rz - at.z;
       const double r2 = dx * dx + dy * dy + dz * dz;
-      const double r1 = sqrt(r2);
-      const double ir1 = 1.0 / r1, ir2 = 1.0 / r2;
+      const double ir1 = rsqrt_f64(fmax(r2, 1.0e-300));
+      const double r1 = r2 * ir1, ir2 = ir1 * ir1;
+      const double qr = at.w * ir1;
 #pragma unroll
       for (int v = 0; v < kMaxV; ++v) {
         if (v < V) {
           const double2 p = lj[v * natu + u];  // (sigma^2, 4 eps)
-          if (r1 < sigh[v * natu + u]) continue;
+          const bool in = r1 >= sigh[v * natu + u];  // r < sigma_uv / 2 is excluded
           const double s2 = p.x * ir2, s6 = s2 * s2 * s2;
-          elj += rho[v] * p.y * s6 * (s6 - 1.0) * gv[v];
-          ees += rho[v] * at.w * qv[v] * ir1 * cc * gv[v];
+          alj[v] += in ? p.y * s6 * (s6 - 1.0) : 0.0;
+          aes[v] += in ? qr : 0.0;
         }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < kMaxV; ++v) {
+      if (v < V) {
+        const double w = rho[v] * g[(long)v * N + ip];
+        elj += w * alj[v];
+        ees += w * qv[v] * cc * aes[v];
       }
     }
   }
@@ -146,14 +156,15 @@ __global__ __launch_bounds__(kThreads) void k_grad_partial(int nx, int ny, int n
       const double dy = (y - ny / 2) * dry - at.y;
       const double dz = (z - nz / 2) * drz - at.z;
       const double r2 = dx * dx + dy * dy + dz * dz;
-      const double r1 = sqrt(r2);
+      const double ir1 = rsqrt_f64(fmax(r2, 1.0e-300));
+      const double r1 = r2 * ir1, ir2 = ir1 * ir1;
       for (int v = 0; v < V; ++v) {
         if (r1 < sigh[v * natu + u]) continue;  // r < sigma_uv / 2 excluded
         const double2 p = lj[v * natu + u];
         const double gg = g[(long)v * N + ip];
-        const double rs2i = p.x / r2, rs6i = rs2i * rs2i * rs2i;
-        const double flj = rho[v] * (p.y * 6.0) * rs6i / r2 * (2.0 * rs6i - 1.0) * gg;
-        const double fes = rho[v] * at.w * qv[v] / (r2 * r1) * cc * gg;
+        const double rs2i = p.x * ir2, rs6i = rs2i * rs2i * rs2i;
+        const double flj = rho[v] * (p.y * 6.0) * rs6i * ir2 * (2.0 * rs6i - 1.0) * gg;
+        const double fes = rho[v] * at.w * qv[v] * (ir2 * ir1) * cc * gg;
         a6[0] += flj * dx; a6[1] += flj * dy; a6[2] += flj * dz;
         a6[3] += fes * dx; a6[4] += fes * dy; a6[5] += fes * dz;
       }

@@ -25,6 +25,17 @@ namespace rism {
 constexpr int kMaxV = 8;    // reduced solvent sites
 constexpr int kHMax = 16;   // Anderson history slots
 
+// 1/sqrt(x) in FP64 for x > 0: the hardware estimate refined by two Newton
+// steps y <- y + y (1 - x y^2) / 2 (each roughly doubles the correct bits),
+// within a few ulp of 1/sqrt(x) and far cheaper than sqrt plus a division.
+__device__ __forceinline__ double rsqrt_f64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double e = fma(-x * y, y, 1.0);
+  y = fma(0.5 * y, e, y);
+  e = fma(-x * y, y, 1.0);
+  return fma(0.5 * y, e, y);
+}
+
 // Solver control block in device memory.  Written by the single-workgroup
 // solve kernel, read by the field kernels to decide whether they run.
 struct Ctl {
@@ -107,7 +118,7 @@ struct LaunchGeom {
   int ept_x, ept_y, ept_z;  // elements-per-thread class (4, 8, 16, 32)
   int xgroups;              // number of x line groups (ny*nz / Lx)
   int xblocks;              // residual-pass grid size
-  size_t lds_x, lds_y, lds_z;
+  size_t lds_x, lds_y, lds_z;  // bytes, including any staged tables
 };
 
 // ---- iteration launchers (k_iterate.hip) ----

@@ -278,9 +278,11 @@ int choose_geometry(rism_handle *h, LaunchGeom &g, size_t rsz) {
   const long cap = std::min<long>(env_long("RISM_LDS_CAP", 65536), lds_max);
   const long xe = env_long("RISM_X_ELEMS", 2048);
   const long lsx = line_stride(nx), lsy = line_stride(ny), lsz = line_stride(nz);
-  g.Lx = largest_divisor_le((long)ny * nz, std::max(1L, std::min(xe / ((long)P * nx), cap / ((long)P * lsx * c2))));
-  g.Ty = largest_divisor_le(nx, std::max(1L, std::min(8192L / ny, cap / (lsy * c2))));
-  g.Tz = largest_divisor_le(nx, std::max(1L, std::min(8192L / (2L * P * nz), cap / (2L * P * lsz * c2))));
+  g.Lx = largest_divisor_le((long)ny * nz,
+                            std::max(1L, std::min(xe / ((long)P * nx), (cap - 2L * nx * c2) / ((long)P * lsx * c2))));
+  // the y and z passes also stage their twiddle and phase tables (2n complex) in LDS
+  g.Ty = largest_divisor_le(nx, std::max(1L, std::min(8192L / ny, (cap - 2L * ny * c2) / (lsy * c2))));
+  g.Tz = largest_divisor_le(nx, std::max(1L, std::min(8192L / (2L * P * nz), (cap - 2L * nz * c2) / (2L * P * lsz * c2))));
   const long lx = env_long("RISM_LX", 0), ty = env_long("RISM_TY", 0), tz = env_long("RISM_TZ", 0);
   if (lx > 0 && ((long)ny * nz) % lx == 0) g.Lx = (int)lx;
   if (ty > 0 && nx % ty == 0) g.Ty = (int)ty;
@@ -288,11 +290,11 @@ int choose_geometry(rism_handle *h, LaunchGeom &g, size_t rsz) {
   g.ept_x = ept_class((long)P * g.Lx * nx);
   g.xgroups = (int)((long)ny * nz / g.Lx);
   g.xblocks = std::min<int>(g.xgroups, (int)env_long("RISM_XBLOCKS", 4096));
-  g.lds_x = (size_t)P * g.Lx * lsx * c2;
+  g.lds_x = ((size_t)P * g.Lx * lsx + 2 * (size_t)nx) * c2;
   g.ept_y = ept_class((long)g.Ty * ny);
-  g.lds_y = (size_t)g.Ty * lsy * c2;
+  g.lds_y = ((size_t)g.Ty * lsy + 2 * (size_t)ny) * c2;
   g.ept_z = ept_class(2L * P * g.Tz * nz);
-  g.lds_z = (size_t)2 * P * g.Tz * lsz * c2;
+  g.lds_z = ((size_t)2 * P * g.Tz * lsz + 2 * (size_t)nz) * c2;
   if (g.ept_x < 0 || g.ept_x > 16 || g.ept_y < 0 || g.ept_z < 0 || (long)g.lds_x > cap || (long)g.lds_y > cap ||
       (long)g.lds_z > cap) {
     set_error("grid too large for the single-workgroup line transforms");

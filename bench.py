@@ -36,6 +36,7 @@ import torch  # noqa: E402,F401  (first: its HIP runtime then serves the solver 
 METRIC = "s-to-converge 256^3 KH, mixed vs FP64 at Δμ≤0.01 kcal/mol; % HBM roofline"
 SOLVENT = os.path.join(REPO, "tests", "golden", "reference_examples", "h2o_vv_16384.xvk")
 HBM_PEAK_GBS = 8000.0
+COPY_BYTES = 256 << 20  # calibration copy: reads and writes this many bytes
 
 
 def kernel_bytes(V, P, H, mixed):
@@ -141,6 +142,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline sample")
     ap.add_argument("--mixed-only", action="store_true", help="time only the mixed path")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run a plain device copy of 256 MiB three times (the reference kernel "
+                         "for scaling FETCH_SIZE / WRITE_SIZE counters to bytes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -152,6 +156,13 @@ def main():
     if lib.rism_device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.calibrate_copy:
+        src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     w = configs.config(1)
     solvent = rism3d.read_xvk(SOLVENT)
     V = solvent.nsite
