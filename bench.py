@@ -37,6 +37,7 @@ METRIC = "s-to-converge 256^3 KH, mixed vs FP64 at Δμ≤0.01 kcal/mol; % HBM r
 SOLVENT = os.path.join(REPO, "tests", "golden", "reference_examples", "h2o_vv_16384.xvk")
 HBM_PEAK_GBS = 8000.0
 COPY_BYTES = 256 << 20  # calibration copy: reads and writes this many bytes
+TRAFFIC = os.path.join(REPO, "profiles", "iteration_kernels_256cubed_mixed.json")
 
 
 def kernel_bytes(V, P, H, mixed):
@@ -64,6 +65,17 @@ def kernel_bytes(V, P, H, mixed):
         # spectrum, [h_ref], t_s, u, H history R_i in; R out
         "resid": P * c + (16 * P if mixed else 0) + V * (s_t + s_f + H * s_h + s_h),
     }
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (scaled by its copy-kernel
+    calibration); None when no summary covers it."""
+    try:
+        with open(TRAFFIC) as f:
+            return json.load(f)["traffic_bytes_per_launch"].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def per_kernel_table(kt, kb, N):
@@ -179,7 +191,7 @@ def main():
     bytes_launch = kb[dom] * N
     achieved = bytes_launch / avg_s / 1e9
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
             "bytes_per_launch": bytes_launch, "avg_launch_us": round(avg_s * 1e6, 2)}
     per_kernel = per_kernel_table(mixed["kernels"], kb, N)
     cpu = None
