@@ -29,7 +29,8 @@ int main(int argc, char **argv) {
   int cu = 0, centering = 1, zero = 0, nhist = 2, threads = 0, cap = 0, json = 0, quiet = 0;
   const char *input = NULL, *structure = NULL;
   int ch;
-  while ((ch = getopt(argc, argv, "c:d:i:s:fzH:T:n:jq")) != -1) {
+  long sdiv = 1;
+  while ((ch = getopt(argc, argv, "c:d:i:s:fzH:T:n:x:jq")) != -1) {
     switch (ch) {
       case 'c': cu = atoi(optarg); break;
       case 'd': break;  /* device index: meaningless for the host oracle */
@@ -40,6 +41,7 @@ int main(int argc, char **argv) {
       case 'H': nhist = atoi(optarg); break;
       case 'T': threads = atoi(optarg); break;
       case 'n': cap = atoi(optarg); break;
+      case 'x': sdiv = atol(optarg); break;  /* timing sample: 1/x of the points for potentials and SE */
       case 'j': json = 1; break;
       case 'q': quiet = 1; break;
       default: return 1;
@@ -70,8 +72,13 @@ int main(int argc, char **argv) {
   char *dot = strrchr(fname, '.');
   if (dot) *dot = '\0';
   if (orc_read_xvk(s, s->fsolvent)) { printf("%s\n", orc_last_error()); return 1; }
+  for (int d = 0; d < 3; ++d)
+    if (s->n[d] < 2 || (s->n[d] & 1)) { printf("grid sizes must be even\n"); return 1; }
+  if (sdiv > 1) s->sample = s->ng / sdiv;
   double t0 = now();
-  if (orc_setup(s)) { printf("%s\n", orc_last_error()); return 1; }
+  if (orc_kgrid(s) || orc_spline_chi(s)) { printf("%s\n", orc_last_error()); return 1; }
+  double t0p = now();
+  if (orc_potentials(s)) { printf("%s\n", orc_last_error()); return 1; }
   double t1 = now();
   char tuv[4200];
   snprintf(tuv, sizeof tuv, "%s.tuv", fname);
@@ -86,10 +93,10 @@ int main(int argc, char **argv) {
   snprintf(out, sizeof out, "%s%s", fname, zero ? ".xmu0" : ".xmu");
   if (strchr(s->outlist, 'm') || strchr(s->outlist, 'M')) orc_write_xmu(s, &x, out);
   if (json)
-    printf("{\"status\": %d, \"steps\": %d, \"rms\": %.6e, \"t_setup\": %.6f, "
-           "\"t_iter\": %.6f, \"t_thermo\": %.6f, \"threads\": %d, \"ng\": %ld, "
+    printf("{\"status\": %d, \"steps\": %d, \"rms\": %.6e, \"t_setup\": %.6f, \"t_pot\": %.6f, "
+           "\"t_iter\": %.6f, \"t_thermo\": %.6f, \"threads\": %d, \"ng\": %ld, \"sample_div\": %ld, "
            "\"sfe_sc\": %.10f, \"sfe_gf\": %.10f, \"pmv\": %.10f}\n",
-           st, steps, rms, t1 - t0, t2 - t1, t3 - t2, nthr, s->ng, x.sfe_sc, x.sfe_gf, x.pmv);
+           st, steps, rms, t1 - t0, t1 - t0p, t2 - t1, t3 - t2, nthr, s->ng, sdiv, x.sfe_sc, x.sfe_gf, x.pmv);
   orc_free(s);
   return st < 0 ? 1 : 0;
 }

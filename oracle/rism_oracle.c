@@ -423,8 +423,9 @@ int orc_potentials(orc_sys *s) {
   const double cut2 = 1.0e-2 * 1.0e-2;
   const double ikbt = 1.0 / (ORC_NA * ORC_KB * s->temper);
   const double ubeta = ORC_HARTREE * ORC_BOHR / (ORC_KB * s->temper);
+  const long Np = (s->sample > 0 && s->sample < N) ? s->sample : N;
 #pragma omp parallel for schedule(static)
-  for (long ip = 0; ip < N; ++ip) {
+  for (long ip = 0; ip < Np; ++ip) {
     int x = (int)(ip % nx), y = (int)((ip / nx) % ny), z = (int)(ip / ((long)nx * ny));
     double rx = (x - nx / 2) * s->dr[0];
     double ry = (y - ny / 2) * s->dr[1];
@@ -974,12 +975,13 @@ static void euv_sum(const orc_sys *s, int v, int u, double sig, double eps,
   long N = s->ng;
   int nx = s->n[0], ny = s->n[1];
   const double cc = ORC_HARTREE * ORC_BOHR * ORC_NA;
-  long nb = (N + ORC_BLK - 1) / ORC_BLK;
+  const long Np = (s->sample > 0 && s->sample < N) ? s->sample : N;
+  long nb = (Np + ORC_BLK - 1) / ORC_BLK;
   double *pl = xcalloc(nb, sizeof(double)), *pe = xcalloc(nb, sizeof(double));
   const double complex *g = s->guv + (size_t)v * N;
   double qq = s->qu[u] * s->qv[v];
   for (long b = 0; b < nb; ++b) {
-    long e = (b + 1) * ORC_BLK < N ? (b + 1) * ORC_BLK : N;
+    long e = (b + 1) * ORC_BLK < Np ? (b + 1) * ORC_BLK : Np;
     double al = 0.0, ae = 0.0;
     for (long ip = b * ORC_BLK; ip < e; ++ip) {
       int x = (int)(ip % nx), y = (int)((ip / nx) % ny), z = (int)(ip / ((long)nx * ny));

@@ -48,6 +48,8 @@ typedef struct orc_sys {
   /* cell (cell.cc:8-22) */
   double dr[3], dv, volume;
   long ng;                /* nx*ny*nz */
+  long sample;            /* > 0: potentials and SE cover only the first `sample`
+                             points (CPU-baseline timing sample; results partial) */
   /* solute (solute.cu) */
   int natu;
   double *sigu, *epsu, *qu, *ru;  /* ru: [natu][3] */
