@@ -133,6 +133,36 @@ template <typename R, typename S>
 hipError_t launch_fx_resid(const IterArgs<R, S> &a, const LaunchGeom &g, int hm, hipStream_t st);
 hipError_t launch_anderson_solve(Ctl *ctl, const double *partials, int nblk, int nv, int hm,
                                  double *rms_hist, int iter, hipStream_t st);
+// ---- adjoint of the converged fixed point (k_adjoint.hip) ----
+struct AdjArgs {
+  long N;
+  int nx, ny, nz, V, closure, natu, ntab;
+  double box[3], dr[3], dv, dk, kt, ikbt, ubeta;
+  double qcf[kMaxV], qv[kMaxV], rho[kMaxV], rho_inv[kMaxV];
+  const double *ts, *u, *fr;   // converged state, FP64 potential, long-range field
+  const double2 *hz;           // real-space h, packed two sites per complex
+  const double4 *chi;          // spline table [V*V][ntab-1]
+  double *gp, *alpha, *beta, *b;       // [V][N]
+  double *lam, *res, *hist_l, *hist_r; // adjoint state, residual, history [H][V][N]
+  double *ubar, *debar, *frbar;        // field adjoints
+  Ctl *ctl;
+  double *partials;
+  const double4 *atoms;        // x, y, z, q
+  const double2 *lj;           // [V][natu] (sigma_uv^2, 4 eps_uv)
+  const double2 *ljd;          // [V][natu] (1 / (2 sigma_uv), 4 eps_uv / (2 eps_u))
+  const double2 *phase;        // [natu][nx+ny+nz] exp(-i k R) per axis
+};
+hipError_t launch_adj_fields(const AdjArgs &a, hipStream_t st);
+hipError_t launch_adj_pack(const AdjArgs &a, const double *x, const double *y, double2 *Z, hipStream_t st);
+hipError_t launch_adj_kspace(const AdjArgs &a, double2 *Z, double2 *Y, hipStream_t st);
+hipError_t launch_adj_rhs(const AdjArgs &a, const double2 *Z, hipStream_t st);
+hipError_t launch_adj_update(const AdjArgs &a, int hm, int iter, hipStream_t st);
+hipError_t launch_adj_resid(const AdjArgs &a, const double2 *Z, int hm, int nblk, int iter, hipStream_t st);
+hipError_t launch_adj_bar(const AdjArgs &a, const double2 *Z, hipStream_t st);
+int adj_grad_blocks(long N);
+hipError_t launch_adj_grad(const AdjArgs &a, const double2 *Y, double *part6, double *part4, double *out6,
+                           double *out4, hipStream_t st);
+
 // standalone 3D FFT on a caller buffer (reference FFT3D::execute seam)
 template <typename R>
 hipError_t launch_fft3d(typename C2<R>::T *data, const AxisPlan &px, const AxisPlan &py,

@@ -114,6 +114,14 @@ struct rism_handle {
   double4 *atoms = nullptr;
   double2 *lj = nullptr;
   double *sigh = nullptr, *d_qv = nullptr, *d_rho = nullptr;
+  // adjoint (allocated on the first rism_adjoint_grad)
+  double *adj_f = nullptr;       // gp, alpha, beta, b, lam, res, ubar: 7 x [V][N]
+  double *adj_hist = nullptr;    // hist_l, hist_r: 2 x [nhist][V][N]
+  double *adj_n = nullptr;       // debar, frbar: 2 x [N]
+  double2 *adj_Z = nullptr, *adj_Y = nullptr;
+  double2 *adj_ljd = nullptr;
+  double *adj_rms = nullptr;
+  Ctl *adj_ctl = nullptr;
   // state
   bool potentials = false, solved = false, evaluated = false;
   double cf = 1.0;
@@ -1023,6 +1031,152 @@ int rism_grad(rism_handle *h, double *out) {
   // kernel layout per atom: LJ xyz then ES xyz, already x dv (output_grad.cu:15-24)
   std::memcpy(out, g.data(), g.size() * sizeof(double));
   return RISM_OK;
+}
+
+// Adjoint of the converged fixed point (FP64 in both precisions); see
+// k_adjoint.hip for the equations.  The adjoint system
+//   lambda = (g' - 1)(K^T lambda - lambda) + b
+// is the transpose of the forward map's linearisation and is solved with
+// the same modified Anderson scheme, on its own control block, to
+// rms(residual) <= rtol * rms(b).
+int rism_adjoint_grad(rism_handle *h, double rtol, double *dxyz, double *dq, double *dsig, double *deps,
+                      int *steps) {
+  if (!h) { set_error("null handle"); return RISM_ERR_ARG; }
+  if (!h->solved) { set_error("rism_solve must run before rism_adjoint_grad"); return RISM_ERR_STATE; }
+  if (!(rtol > 0.0)) { set_error("rtol must be positive"); return RISM_ERR_ARG; }
+  HIP_OK(hipSetDevice(h->device));
+  if (!h->evaluated)
+    if (int rc = evaluate(h)) return rc;
+  const long N = h->N;
+  const int V = h->V, P = h->P, nu = h->natu, nh = h->opts.nhist;
+  const size_t VN = (size_t)V * N;
+  PrecSet &ps = h->ps[0];
+  int rc = 0;
+  if (!h->adj_f) {
+    if ((rc = dalloc_t(h, &h->adj_f, 7 * VN)) || (rc = dalloc_t(h, &h->adj_hist, 2 * (size_t)nh * VN)) ||
+        (rc = dalloc_t(h, &h->adj_n, 2 * (size_t)N)) || (rc = dalloc_t(h, &h->adj_Z, (size_t)P * N)) ||
+        (rc = dalloc_t(h, &h->adj_Y, (size_t)N)) || (rc = dalloc_t(h, &h->adj_ljd, (size_t)V * std::max(1, nu))) ||
+        (rc = dalloc_t(h, &h->adj_rms, (size_t)h->opts.maxstep + 4)) || (rc = dalloc_t(h, &h->adj_ctl, 1)))
+      return rc;
+  }
+  // (1 / (2 sigma_uv), 4 eps_uv / (2 eps_u)): Lorentz-Berthelot derivatives
+  std::vector<double2> ljd((size_t)V * std::max(1, nu));
+  for (int v = 0; v < V; ++v)
+    for (int u = 0; u < nu; ++u) {
+      const double s = (h->usig[u] + h->sigv[v]) * 0.5;
+      const double e = std::sqrt(h->ueps[u] * h->epsv[v]);
+      ljd[(size_t)v * nu + u] = make_double2(s > 0.0 ? 0.5 / s : 0.0, h->ueps[u] > 0.0 ? 2.0 * e / h->ueps[u] : 0.0);
+    }
+  HIP_OK(hipMemcpyAsync(h->adj_ljd, ljd.data(), ljd.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+  AdjArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.N = N; a.nx = h->nx; a.ny = h->ny; a.nz = h->nz; a.V = V; a.closure = h->opts.closure; a.natu = nu;
+  a.ntab = h->ntab;
+  for (int k = 0; k < 3; ++k) { a.box[k] = h->box[k]; a.dr[k] = h->dr[k]; }
+  a.dv = h->dv; a.dk = h->dk;
+  a.kt = kAvogadro * kBoltzmann * h->temper;
+  a.ikbt = 1.0 / a.kt;
+  a.ubeta = kHartree * kBohr / (kBoltzmann * h->temper);
+  for (int v = 0; v < V; ++v) {
+    a.qcf[v] = h->qv[v] * h->cf;
+    a.qv[v] = h->qv[v];
+    a.rho[v] = h->rho[v];
+    a.rho_inv[v] = 1.0 / h->rho[v];
+  }
+  a.ts = h->ts; a.u = (const double *)ps.u; a.fr = h->fr;
+  a.hz = (const double2 *)ps.Z; a.chi = (const double4 *)ps.chi;
+  a.gp = h->adj_f; a.alpha = a.gp + VN; a.beta = a.alpha + VN; a.b = a.beta + VN;
+  a.lam = a.b + VN; a.res = a.lam + VN; a.ubar = a.res + VN;
+  a.hist_l = h->adj_hist; a.hist_r = h->adj_hist + (size_t)nh * VN;
+  a.debar = h->adj_n; a.frbar = h->adj_n + N;
+  a.ctl = h->adj_ctl; a.partials = h->partials;
+  a.atoms = h->atoms; a.lj = h->lj; a.ljd = h->adj_ljd; a.phase = h->phase;
+  hipStream_t st = h->stream;
+  const double ivol = 1.0 / h->volume;
+  auto fft = [&](int dir) -> hipError_t {
+    for (int p = 0; p < P; ++p) {
+      hipError_t e = launch_fft3d<double>(h->adj_Z + (size_t)p * N, ps.plan[0], ps.plan[1], ps.plan[2], ps.geom,
+                                          h->nx, h->ny, h->nz, h->dv, ivol, dir, st);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  };
+  // apply K^T to x (+ y) into adj_Z (real space, packed); Y receives the k-space weight field
+  auto apply_kt = [&](const double *x, const double *y, double2 *Y) -> hipError_t {
+    hipError_t e = launch_adj_pack(a, x, y, h->adj_Z, st);
+    if (e == hipSuccess) e = fft(-1);
+    if (e == hipSuccess) e = launch_adj_kspace(a, h->adj_Z, Y, st);
+    if (e == hipSuccess) e = fft(+1);
+    return e;
+  };
+  HIP_OK(launch_adj_fields(a, st));
+  HIP_OK(apply_kt(a.alpha, nullptr, nullptr));
+  HIP_OK(launch_adj_rhs(a, h->adj_Z, st));
+  HIP_OK(hipMemsetAsync(a.lam, 0, VN * sizeof(double), st));
+  HIP_OK(hipMemsetAsync(h->adj_hist, 0, 2 * (size_t)nh * VN * sizeof(double), st));
+  Ctl c;
+  std::memset(&c, 0, sizeof c);
+  c.run_iter = 0; c.upd_iter = -1; c.status = -1;
+  c.maxstep = h->opts.maxstep; c.count = h->opts.count; c.nhist = nh;
+  c.tol = -1.0;  // set after the first step from rms(b)
+  c.m = h->opts.m; c.mp = h->opts.mp; c.npts = (double)VN;
+  HIP_OK(hipMemcpyAsync(h->adj_ctl, &c, sizeof c, hipMemcpyHostToDevice, st));
+  const int nblk = ps.geom.xblocks;
+  const int nv = 1 + h->hm + h->hm * (h->hm + 1) / 2;
+  int it = 0;
+  for (;;) {
+    HIP_OK(launch_adj_update(a, h->hm, it, st));
+    HIP_OK(apply_kt(a.lam, nullptr, nullptr));
+    HIP_OK(launch_adj_resid(a, h->adj_Z, h->hm, nblk, it, st));
+    HIP_OK(launch_anderson_solve(h->adj_ctl, h->partials, nblk, nv, h->hm, h->adj_rms, it, st));
+    ++it;
+    if (it == 1 || it % 8 == 0 || it >= h->opts.maxstep + 1) {
+      HIP_OK(hipMemcpyAsync(&c, h->adj_ctl, sizeof c, hipMemcpyDeviceToHost, st));
+      HIP_OK(hipStreamSynchronize(st));
+      if (it == 1 && c.status == -1) {  // lambda = 0: the first residual is b
+        c.tol = rtol * c.rms;
+        HIP_OK(hipMemcpyAsync(h->adj_ctl, &c, sizeof c, hipMemcpyHostToDevice, st));
+      }
+      if (c.status != -1 || it >= h->opts.maxstep + 1) break;
+    }
+  }
+  if (steps) *steps = c.total;
+  if (c.status == RISM_SOLVE_DIVERGED || c.status == -1) {
+    set_error("adjoint solve did not converge");
+    return RISM_SOLVE_DIVERGED;
+  }
+  // field adjoints from mu = lambda + alpha, and the k-space weight Y
+  HIP_OK(apply_kt(a.lam, a.alpha, h->adj_Y));
+  HIP_OK(launch_adj_bar(a, h->adj_Z, st));
+  if (nu > 0) {
+    const int gblk = adj_grad_blocks(N);
+    double *p6 = nullptr, *p4 = nullptr, *o = nullptr;
+    HIP_OK(hipMalloc((void **)&p6, (size_t)gblk * nu * 6 * sizeof(double)));
+    HIP_OK(hipMalloc((void **)&p4, (size_t)gblk * nu * 4 * sizeof(double)));
+    HIP_OK(hipMalloc((void **)&o, (size_t)nu * 10 * sizeof(double)));
+    hipError_t e = launch_adj_grad(a, h->adj_Y, p6, p4, o, o + (size_t)nu * 6, st);
+    std::vector<double> r((size_t)nu * 10);
+    if (e == hipSuccess) e = hipMemcpyAsync(r.data(), o, r.size() * sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(p6);
+    (void)hipFree(p4);
+    (void)hipFree(o);
+    if (e != hipSuccess) {
+      set_error(std::string("HIP error in rism_adjoint_grad: ") + hipGetErrorString(e));
+      return RISM_ERR_HIP;
+    }
+    const double *r6 = r.data(), *r4 = r.data() + (size_t)nu * 6;
+    const double ks = -h->cf * a.ubeta / (h->volume * h->dv);  // fk term: -(cf / (vol dv)) sum_k Y dfk
+    for (int u = 0; u < nu; ++u) {
+      if (dxyz)
+        for (int k = 0; k < 3; ++k) dxyz[3 * u + k] = r6[6 * u + k] + ks * h->uq[u] * r4[4 * u + 1 + k];
+      if (dq) dq[u] = r6[6 * u + 3] + ks * r4[4 * u];
+      if (dsig) dsig[u] = r6[6 * u + 4];
+      if (deps) deps[u] = r6[6 * u + 5];
+    }
+  }
+  HIP_OK(hipStreamSynchronize(st));
+  return c.status == 0 ? RISM_OK : c.status;
 }
 
 int rism_get_field(rism_handle *h, int which, double *out) {

@@ -36,6 +36,7 @@ EXPORTED = [
     "rism_thermo", "rism_grad", "rism_get_field", "rism_set_t", "rism_save_tuv", "rism_load_tuv",
     "rism_write_xmu", "rism_write_gra", "rism_stats", "rism_rms_history", "rism_destroy",
     "rism_fft3d", "rism_calculate", "rism_profile", "rism_kernel_times", "rism_geometry",
+    "rism_adjoint_grad",
 ]
 
 KERNELS = ["closure_fx", "fft_y_fwd", "fft_z_oz", "fft_y_inv", "resid", "solve", "potentials", "fk",
@@ -135,6 +136,7 @@ def lib():
         L.rism_destroy.restype = None
         L.rism_fft3d.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
         L.rism_calculate.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.rism_adjoint_grad.argtypes = [C.c_void_p, C.c_double, dp, dp, dp, dp, C.POINTER(C.c_int)]
         L.rism_profile.argtypes = [C.c_void_p, C.c_int]
         L.rism_kernel_times.argtypes = [C.c_void_p, dp, P(C.c_long)]
         L.rism_geometry.argtypes = [C.c_void_p, P(C.c_int)]
@@ -311,6 +313,19 @@ class Solver:
         out = np.zeros(self.natom * 6)
         _check(lib().rism_grad(self.h, _dptr(out)), "rism_grad")
         return out.reshape(self.natom, 6)
+
+    def adjoint_grad(self, rtol: float = 1e-10) -> dict:
+        """dDelta-mu/d(solute parameters) by the adjoint of the converged
+        fixed point (rism_adjoint_grad): {"xyz": [n][3], "q": [n], "sig": [n],
+        "eps": [n], "steps": adjoint iterations}, J/mol per unit."""
+        n = self.natom
+        dxyz, dq, dsig, deps = np.zeros(3 * n), np.zeros(n), np.zeros(n), np.zeros(n)
+        steps = C.c_int()
+        rc = _check(lib().rism_adjoint_grad(self.h, float(rtol), _dptr(dxyz), _dptr(dq), _dptr(dsig),
+                                            _dptr(deps), C.byref(steps)), "rism_adjoint_grad")
+        if rc == SOLVE_DIVERGED:
+            raise RuntimeError(f"rism_adjoint_grad: adjoint solve diverged: {last_error()}")
+        return {"xyz": dxyz.reshape(n, 3), "q": dq, "sig": dsig, "eps": deps, "steps": steps.value}
 
     def field(self, which: int) -> np.ndarray:
         if which == FIELD_FK:

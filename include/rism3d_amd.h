@@ -174,6 +174,20 @@ int rism_thermo(rism_handle *h, rism_xmu *out);
 /* cal_grad (cal_grad.cu:8-48) scaled as written by output_grad.cu:15-24:
    out[n][6] = (dLJ_x, dLJ_y, dLJ_z, dES_x, dES_y, dES_z), J/mol/Angstrom */
 int rism_grad(rism_handle *h, double *out);
+/* Adjoint gradient of Delta-mu (SFE_SC, J/mol) with respect to the solute
+   parameters, at the converged state of the last rism_solve.  New: the
+   reference has no counterpart (its cal_grad, rism_grad above, is the
+   explicit formula sum_v rho_v sum_r g dU/dR, which the adjoint reproduces
+   when the functional is stationary).  Solves the transposed fixed-point
+   system lambda = (g'-1)(K^T lambda - lambda) + b with the Anderson scheme
+   of the forward solve to rms <= rtol * rms(b), then sums the field
+   adjoints against the derivatives of the potentials (cal_LJ.cu,
+   cal_Coulomb.cu).  Outputs (any may be NULL): dxyz[n][3] J/mol/Angstrom,
+   dq[n] J/mol/e, dsig[n] J/mol/Angstrom, deps[n] (J/mol per J/mol).
+   Returns RISM_OK, RISM_SOLVE_MAXSTEP (outputs from the last iterate) or an
+   error; steps receives the adjoint iteration count. */
+int rism_adjoint_grad(rism_handle *h, double rtol, double *dxyz, double *dq, double *dsig,
+                      double *deps, int *steps);
 /* copy a field (RISM_FIELD_*) to host memory */
 int rism_get_field(rism_handle *h, int which, double *out);
 /* replace t (host [V][N]), e.g. a restart from memory */
