@@ -36,7 +36,7 @@ EXPORTED = [
     "rism_thermo", "rism_grad", "rism_get_field", "rism_set_t", "rism_save_tuv", "rism_load_tuv",
     "rism_write_xmu", "rism_write_gra", "rism_stats", "rism_rms_history", "rism_destroy",
     "rism_fft3d", "rism_calculate", "rism_profile", "rism_kernel_times", "rism_geometry",
-    "rism_adjoint_grad",
+    "rism_adjoint_grad", "rism_batch_solve",
 ]
 
 KERNELS = ["closure_fx", "fft_y_fwd", "fft_z_oz", "fft_y_inv", "resid", "solve", "potentials", "fk",
@@ -137,6 +137,9 @@ def lib():
         L.rism_fft3d.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
         L.rism_calculate.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.rism_adjoint_grad.argtypes = [C.c_void_p, C.c_double, dp, dp, dp, dp, C.POINTER(C.c_int)]
+        ip = P(C.c_int)
+        L.rism_batch_solve.argtypes = [C.c_int, P(Grid), P(SolventC), P(SoluteC), P(Opts), C.c_int, C.c_int,
+                                       ip, ip, dp, P(Xmu), dp]
         L.rism_profile.argtypes = [C.c_void_p, C.c_int]
         L.rism_kernel_times.argtypes = [C.c_void_p, dp, P(C.c_long)]
         L.rism_geometry.argtypes = [C.c_void_p, P(C.c_int)]
@@ -401,3 +404,40 @@ class Solver:
             self.close()
         except Exception:
             pass
+
+
+def batch_solve(systems, solvent: Solvent, opts: Options = Options(), workers: int = 8, device: int = 0):
+    """Solve many solutes concurrently (rism_batch_solve).  systems: list of
+    (n, box, Solute).  Returns (results, seconds) with one dict per system:
+    status, steps, rms and the rism_thermo quantities."""
+    L = lib()
+    keep = []
+
+    def dptr(a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        keep.append(a)
+        return _dptr(a)
+
+    n = len(systems)
+    grids = (Grid * n)()
+    sols = (SoluteC * n)()
+    for i, (g, box, su) in enumerate(systems):
+        grids[i] = Grid((C.c_int * 3)(*g), (C.c_double * 3)(*box))
+        sols[i] = SoluteC(len(su.q), dptr(su.sig), dptr(su.eps), dptr(su.q), dptr(np.asarray(su.xyz).reshape(-1)))
+    sv = SolventC(solvent.nsite, solvent.chi.shape[0], solvent.dk, solvent.temper, solvent.xt,
+                  dptr(solvent.rho), dptr(solvent.q), dptr(solvent.sig), dptr(solvent.eps), dptr(solvent.chi))
+    o = Opts(opts.closure, opts.tol, opts.maxstep, opts.count, opts.m, opts.mp, opts.nhist,
+             opts.precision, opts.chargeup, opts.verbose, opts.ksave)
+    status, steps = (C.c_int * n)(), (C.c_int * n)()
+    rms = (C.c_double * n)()
+    xmu = (Xmu * n)()
+    sec = C.c_double()
+    _check(L.rism_batch_solve(n, grids, C.byref(sv), sols, C.byref(o), device, workers, status, steps, rms, xmu,
+                              C.byref(sec)), "rism_batch_solve")
+    out = []
+    for i in range(n):
+        d = {"status": status[i], "steps": steps[i], "rms": rms[i]}
+        if status[i] >= 0:
+            d.update(xmu[i].as_dict())
+        out.append(d)
+    return out, sec.value

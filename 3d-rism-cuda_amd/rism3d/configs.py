@@ -12,6 +12,10 @@ with the reference's integer-rounded shift (src-cuda/solute.cu:13-43).
       steps confined to a sphere (radius of gyration ~14 A), same parameter
       distributions, net charge -4.
   c2  128^3 / 64 A box, 200 atoms (same chain model, net charge 0).
+  c3  many small solutes (batch_config): molecules of 8-40 atoms (uniform
+      count), random walks of 1.5 A steps within 6 A of the centre, on
+      0.5 A grids sized to the solute extent plus a 12 A margin on each
+      side, rounded up to a 2,3,5-smooth multiple of 4 within 48..96.
   c4  512^3 / 256 A box, 5000 atoms, net charge -12.
 """
 from __future__ import annotations
@@ -125,6 +129,38 @@ def config(index: int) -> Workload:
         return Workload("c4: 512^3 / 256 A, 5000-atom chain (net -12), KH", (512, 512, 512),
                         (256.0, 256.0, 256.0), Solute(sig, eps, q, centre(xyz)), nhist=10)
     raise ValueError(f"no single-solute workload {index}")
+
+
+def smooth_grid(n: int, lo: int = 48, hi: int = 96) -> int:
+    """Smallest multiple of 4 >= n whose only prime factors are 2, 3, 5,
+    clamped to [lo, hi]."""
+    m = max(lo, n + (-n) % 4)
+    while True:
+        r = m
+        for p in (2, 3, 5):
+            while r % p == 0:
+                r //= p
+        if r == 1 or m >= hi:
+            return min(m, hi)
+        m += 4
+
+
+def batch_config(count: int = 256, spacing: float = 0.5, margin: float = 12.0) -> list:
+    """The batched-solute workload: `count` seeded small molecules, each on
+    its own grid (index 3 of the seeded configurations)."""
+    rng = np.random.default_rng(SEED0 + 3)
+    out = []
+    for i in range(count):
+        na = int(rng.integers(8, 41))
+        xyz = random_walk(rng, na, 1.5, 6.0)
+        sig, eps, q = _params(rng, na, 0.0)
+        xyz = centre(xyz)
+        ext = float((xyz.max(0) - xyz.min(0)).max())
+        n = smooth_grid(int(np.ceil((ext + 2 * margin) / spacing)))
+        b = n * spacing
+        out.append(Workload(f"c3[{i}]: {n}^3 / {b:g} A, {na} atoms, KH", (n, n, n), (b, b, b),
+                            Solute(sig, eps, q, xyz)))
+    return out
 
 
 def write_inp(path: str, w: Workload, solvent_file: str, tol=1e-6, maxstep=10000, count=2, m=0.6,

@@ -205,6 +205,19 @@ int rism_stats(rism_handle *h, int *steps, double *rms, double *t_setup,
 int rism_rms_history(rism_handle *h, double *out, int n);
 void rism_destroy(rism_handle *h);
 
+/* ---- many solutes (no counterpart in the reference: one solve per run) ---- */
+/* n independent solves sharing one solvent, each solute on its own grid
+   (grids[i], solutes[i]), run concurrently on one device: nworkers host
+   threads each own a handle and its HIP stream and take solutes from a
+   shared queue (rism_create, rism_build_potentials, rism_solve, rism_thermo,
+   rism_destroy).  Per solute: status[i] (RISM_SOLVE_* or RISM_ERR_*),
+   steps[i], rms[i], xmu[i] (any of steps/rms/xmu may be NULL); seconds
+   receives the wall time.  Returns RISM_OK once every solve has run. */
+int rism_batch_solve(int n, const rism_grid *grids, const rism_solvent *solvent,
+                     const rism_solute *solutes, const rism_opts *opts, int device,
+                     int nworkers, int *status, int *steps, double *rms, rism_xmu *xmu,
+                     double *seconds);
+
 /* ---- measurement (no counterpart in the reference, which has no timers) */
 /* kernel ids for rism_kernel_times */
 #define RISM_K_CLOSURE_FX 0   /* Anderson update + closure + forward x FFT */
