@@ -255,6 +255,71 @@ __device__ __forceinline__ void fft_lines(T2 *lds, int nlines, int ls, const Axi
   }
 }
 
+// ---------------------------------------------------------------------
+// Compile-time plans for power-of-two lengths N = 2^e on tiles that fill
+// every thread (nlines * N == EPT * kThreads).  The stages are those of
+// the runtime plan (ceil(e/4) radices of balanced bit counts, so results
+// are identical); with every size a constant, the butterfly index
+// arithmetic reduces to shifts and masks and no bounds test remains.
+// ---------------------------------------------------------------------
+constexpr int ct_log2(int n) { return n <= 1 ? 0 : 1 + ct_log2(n / 2); }
+constexpr int ct_radix(int N, int k) {  // radix of stage k
+  const int e = ct_log2(N), ns = (e + 3) / 4;
+  return 1 << (e / ns + (k < e % ns ? 1 : 0));
+}
+constexpr int ct_stages(int N) { return (ct_log2(N) + 3) / 4; }
+
+template <int F, bool INV, int N, int S, int EPT, typename T2>
+__device__ __forceinline__ void fft_stage_ct(T2 *lds, int ls, const T2 *__restrict__ tw, int tid) {
+  constexpr int TOT = EPT * kThreads / F;  // butterflies in the tile
+  constexpr int KMAX = (TOT + kThreads - 1) / kThreads;
+  constexpr bool FULL = TOT % kThreads == 0;
+  constexpr int NBL = N / F;     // butterflies per line
+  constexpr int M = N / (S * F);
+  T2 v[KMAX][F];
+  int lbase[KMAX], q0[KMAX], ppv[KMAX];
+#pragma unroll
+  for (int k = 0; k < KMAX; ++k) {
+    const int bf = tid + k * kThreads;
+    const int line = bf / NBL;
+    const int j = bf % NBL;
+    const int pp = j / S;
+    lbase[k] = line * ls;
+    q0[k] = j % S;
+    ppv[k] = pp;
+  }
+  const bool act = FULL || tid < TOT;
+#pragma unroll
+  for (int k = 0; k < KMAX; ++k)
+    if (act)
+#pragma unroll
+      for (int a = 0; a < F; ++a) v[k][a] = lds[lbase[k] + lpad(q0[k] + S * (ppv[k] + a * M))];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < KMAX; ++k) {
+    if (act) {
+      Dft<F, INV, T2>::run(v[k]);
+      const int o = q0[k] + S * (F * ppv[k]);
+      lds[lbase[k] + lpad(o)] = v[k][0];
+#pragma unroll
+      for (int b = 1; b < F; ++b) {
+        const T2 w = tw[ppv[k] * b * S];  // tstep = S
+        lds[lbase[k] + lpad(o + S * b)] = INV ? cmulc(v[k][b], w) : cmul(v[k][b], w);
+      }
+    }
+  }
+  __syncthreads();
+}
+
+template <int N, int EPT, bool INV, int K = 0, int S = 1, typename T2>
+__device__ __forceinline__ void fft_lines_ct(T2 *lds, int ls, const T2 *tw, int tid = threadIdx.x) {
+  if constexpr (K < ct_stages(N)) {
+    constexpr int F = ct_radix(N, K);
+    fft_stage_ct<F, INV, N, S, EPT>(lds, ls, tw, tid);
+    fft_lines_ct<N, EPT, INV, K + 1, S * F>(lds, ls, tw, tid);
+  }
+}
+
 // sign (-1)^(j + n/2) applied after a forward / before an inverse pass
 __device__ __forceinline__ int jsign(int j, int fixsign) { return (j & 1) ? -fixsign : fixsign; }
 

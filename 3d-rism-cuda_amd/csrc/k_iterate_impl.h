@@ -248,13 +248,16 @@ __global__ __launch_bounds__(kThreads) void k_closure_fx(IterArgs<R, S> a) {
 // line for FP32 at T = 16).  One tile per workgroup; the twiddle and phase
 // tables are staged in LDS behind the tile.
 // ---------------------------------------------------------------------
-template <typename R, int EPT, bool INV>
+template <typename R, int EPT, bool INV, int NL = 0>
 __global__ __launch_bounds__(kThreads) void k_fft_y(typename C2<R>::T *Z, const Ctl *ctl, int iter, int nx,
-                                                     int ny, int nz, long N, int T, int P, AxisPlan py) {
+                                                     int ny_rt, int nz, long N, int T_rt, int P, AxisPlan py) {
+  // NL > 0: compile-time line length (power of two) with T = EPT * kThreads / NL
   using T2 = typename C2<R>::T;
   if (ctl && ctl->run_iter != iter) return;
   extern __shared__ unsigned char smem_raw[];
   T2 *lds = reinterpret_cast<T2 *>(smem_raw);
+  const int ny = NL > 0 ? NL : ny_rt;
+  const int T = NL > 0 ? EPT * kThreads / (NL > 0 ? NL : 1) : T_rt;
   const int ls = line_stride(ny);
   const int nxt = nx / T;
   const int nel = T * ny;
@@ -279,7 +282,10 @@ __global__ __launch_bounds__(kThreads) void k_fft_y(typename C2<R>::T *Z, const 
     }
   }
   __syncthreads();
-  fft_lines<EPT, INV>(lds, T, ls, py, ltw);
+  if constexpr (NL > 0)
+    fft_lines_ct<NL, EPT, INV>(lds, ls, (const T2 *)ltw);
+  else
+    fft_lines<EPT, INV>(lds, T, ls, py, ltw);
 #pragma unroll
   for (int k = 0; k < EPT; ++k) {
     const int e = threadIdx.x + k * kThreads;
@@ -305,14 +311,18 @@ __global__ __launch_bounds__(kThreads) void k_fft_y(typename C2<R>::T *Z, const 
 #define RISM_OZ_GROUP 2
 #endif
 constexpr int kXcds = 8;
-template <typename R, typename S, int EPT, int VC>
+template <typename R, typename S, int EPT, int VC, int NL = 0>
 __global__ __launch_bounds__(kThreads) void k_fft_z_oz(IterArgs<R, S> a) {
+  // NL > 0: compile-time z length (power of two), one pack (VC == 2), T = EPT * kThreads / (2 NL)
   using T2 = typename C2<R>::T;
   using T4 = typename R4T<R>::T;
   if (a.ctl->run_iter != a.iter) return;
   extern __shared__ unsigned char smem_raw[];
   T2 *lds = reinterpret_cast<T2 *>(smem_raw);
-  const int nx = a.nx, ny = a.ny, nz = a.nz, T = a.Tz, P = a.P, V = a.V;
+  const int nx = a.nx, ny = a.ny, V = a.V;
+  const int nz = NL > 0 ? NL : a.nz;
+  const int T = NL > 0 ? EPT * kThreads / (2 * (NL > 0 ? NL : 1)) : a.Tz;
+  const int P = NL > 0 ? 1 : a.P;
   const long N = a.N, nxy = (long)nx * ny;
   const int ls = line_stride(nz);
   const int nxt = nx / T, ntile = nxt * (ny / 2);
@@ -347,7 +357,10 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_oz(IterArgs<R, S> a) {
       }
     }
     __syncthreads();
-    fft_lines<EPT, false>(lds, 2 * P * T, ls, a.pz, ltw);
+    if constexpr (NL > 0)
+      fft_lines_ct<NL, EPT, false>(lds, ls, (const T2 *)ltw);
+    else
+      fft_lines<EPT, false>(lds, 2 * P * T, ls, a.pz, ltw);
     // k-space: pair (i, j) of tile A with (T-1-i, nz-1-j) of tile B.  |k|
     // and the spline interval are evaluated in FP64 in both precisions.
     const double tpx = 2.0 * M_PI / a.box[0], tpz = 2.0 * M_PI / a.box[2];
@@ -436,7 +449,10 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_oz(IterArgs<R, S> a) {
       if ((k + 1) % RISM_OZ_GROUP == 0) __builtin_amdgcn_sched_barrier(0);
     }
     __syncthreads();
-    fft_lines<EPT, true>(lds, 2 * P * T, ls, a.pz, ltw);
+    if constexpr (NL > 0)
+      fft_lines_ct<NL, EPT, true>(lds, ls, (const T2 *)ltw);
+    else
+      fft_lines<EPT, true>(lds, 2 * P * T, ls, a.pz, ltw);
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       const int e = tid + k * kThreads;
@@ -839,15 +855,41 @@ hipError_t launch_closure_fx(const IterArgs<R, S> &a, const LaunchGeom &g, bool 
   return hipGetLastError();
 }
 
+// compile-time line length for the y / z passes: a power of two in
+// 64..512 on a tile of exactly EPT (8 or 16) elements per thread
+inline int ct_length(int n, long tile_elems, int ept) {
+  if ((ept != 8 && ept != 16) || tile_elems != (long)ept * kThreads) return 0;
+  return (n == 64 || n == 128 || n == 256 || n == 512) ? n : 0;
+}
+#define RISM_CT_SWITCH(nl, ept, GENERIC, ...)                                     \
+  switch ((nl) * 100 + (ept)) {                                                   \
+    case 6408: { constexpr int E = 8, NLV = 64; __VA_ARGS__; } break;             \
+    case 6416: { constexpr int E = 16, NLV = 64; __VA_ARGS__; } break;            \
+    case 12808: { constexpr int E = 8, NLV = 128; __VA_ARGS__; } break;           \
+    case 12816: { constexpr int E = 16, NLV = 128; __VA_ARGS__; } break;          \
+    case 25608: { constexpr int E = 8, NLV = 256; __VA_ARGS__; } break;           \
+    case 25616: { constexpr int E = 16, NLV = 256; __VA_ARGS__; } break;          \
+    case 51208: { constexpr int E = 8, NLV = 512; __VA_ARGS__; } break;           \
+    case 51216: { constexpr int E = 16, NLV = 512; __VA_ARGS__; } break;          \
+    default: GENERIC;                                                             \
+  }
+
 template <typename R, typename S>
 hipError_t launch_fft_y(const IterArgs<R, S> &a, const LaunchGeom &g, bool inverse, hipStream_t st) {
   const dim3 grid(a.nx / g.Ty * a.nz * a.P), block(kThreads);
+  const int nl = ct_length(a.ny, (long)g.Ty * a.ny, g.ept_y);
   if (inverse) {
-    RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, true>), grid, block, g.lds_y, st, a.Z,
-                                                a.ctl, a.iter, a.nx, a.ny, a.nz, a.N, g.Ty, a.P, a.py))
+    RISM_CT_SWITCH(nl, g.ept_y,
+                   RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, true>), grid, block, g.lds_y, st, a.Z,
+                                                               a.ctl, a.iter, a.nx, a.ny, a.nz, a.N, g.Ty, a.P, a.py)),
+                   hipLaunchKernelGGL((k_fft_y<R, E, true, NLV>), grid, block, g.lds_y, st, a.Z, a.ctl, a.iter, a.nx,
+                                      a.ny, a.nz, a.N, g.Ty, a.P, a.py))
   } else {
-    RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, false>), grid, block, g.lds_y, st, a.Z,
-                                                a.ctl, a.iter, a.nx, a.ny, a.nz, a.N, g.Ty, a.P, a.py))
+    RISM_CT_SWITCH(nl, g.ept_y,
+                   RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, false>), grid, block, g.lds_y, st, a.Z,
+                                                               a.ctl, a.iter, a.nx, a.ny, a.nz, a.N, g.Ty, a.P, a.py)),
+                   hipLaunchKernelGGL((k_fft_y<R, E, false, NLV>), grid, block, g.lds_y, st, a.Z, a.ctl, a.iter, a.nx,
+                                      a.ny, a.nz, a.N, g.Ty, a.P, a.py))
   }
   return hipGetLastError();
 }
@@ -855,7 +897,11 @@ hipError_t launch_fft_y(const IterArgs<R, S> &a, const LaunchGeom &g, bool inver
 template <typename R, typename S>
 hipError_t launch_fft_z_oz(const IterArgs<R, S> &a, const LaunchGeom &g, hipStream_t st) {
   const dim3 grid(a.nx / g.Tz * (a.ny / 2)), block(kThreads);
-  RISM_VC_SWITCH(a.V, RISM_EPT_SWITCH(g.ept_z, hipLaunchKernelGGL((k_fft_z_oz<R, S, E, VC>), grid, block, g.lds_z, st, a)))
+  const int nl = a.V <= 2 ? ct_length(a.nz, 2L * g.Tz * a.nz, g.ept_z) : 0;
+  RISM_CT_SWITCH(nl, g.ept_z,
+                 RISM_VC_SWITCH(a.V, RISM_EPT_SWITCH(g.ept_z, hipLaunchKernelGGL((k_fft_z_oz<R, S, E, VC>), grid, block,
+                                                                                 g.lds_z, st, a))),
+                 hipLaunchKernelGGL((k_fft_z_oz<R, S, E, 2, NLV>), grid, block, g.lds_z, st, a))
   return hipGetLastError();
 }
 
