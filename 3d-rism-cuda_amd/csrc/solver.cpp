@@ -297,7 +297,7 @@ int choose_geometry(rism_handle *h, LaunchGeom &g, size_t rsz) {
   if (tz > 0 && nx % tz == 0) g.Tz = (int)tz;
   g.ept_x = ept_class((long)P * g.Lx * nx);
   g.xgroups = (int)((long)ny * nz / g.Lx);
-  g.xblocks = std::min<int>(g.xgroups, (int)env_long("RISM_XBLOCKS", 4096));
+  g.xblocks = std::min<int>(g.xgroups, (int)env_long("RISM_XBLOCKS", 1024));
   g.lds_x = ((size_t)P * g.Lx * lsx + 2 * (size_t)nx) * c2;
   g.ept_y = ept_class((long)g.Ty * ny);
   g.lds_y = ((size_t)g.Ty * lsy + 2 * (size_t)ny) * c2;

@@ -155,6 +155,53 @@ def cpu_baseline(w, fp64_iterations, iters=8, sdiv=4):
             "measured_s": round(rec["t_setup"] + rec["t_iter"] + rec["t_thermo"], 2)}
 
 
+def side_workload(which, rism3d, configs):
+    """The other two measured configurations (not the headline line)."""
+    import numpy as np
+    if which == "batch":
+        sv = rism3d.read_xvk(os.path.join(REPO, "tests", "golden", "reference_examples", "h2o_vv.xvk"))
+        systems = [(w.n, w.box, w.solute) for w in configs.batch_config(256)]
+        rism3d.batch_solve(systems[:8], sv, rism3d.Options(), workers=4)  # warm-up
+        r64, s64 = rism3d.batch_solve(systems, sv, rism3d.Options(), workers=8)
+        rmx, smx = rism3d.batch_solve(systems, sv, rism3d.Options(precision=rism3d.MIXED), workers=8)
+        err = max(abs(a["SFE_SC"] - b["SFE_SC"]) for a, b in zip(rmx, r64))
+        return {"metric": "solutes/s, 256 seeded 8-40-atom solutes on 60-72^3 grids (KH, RMS 1e-6)",
+                "value": round(len(systems) / smx, 2), "unit": "solutes/s", "higher_is_better": True,
+                "fp64_solutes_per_s": round(len(systems) / s64, 2), "workers": 8,
+                "converged": sum(r["status"] == 0 for r in rmx), "max_dmu_err_vs_fp64_J_mol": round(err, 4)}
+    sv = rism3d.read_xvk(os.path.join(REPO, "tests", "golden", "reference_examples", "naclaq_vv.xvk"))
+    w = configs.config(4)
+    out = {"metric": "s-to-converge 512^3 / 256 A / 5000-atom KH job (potentials + solve + thermo)",
+           "unit": "s", "higher_is_better": False,
+           "solvent": "reference example_rmdft/naclaq_vv.xvk (water + 0.1 M NaCl, 4 reduced sites)"}
+    for name, prec in (("mixed", rism3d.MIXED), ("fp64", rism3d.FP64)):
+        free0, _ = torch.cuda.mem_get_info()
+        s = rism3d.Solver(w.n, w.box, sv, w.solute, rism3d.Options(precision=prec, nhist=w.nhist))
+        free1, _ = torch.cuda.mem_get_info()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.build_potentials()
+        st, steps, rms = s.solve()
+        x = s.thermo()
+        dt = time.perf_counter() - t0
+        r = {"status": st, "iterations": steps, "s": round(dt, 3), "device_GB": round((free0 - free1) / 1e9, 1),
+             "SFE_SC": x["SFE_SC"], "PMV": x["PMV"]}
+        if name == "mixed":
+            t1 = time.perf_counter()
+            g = s.adjoint_grad(1e-8)
+            r["adjoint_s"] = round(time.perf_counter() - t1, 3)
+            r["adjoint_iterations"] = g["steps"]
+            cg = s.grad()
+            ref = cg[:, :3] + cg[:, 3:]
+            r["adjoint_vs_cal_grad_max_rel"] = float(np.abs(g["xyz"] - ref).max() / np.abs(ref).max())
+        s.close()
+        out[name] = r
+    out["value"] = out["mixed"]["s"]
+    out["speedup_mixed_vs_fp64"] = round(out["fp64"]["s"] / out["mixed"]["s"], 3)
+    out["dmu_mixed_minus_fp64_J_mol"] = out["mixed"]["SFE_SC"] - out["fp64"]["SFE_SC"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,6 +209,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline sample")
     ap.add_argument("--mixed-only", action="store_true", help="time only the mixed path")
+    ap.add_argument("--workload", choices=("256", "512", "batch"), default="256",
+                    help="256 (default): the headline 256^3 job; 512: the 512^3 / 5000-atom electrolyte "
+                         "job (memory, times, adjoint forces); batch: 256 small solutes (solutes/s)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain device copy of 256 MiB three times (the reference kernel "
                          "for scaling FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -176,6 +226,9 @@ def main():
     if lib.rism_device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.workload != "256":
+        print(json.dumps(side_workload(args.workload, rism3d, configs)))
+        return
     if args.calibrate_copy:
         src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
