@@ -10,6 +10,7 @@
 // blocks).  Only the C ABI is used.
 #include <atomic>
 #include <chrono>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -39,6 +40,7 @@ extern "C" int rism_batch_solve(int n, const rism_grid *grids, const rism_solven
         const int t = rism_thermo(h, &xmu[i]);
         if (t < 0) st = t;
       }
+      if (st < 0) std::fprintf(stderr, "rism_batch_solve: solute %d: %s\n", i, rism_last_error());
       status[i] = st;
       if (steps) steps[i] = nstep;
       if (rms) rms[i] = r;

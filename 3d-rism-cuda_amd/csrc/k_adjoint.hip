@@ -146,9 +146,9 @@ __global__ __launch_bounds__(kThreads) void k_adj_rhs(AdjArgs a, const double2 *
 // pending Anderson update of lambda (absolute history form, as the FP64
 // forward pass: anderson_cuda.cu:1-23)
 template <int HM>
-__global__ __launch_bounds__(kThreads) void k_adj_update(AdjArgs a, int iter) {
+__global__ __launch_bounds__(kThreads) void k_adj_update(AdjArgs a) {
   const Ctl *ctl = a.ctl;
-  if (ctl->upd_iter != iter) return;
+  if (ctl->upd_iter != ctl->cur) return;
   const long VN = (long)a.V * a.N;
   const long iv = (long)blockIdx.x * kThreads + threadIdx.x;
   if (iv >= VN) return;
@@ -171,10 +171,10 @@ __global__ __launch_bounds__(kThreads) void k_adj_update(AdjArgs a, int iter) {
 // residual r = (g' - 1)(K^T lambda - lambda) + b - lambda and the Anderson
 // inner products (same partial layout as the forward residual pass)
 template <int HM>
-__global__ __launch_bounds__(kThreads) void k_adj_resid(AdjArgs a, const double2 *Z, int iter) {
+__global__ __launch_bounds__(kThreads) void k_adj_resid(AdjArgs a, const double2 *Z) {
   constexpr int NV = 1 + HM + HM * (HM + 1) / 2;
   const Ctl *ctl = a.ctl;
-  if (ctl->run_iter != iter) return;
+  if (ctl->run_iter != ctl->cur) return;
   const long N = a.N, VN = (long)a.V * N;
   const int nf = ctl->filled < HM ? ctl->filled : HM;
   double acc[NV];
@@ -379,21 +379,21 @@ hipError_t launch_adj_rhs(const AdjArgs &a, const double2 *Z, hipStream_t st) {
   hipLaunchKernelGGL(k_adj_rhs, agrid(a.N), dim3(kThreads), 0, st, a, Z);
   return hipGetLastError();
 }
-hipError_t launch_adj_update(const AdjArgs &a, int hm, int iter, hipStream_t st) {
+hipError_t launch_adj_update(const AdjArgs &a, int hm, hipStream_t st) {
   const dim3 g = agrid((long)a.V * a.N);
   switch (hm) {
-    case 2: hipLaunchKernelGGL(k_adj_update<2>, g, dim3(kThreads), 0, st, a, iter); break;
-    case 5: hipLaunchKernelGGL(k_adj_update<5>, g, dim3(kThreads), 0, st, a, iter); break;
-    case 10: hipLaunchKernelGGL(k_adj_update<10>, g, dim3(kThreads), 0, st, a, iter); break;
+    case 2: hipLaunchKernelGGL(k_adj_update<2>, g, dim3(kThreads), 0, st, a); break;
+    case 5: hipLaunchKernelGGL(k_adj_update<5>, g, dim3(kThreads), 0, st, a); break;
+    case 10: hipLaunchKernelGGL(k_adj_update<10>, g, dim3(kThreads), 0, st, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
-hipError_t launch_adj_resid(const AdjArgs &a, const double2 *Z, int hm, int nblk, int iter, hipStream_t st) {
+hipError_t launch_adj_resid(const AdjArgs &a, const double2 *Z, int hm, int nblk, hipStream_t st) {
   switch (hm) {
-    case 2: hipLaunchKernelGGL(k_adj_resid<2>, dim3(nblk), dim3(kThreads), 0, st, a, Z, iter); break;
-    case 5: hipLaunchKernelGGL(k_adj_resid<5>, dim3(nblk), dim3(kThreads), 0, st, a, Z, iter); break;
-    case 10: hipLaunchKernelGGL(k_adj_resid<10>, dim3(nblk), dim3(kThreads), 0, st, a, Z, iter); break;
+    case 2: hipLaunchKernelGGL(k_adj_resid<2>, dim3(nblk), dim3(kThreads), 0, st, a, Z); break;
+    case 5: hipLaunchKernelGGL(k_adj_resid<5>, dim3(nblk), dim3(kThreads), 0, st, a, Z); break;
+    case 10: hipLaunchKernelGGL(k_adj_resid<10>, dim3(nblk), dim3(kThreads), 0, st, a, Z); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -176,8 +176,9 @@ __global__ __launch_bounds__(kThreads) void k_closure_fx(IterArgs<R, S> a) {
   using Q = typename HistT<R>::T;
   constexpr bool kIncr = sizeof(Q) < sizeof(S);
   const Ctl *ctl = a.ctl;
-  const bool do_upd = UPDATE && (ctl->upd_iter == a.iter);
-  const bool do_run = (ctl->run_iter == a.iter);
+  const int iter = ctl->cur;
+  const bool do_upd = UPDATE && (ctl->upd_iter == iter);
+  const bool do_run = (ctl->run_iter == iter);
   if (!do_upd && !do_run) return;
   extern __shared__ unsigned char smem_raw[];
   T2 *lds = reinterpret_cast<T2 *>(smem_raw);
@@ -249,11 +250,11 @@ __global__ __launch_bounds__(kThreads) void k_closure_fx(IterArgs<R, S> a) {
 // tables are staged in LDS behind the tile.
 // ---------------------------------------------------------------------
 template <typename R, int EPT, bool INV, int NL = 0>
-__global__ __launch_bounds__(kThreads) void k_fft_y(typename C2<R>::T *Z, const Ctl *ctl, int iter, int nx,
+__global__ __launch_bounds__(kThreads) void k_fft_y(typename C2<R>::T *Z, const Ctl *ctl, int nx,
                                                      int ny_rt, int nz, long N, int T_rt, int P, AxisPlan py) {
   // NL > 0: compile-time line length (power of two) with T = EPT * kThreads / NL
   using T2 = typename C2<R>::T;
-  if (ctl && ctl->run_iter != iter) return;
+  if (ctl && ctl->run_iter != ctl->cur) return;
   extern __shared__ unsigned char smem_raw[];
   T2 *lds = reinterpret_cast<T2 *>(smem_raw);
   const int ny = NL > 0 ? NL : ny_rt;
@@ -316,7 +317,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_oz(IterArgs<R, S> a) {
   // NL > 0: compile-time z length (power of two), one pack (VC == 2), T = EPT * kThreads / (2 NL)
   using T2 = typename C2<R>::T;
   using T4 = typename R4T<R>::T;
-  if (a.ctl->run_iter != a.iter) return;
+  if (a.ctl->run_iter != a.ctl->cur) return;
   extern __shared__ unsigned char smem_raw[];
   T2 *lds = reinterpret_cast<T2 *>(smem_raw);
   const int nx = a.nx, ny = a.ny, V = a.V;
@@ -488,8 +489,7 @@ static __device__ void leg_solve_dev(double *A, double *x, double *b, int n) {
 // One workgroup: reduce the per-workgroup partials (fixed order), rms,
 // convergence test, Picard / Anderson coefficients (leg), control block.
 template <int HM>
-static __device__ void anderson_solve_block(Ctl *ctl, const double *partials, int nblk, double *rms_hist,
-                                            int iter) {
+static __device__ void anderson_solve_block(Ctl *ctl, const double *partials, int nblk, double *rms_hist) {
   constexpr int NV = 1 + HM + HM * (HM + 1) / 2;
   const int hm = HM;
   __shared__ double tot[NV];
@@ -527,6 +527,8 @@ static __device__ void anderson_solve_block(Ctl *ctl, const double *partials, in
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
+  const int iter = ctl->cur;
+  ctl->cur = iter + 1;
   const double rms = sqrt(tot[0] / ctl->npts);  // cal_rms.cu:18-20
   ctl->step += 1;
   ctl->total += 1;
@@ -583,9 +585,9 @@ static __device__ void anderson_solve_block(Ctl *ctl, const double *partials, in
 
 template <int HM>
 __global__ __launch_bounds__(kThreads) void k_anderson_solve(Ctl *ctl, const double *partials, int nblk,
-                                                             double *rms_hist, int iter) {
-  if (ctl->run_iter != iter) return;
-  anderson_solve_block<HM>(ctl, partials, nblk, rms_hist, iter);
+                                                             double *rms_hist) {
+  if (ctl->run_iter != ctl->cur) return;
+  anderson_solve_block<HM>(ctl, partials, nblk, rms_hist);
 }
 
 // ---------------------------------------------------------------------
@@ -598,7 +600,7 @@ __global__ __launch_bounds__(kThreads) void k_fx_resid(IterArgs<R, S> a) {
   using Q = typename HistT<R>::T;
   constexpr int NV = 1 + HM + HM * (HM + 1) / 2;
   const Ctl *ctl = a.ctl;
-  if (ctl->run_iter != a.iter) return;
+  if (ctl->run_iter != ctl->cur) return;
   extern __shared__ unsigned char smem_raw[];
   T2 *lds = reinterpret_cast<T2 *>(smem_raw);
   const int nx = a.nx, L = a.Lx, ls = line_stride(nx), V = a.V, P = a.P;
@@ -881,14 +883,14 @@ hipError_t launch_fft_y(const IterArgs<R, S> &a, const LaunchGeom &g, bool inver
   if (inverse) {
     RISM_CT_SWITCH(nl, g.ept_y,
                    RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, true>), grid, block, g.lds_y, st, a.Z,
-                                                               a.ctl, a.iter, a.nx, a.ny, a.nz, a.N, g.Ty, a.P, a.py)),
-                   hipLaunchKernelGGL((k_fft_y<R, E, true, NLV>), grid, block, g.lds_y, st, a.Z, a.ctl, a.iter, a.nx,
+                                                               a.ctl, a.nx, a.ny, a.nz, a.N, g.Ty, a.P, a.py)),
+                   hipLaunchKernelGGL((k_fft_y<R, E, true, NLV>), grid, block, g.lds_y, st, a.Z, a.ctl, a.nx,
                                       a.ny, a.nz, a.N, g.Ty, a.P, a.py))
   } else {
     RISM_CT_SWITCH(nl, g.ept_y,
                    RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, false>), grid, block, g.lds_y, st, a.Z,
-                                                               a.ctl, a.iter, a.nx, a.ny, a.nz, a.N, g.Ty, a.P, a.py)),
-                   hipLaunchKernelGGL((k_fft_y<R, E, false, NLV>), grid, block, g.lds_y, st, a.Z, a.ctl, a.iter, a.nx,
+                                                               a.ctl, a.nx, a.ny, a.nz, a.N, g.Ty, a.P, a.py)),
+                   hipLaunchKernelGGL((k_fft_y<R, E, false, NLV>), grid, block, g.lds_y, st, a.Z, a.ctl, a.nx,
                                       a.ny, a.nz, a.N, g.Ty, a.P, a.py))
   }
   return hipGetLastError();
@@ -921,11 +923,11 @@ hipError_t launch_fx_resid(const IterArgs<R, S> &a, const LaunchGeom &g, int hm,
 }
 
 inline hipError_t launch_anderson_solve_impl(Ctl *ctl, const double *partials, int nblk, int nv, int hm,
-                                 double *rms_hist, int iter, hipStream_t st) {
+                                 double *rms_hist, hipStream_t st) {
   switch (hm) {
-    case 2: hipLaunchKernelGGL(k_anderson_solve<2>, dim3(1), dim3(kThreads), 0, st, ctl, partials, nblk, rms_hist, iter); break;
-    case 5: hipLaunchKernelGGL(k_anderson_solve<5>, dim3(1), dim3(kThreads), 0, st, ctl, partials, nblk, rms_hist, iter); break;
-    case 10: hipLaunchKernelGGL(k_anderson_solve<10>, dim3(1), dim3(kThreads), 0, st, ctl, partials, nblk, rms_hist, iter); break;
+    case 2: hipLaunchKernelGGL(k_anderson_solve<2>, dim3(1), dim3(kThreads), 0, st, ctl, partials, nblk, rms_hist); break;
+    case 5: hipLaunchKernelGGL(k_anderson_solve<5>, dim3(1), dim3(kThreads), 0, st, ctl, partials, nblk, rms_hist); break;
+    case 10: hipLaunchKernelGGL(k_anderson_solve<10>, dim3(1), dim3(kThreads), 0, st, ctl, partials, nblk, rms_hist); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -946,11 +948,11 @@ hipError_t launch_fft3d(typename C2<R>::T *data, const AxisPlan &px, const AxisP
   const int ez = (g.Tz * nz <= 4 * kThreads) ? 4 : (g.Tz * nz <= 8 * kThreads) ? 8 : (g.Tz * nz <= 16 * kThreads) ? 16 : 32;
   if (dir < 0) {
     RISM_EPTX_SWITCH(g.ept_x, hipLaunchKernelGGL((k_fft_x_pure<R, E, false>), gx, block, ldsx, st, data, nx, L, px, (R)dv))
-    RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, false>), gy, block, ldsy, st, data, (const Ctl *)nullptr, 0, nx, ny, nz, N, g.Ty, 1, py))
+    RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, false>), gy, block, ldsy, st, data, (const Ctl *)nullptr, nx, ny, nz, N, g.Ty, 1, py))
     RISM_EPT_SWITCH(ez, hipLaunchKernelGGL((k_fft_z_pure<R, E, false>), gz, block, ldsz, st, data, nx, ny, nz, g.Tz, pz))
   } else {
     RISM_EPT_SWITCH(ez, hipLaunchKernelGGL((k_fft_z_pure<R, E, true>), gz, block, ldsz, st, data, nx, ny, nz, g.Tz, pz))
-    RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, true>), gy, block, ldsy, st, data, (const Ctl *)nullptr, 0, nx, ny, nz, N, g.Ty, 1, py))
+    RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, true>), gy, block, ldsy, st, data, (const Ctl *)nullptr, nx, ny, nz, N, g.Ty, 1, py))
     RISM_EPTX_SWITCH(g.ept_x, hipLaunchKernelGGL((k_fft_x_pure<R, E, true>), gx, block, ldsx, st, data, nx, L, px, (R)ivol))
   }
   return hipGetLastError();

@@ -39,6 +39,9 @@ __device__ __forceinline__ double rsqrt_f64(double x) {
 // Solver control block in device memory.  Written by the single-workgroup
 // solve kernel, read by the field kernels to decide whether they run.
 struct Ctl {
+  int cur;        // current iteration index; advanced by the solve kernel, so
+                  // one iteration's launches are identical from step to step
+                  // (replayable as a HIP graph)
   int run_iter;   // iteration index whose closure/FFT/residual pass runs (-1 none)
   int upd_iter;   // iteration whose first pass applies the pending update (-1 none)
   int status;     // -1 running, else RISM_SOLVE_*
@@ -95,7 +98,6 @@ struct IterArgs {
   int Ty, Tz;       // tile widths (consecutive x) of the y / z passes
   int xblocks;      // workgroups of the x passes (grid-stride over line groups)
   int store_h;      // residual pass also writes real-space h into Z
-  int iter;
   // ---- refinement of the FP32 convolution (mixed precision) ----
   // The OZ step is linear in c_s.  With a reference c_ref (FP32 values)
   // whose real-space response h_ref was computed once in FP64, the FP32
@@ -132,7 +134,7 @@ hipError_t launch_fft_z_oz(const IterArgs<R, S> &a, const LaunchGeom &g, hipStre
 template <typename R, typename S>
 hipError_t launch_fx_resid(const IterArgs<R, S> &a, const LaunchGeom &g, int hm, hipStream_t st);
 hipError_t launch_anderson_solve(Ctl *ctl, const double *partials, int nblk, int nv, int hm,
-                                 double *rms_hist, int iter, hipStream_t st);
+                                 double *rms_hist, hipStream_t st);
 // ---- adjoint of the converged fixed point (k_adjoint.hip) ----
 struct AdjArgs {
   long N;
@@ -156,8 +158,8 @@ hipError_t launch_adj_fields(const AdjArgs &a, hipStream_t st);
 hipError_t launch_adj_pack(const AdjArgs &a, const double *x, const double *y, double2 *Z, hipStream_t st);
 hipError_t launch_adj_kspace(const AdjArgs &a, double2 *Z, double2 *Y, hipStream_t st);
 hipError_t launch_adj_rhs(const AdjArgs &a, const double2 *Z, hipStream_t st);
-hipError_t launch_adj_update(const AdjArgs &a, int hm, int iter, hipStream_t st);
-hipError_t launch_adj_resid(const AdjArgs &a, const double2 *Z, int hm, int nblk, int iter, hipStream_t st);
+hipError_t launch_adj_update(const AdjArgs &a, int hm, hipStream_t st);
+hipError_t launch_adj_resid(const AdjArgs &a, const double2 *Z, int hm, int nblk, hipStream_t st);
 hipError_t launch_adj_bar(const AdjArgs &a, const double2 *Z, hipStream_t st);
 int adj_grad_blocks(long N);
 hipError_t launch_adj_grad(const AdjArgs &a, const double2 *Y, double *part6, double *part4, double *out6,

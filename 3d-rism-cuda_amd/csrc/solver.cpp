@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "rism_host.h"
@@ -127,7 +128,10 @@ struct rism_handle {
   double cf = 1.0;
   bool delta = false;       // mixed: FP32 passes work on c_s - cref
   int refreshes = 0;
-  int iter = 0;
+  // replayable chunks of iterations: [precision][delta mode]
+  hipGraphExec_t gexec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  int gchunk[2][2] = {{0, 0}, {0, 0}};
+  bool graph_ok = true;  // false after a capture failed on this handle
   int steps = 0;
   double rms = 0.0;
   double t_setup = 0, t_solve = 0, t_thermo = 0;
@@ -193,9 +197,25 @@ int prof_collect(rism_handle *h) {
     if (int _rc = prof_done(h)) return _rc;               \
   } while (0)
 
+// Device allocation and release (which synchronise the device) never
+// overlap the recording of a HIP graph on another handle's stream: handles
+// used from several host threads at once take this lock around both.
+static std::mutex g_device_mu;
+
+// blocking copy ordered on the handle's own stream (never on the legacy
+// stream, which would tie this handle to others, e.g. one being captured)
+hipError_t hcopy(rism_handle *h, void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, h->stream);
+  return e == hipSuccess ? hipStreamSynchronize(h->stream) : e;
+}
+
 int dalloc(rism_handle *h, void **p, size_t bytes) {
   if (bytes == 0) bytes = 16;
-  hipError_t e = hipMalloc(p, bytes);
+  hipError_t e;
+  {
+    std::lock_guard<std::mutex> lk(g_device_mu);
+    e = hipMalloc(p, bytes);
+  }
   if (e != hipSuccess) {
     set_error("device allocation of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
     *p = nullptr;
@@ -260,7 +280,7 @@ template <typename R> int build_plans(rism_handle *h) {
     p.pre = dt + off + n;
     off += 2 * (size_t)n;
   }
-  HIP_OK(hipMemcpy(dt, host.data(), tot * sizeof(T2), hipMemcpyHostToDevice));
+  HIP_OK(hcopy(h, dt, host.data(), tot * sizeof(T2), hipMemcpyHostToDevice));
   ps.tables = dt;
   return RISM_OK;
 }
@@ -311,7 +331,7 @@ int choose_geometry(rism_handle *h, LaunchGeom &g, size_t rsz) {
   return RISM_OK;
 }
 
-template <typename R> IterArgs<R, double> make_args(rism_handle *h, int iter) {
+template <typename R> IterArgs<R, double> make_args(rism_handle *h) {
   IterArgs<R, double> a;
   std::memset(&a, 0, sizeof a);
   const PrecSet &ps = h->ps[pidx<R>()];
@@ -347,7 +367,6 @@ template <typename R> IterArgs<R, double> make_args(rism_handle *h, int iter) {
   a.Lx = ps.geom.Lx; a.Ty = ps.geom.Ty; a.Tz = ps.geom.Tz;
   a.xblocks = ps.geom.xblocks;
   a.store_h = 0;
-  a.iter = iter;
   a.use_fk = 1;
   if (pidx<R>() == 1) {
     a.cref = h->cref;
@@ -359,9 +378,9 @@ template <typename R> IterArgs<R, double> make_args(rism_handle *h, int iter) {
   return a;
 }
 
-template <typename R> int launch_iteration(rism_handle *h, int it, bool store_h, bool solve,
+template <typename R> int launch_iteration(rism_handle *h, bool store_h, bool solve,
                                           bool h_only = false) {
-  IterArgs<R, double> a = make_args<R>(h, it);
+  IterArgs<R, double> a = make_args<R>(h);
   const LaunchGeom &g = h->ps[pidx<R>()].geom;
   a.store_h = store_h ? 1 : 0;
   a.href_only = h_only ? 1 : 0;
@@ -371,7 +390,7 @@ template <typename R> int launch_iteration(rism_handle *h, int it, bool store_h,
   PROF(h, 3, launch_fft_y<R, double>(a, g, true, h->stream));
   PROF(h, 4, launch_fx_resid<R, double>(a, g, h->hm, h->stream));
   if (solve)
-    PROF(h, 5, launch_anderson_solve(h->ctl, h->partials, g.xblocks, a.nv, h->hm, h->rms_hist, it,
+    PROF(h, 5, launch_anderson_solve(h->ctl, h->partials, g.xblocks, a.nv, h->hm, h->rms_hist,
                                      h->stream));
   return RISM_OK;
 }
@@ -382,14 +401,14 @@ template <typename R> int launch_iteration(rism_handle *h, int it, bool store_h,
 // transform the reference into h_ref (long-range term included) and the
 // FP32 residual pass adds it back.  From here on the FP32 passes run in
 // delta mode.
-int launch_refresh_iteration(rism_handle *h, int it) {
+int launch_refresh_iteration(rism_handle *h) {
   h->delta = true;
   ++h->refreshes;
-  IterArgs<float, double> a = make_args<float>(h, it);
+  IterArgs<float, double> a = make_args<float>(h);
   const LaunchGeom &g = h->ps[1].geom;
   a.refresh = 1;
   PROF(h, 0, launch_closure_fx<float, double>(a, g, true, h->hm, h->stream));
-  IterArgs<double, double> d = make_args<double>(h, it);
+  IterArgs<double, double> d = make_args<double>(h);
   const LaunchGeom &gd = h->ps[0].geom;
   d.src = 1;
   d.cref = h->cref;
@@ -405,7 +424,7 @@ int launch_refresh_iteration(rism_handle *h, int it) {
   PROF(h, 2, launch_fft_z_oz<float, double>(a, g, h->stream));
   PROF(h, 3, launch_fft_y<float, double>(a, g, true, h->stream));
   PROF(h, 4, launch_fx_resid<float, double>(a, g, h->hm, h->stream));
-  PROF(h, 5, launch_anderson_solve(h->ctl, h->partials, g.xblocks, a.nv, h->hm, h->rms_hist, it, h->stream));
+  PROF(h, 5, launch_anderson_solve(h->ctl, h->partials, g.xblocks, a.nv, h->hm, h->rms_hist, h->stream));
   return RISM_OK;
 }
 
@@ -440,8 +459,8 @@ int get_ctl(rism_handle *h) {
 int fetch_t(rism_handle *h, double *out) {
   const size_t VN = (size_t)h->V * h->N;
   std::vector<double> fr(h->N);
-  HIP_OK(hipMemcpy(out, h->ts, VN * sizeof(double), hipMemcpyDeviceToHost));
-  HIP_OK(hipMemcpy(fr.data(), h->fr, h->N * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_OK(hcopy(h, out, h->ts, VN * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_OK(hcopy(h, fr.data(), h->fr, h->N * sizeof(double), hipMemcpyDeviceToHost));
   for (int v = 0; v < h->V; ++v) {
     const double q = h->qv[v] * h->cf;
     for (long i = 0; i < h->N; ++i) out[(size_t)v * h->N + i] += q * fr[i];
@@ -452,12 +471,12 @@ int fetch_t(rism_handle *h, double *out) {
 int store_t(rism_handle *h, const double *t) {
   const size_t VN = (size_t)h->V * h->N;
   std::vector<double> fr(h->N), ts(VN);
-  HIP_OK(hipMemcpy(fr.data(), h->fr, h->N * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_OK(hcopy(h, fr.data(), h->fr, h->N * sizeof(double), hipMemcpyDeviceToHost));
   for (int v = 0; v < h->V; ++v) {
     const double q = h->qv[v] * h->cf;
     for (long i = 0; i < h->N; ++i) ts[(size_t)v * h->N + i] = t[(size_t)v * h->N + i] - q * fr[i];
   }
-  HIP_OK(hipMemcpy(h->ts, ts.data(), VN * sizeof(double), hipMemcpyHostToDevice));
+  HIP_OK(hcopy(h, h->ts, ts.data(), VN * sizeof(double), hipMemcpyHostToDevice));
   return RISM_OK;
 }
 
@@ -507,6 +526,58 @@ int read_tuv_file(rism_handle *h, const char *path) {
   return store_t(h, t.data());
 }
 
+static void drop_graphs(rism_handle *h) {
+  for (int p = 0; p < 2; ++p)
+    for (int d = 0; d < 2; ++d) {
+      if (h->gexec[p][d]) (void)hipGraphExecDestroy(h->gexec[p][d]);
+      h->gexec[p][d] = nullptr;
+      h->gchunk[p][d] = 0;
+    }
+}
+
+// n iterations as one HIP graph: captured on first use for the current
+// precision and refinement mode, then replayed.  The launches of an
+// iteration do not depend on its index (the kernels read it from the
+// control block), and they hold no memset or copy nodes.
+template <typename R> int launch_chunk_graph(rism_handle *h, int n) {
+  hipGraphExec_t &ex = h->gexec[pidx<R>()][h->delta ? 1 : 0];
+  int &gn = h->gchunk[pidx<R>()][h->delta ? 1 : 0];
+  if (!h->graph_ok) {
+    for (int k = 0; k < n; ++k)
+      if (int rc = launch_iteration<R>(h, false, true)) return rc;
+    return RISM_OK;
+  }
+  if (!ex || gn != n) {
+    if (ex) (void)hipGraphExecDestroy(ex);
+    ex = nullptr;
+    // relaxed mode: other host threads (other handles) keep allocating and
+    // copying while this stream records
+    std::unique_lock<std::mutex> lk(g_device_mu);
+    hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed);
+    int rc = RISM_OK;
+    hipGraph_t g = nullptr;
+    if (e == hipSuccess) {
+      for (int k = 0; k < n && rc == RISM_OK; ++k) rc = launch_iteration<R>(h, false, true);
+      e = hipStreamEndCapture(h->stream, &g);
+    }
+    if (e == hipSuccess && rc == RISM_OK) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    lk.unlock();
+    if (e != hipSuccess || rc != RISM_OK) {  // no graph: launch this handle's chunks eagerly
+      (void)hipGetLastError();
+      if (ex) (void)hipGraphExecDestroy(ex);
+      ex = nullptr;
+      h->graph_ok = false;
+      for (int k = 0; k < n; ++k)
+        if (int r2 = launch_iteration<R>(h, false, true)) return r2;
+      return RISM_OK;
+    }
+    gn = n;
+  }
+  HIP_OK(hipGraphLaunch(ex, h->stream));
+  return RISM_OK;
+}
+
 template <typename R> int solve_impl(rism_handle *h, const char *tuv_path, int *steps_out, double *rms_out) {
   const double t0 = now_s();
   const int V = h->V;
@@ -546,7 +617,11 @@ template <typename R> int solve_impl(rism_handle *h, const char *tuv_path, int *
   c.upd_iter = -1;
   int status = RISM_SOLVE_CONVERGED;
   h->rms_log.clear();
-  int it = h->iter;
+  // iteration chunks replay as HIP graphs unless timing or tracing every
+  // step (RISM_GRAPHS=0 turns them off)
+  const char *genv = std::getenv("RISM_GRAPHS");
+  const bool graphs = !(genv && genv[0] == '0') && !h->opts.verbose && !h->prof && h->opts.ksave <= 0;
+  drop_graphs(h);
   // iterations enqueued per control-block read; 1 when timing kernels so no
   // gated (empty) launches enter the averages
   const int chunk = (h->opts.verbose || h->prof) ? 1 : (N >= (1L << 21) ? 4 : 16);
@@ -577,6 +652,7 @@ template <typename R> int solve_impl(rism_handle *h, const char *tuv_path, int *
       }
     }
     if (int rc = stage_potentials(h)) return rc;
+    drop_graphs(h);    // the captured launches carry q_v cf
     h->delta = false;  // a new stage (new cf) starts without a reference
     double ref_rms = -1.0, best_rms = HUGE_VAL;
     int best_at = 0, since_ref = 0;
@@ -584,20 +660,22 @@ template <typename R> int solve_impl(rism_handle *h, const char *tuv_path, int *
     if (verbose) std::printf("relaxing 3D UV RISM: Charge Up Factor = %g\n", h->cf);
     c.status = -1;
     c.step = 0;
-    c.run_iter = it;
+    c.run_iter = c.cur;
     c.upd_iter = -1;
     if (int rc = put_ctl(h)) return rc;
     while (true) {
-      const int n_launch = (h->opts.ksave > 0) ? 1 : chunk;
-      for (int k = 0; k < n_launch; ++k, ++it) {
-        if (pidx<R>() == 1 && want_refresh && k == 0) {
-          if (int rc = launch_refresh_iteration(h, it)) return rc;
-          want_refresh = false;
-          since_ref = 0;
-          ref_rms = -2.0;  // set from the next rms read
-        } else {
-          if (int rc = launch_iteration<R>(h, it, false, true)) return rc;
-        }
+      int n_launch = (h->opts.ksave > 0) ? 1 : chunk;
+      if (pidx<R>() == 1 && want_refresh) {
+        if (int rc = launch_refresh_iteration(h)) return rc;
+        want_refresh = false;
+        since_ref = 0;
+        ref_rms = -2.0;  // set from the next rms read
+        n_launch = 1;
+      } else if (graphs && n_launch > 1) {
+        if (int rc = launch_chunk_graph<R>(h, n_launch)) return rc;
+      } else {
+        for (int k = 0; k < n_launch; ++k)
+          if (int rc = launch_iteration<R>(h, false, true)) return rc;
       }
       if (int rc = get_ctl(h)) return rc;
       if (pidx<R>() == 1 && c.status < 0) {
@@ -612,7 +690,7 @@ template <typename R> int solve_impl(rism_handle *h, const char *tuv_path, int *
       // progress lines (iterate.cu:63)
       if (c.total > printed) {
         std::vector<double> tail(c.total - printed);
-        HIP_OK(hipMemcpy(tail.data(), h->rms_hist + printed, tail.size() * sizeof(double),
+        HIP_OK(hcopy(h, tail.data(), h->rms_hist + printed, tail.size() * sizeof(double),
                          hipMemcpyDeviceToHost));
         for (size_t k = 0; k < tail.size(); ++k) {
           h->rms_log.push_back(tail[k]);
@@ -626,8 +704,8 @@ template <typename R> int solve_impl(rism_handle *h, const char *tuv_path, int *
         // iterate.cu:64-66 writes t after the Anderson update of this step:
         // apply the pending update now (one first pass), then clear it so
         // the next iteration does not apply it again
-        if (c.upd_iter == it) {
-          IterArgs<R, double> a = make_args<R>(h, it);
+        if (c.upd_iter == c.cur) {
+          IterArgs<R, double> a = make_args<R>(h);
           HIP_OK(launch_closure_fx<R, double>(a, h->ps[pidx<R>()].geom, true, h->hm, h->stream));
           c.upd_iter = -1;
           if (int rc = put_ctl(h)) return rc;
@@ -639,12 +717,11 @@ template <typename R> int solve_impl(rism_handle *h, const char *tuv_path, int *
     }
     // a step-limit stop still applies its pending update (iterate.cu:61 runs
     // before the loop ends)
-    if (c.upd_iter >= 0 && c.upd_iter == it) {
+    if (c.upd_iter >= 0 && c.upd_iter == c.cur) {
       c.run_iter = -1;  // update only, no closure/FFT
       if (int rc = put_ctl(h)) return rc;
-      IterArgs<R, double> a = make_args<R>(h, it);
+      IterArgs<R, double> a = make_args<R>(h);
       HIP_OK(launch_closure_fx<R, double>(a, h->ps[pidx<R>()].geom, true, h->hm, h->stream));
-      ++it;
     }
     status = c.status;
     if (status == RISM_SOLVE_DIVERGED) {
@@ -659,7 +736,6 @@ template <typename R> int solve_impl(rism_handle *h, const char *tuv_path, int *
   }
   HIP_OK(hipStreamSynchronize(h->stream));
   if (int rc = prof_collect(h)) return rc;
-  h->iter = it;
   h->steps = c.total;
   h->rms = c.rms;
   h->solved = true;
@@ -673,13 +749,12 @@ template <typename R> int solve_impl(rism_handle *h, const char *tuv_path, int *
 template <typename R> int evaluate_impl(rism_handle *h) {
   Ctl &c = *h->ctl_host;
   if (int rc = get_ctl(h)) return rc;
-  const int it = h->iter++;
-  c.run_iter = it;
+  c.run_iter = c.cur;
   c.upd_iter = -1;
   const int keep = c.status;
   if (int rc = put_ctl(h)) return rc;
-  if (int rc = launch_iteration<R>(h, it, true, false, true)) return rc;
-  IterArgs<R, double> a = make_args<R>(h, it);
+  if (int rc = launch_iteration<R>(h, true, false, true)) return rc;
+  IterArgs<R, double> a = make_args<R>(h);
   HIP_OK(launch_gfield<R, double>(a, h->gfield, h->stream));
   c.run_iter = -1;
   c.status = keep;
@@ -696,7 +771,7 @@ int evaluate(rism_handle *h) { return evaluate_impl<double>(h); }
 template <typename R> int thermo_impl(rism_handle *h, rism_xmu *x) {
   const int V = h->V;
   const int nblk = 1024;
-  IterArgs<R, double> a = make_args<R>(h, h->iter);
+  IterArgs<R, double> a = make_args<R>(h);
   PROF(h, 8, launch_exchem<R, double>(a, h->fr, h->partials, nblk, h->stream));
   std::vector<double> part((size_t)nblk * 4 * V);
   HIP_OK(hipMemcpyAsync(part.data(), h->partials, part.size() * sizeof(double), hipMemcpyDeviceToHost,
@@ -784,7 +859,7 @@ template <typename R> int alloc_precision(rism_handle *h) {
       }
     }
   if (int rc = dalloc(h, &ps.chi, tab.size() * sizeof(T4))) return rc;
-  HIP_OK(hipMemcpy(ps.chi, tab.data(), tab.size() * sizeof(T4), hipMemcpyHostToDevice));
+  HIP_OK(hcopy(h, ps.chi, tab.data(), tab.size() * sizeof(T4), hipMemcpyHostToDevice));
   ps.ready = true;
   return RISM_OK;
 }
@@ -834,7 +909,11 @@ int create_impl(const rism_grid *grid, const rism_solvent *sv, const rism_solute
   }
   h->device = device;
   HIP_OK(hipSetDevice(device));
-  HIP_OK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  {
+    std::lock_guard<std::mutex> lk(g_device_mu);
+    HIP_OK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    HIP_OK(hipHostMalloc((void **)&h->ctl_host, sizeof(Ctl), hipHostMallocDefault));
+  }
   h->nx = grid->n[0]; h->ny = grid->n[1]; h->nz = grid->n[2];
   h->N = (long)h->nx * h->ny * h->nz;
   for (int k = 0; k < 3; ++k) {  // cell.cc:8-22
@@ -887,7 +966,6 @@ int create_impl(const rism_grid *grid, const rism_solvent *sv, const rism_solute
       (rc = dalloc_t(h, &h->sigh, (size_t)h->V * std::max(1, h->natu))) || (rc = dalloc_t(h, &h->d_qv, h->V)) ||
       (rc = dalloc_t(h, &h->d_rho, h->V)))
     return rc;
-  HIP_OK(hipHostMalloc((void **)&h->ctl_host, sizeof(Ctl), hipHostMallocDefault));
   std::memset(h->ctl_host, 0, sizeof(Ctl));
   if ((rc = alloc_precision<double>(h))) return rc;
   if (o->precision == RISM_PREC_MIXED) {
@@ -912,11 +990,11 @@ int create_impl(const rism_grid *grid, const rism_solvent *sv, const rism_solute
       lj[(size_t)v * h->natu + u] = make_double2(s * s, e * 4.0);
       sigh[(size_t)v * h->natu + u] = s * 0.5;
     }
-  HIP_OK(hipMemcpy(h->atoms, at.data(), at.size() * sizeof(double4), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(h->lj, lj.data(), lj.size() * sizeof(double2), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(h->sigh, sigh.data(), sigh.size() * sizeof(double), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(h->d_qv, h->qv.data(), h->V * sizeof(double), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(h->d_rho, h->rho.data(), h->V * sizeof(double), hipMemcpyHostToDevice));
+  HIP_OK(hcopy(h, h->atoms, at.data(), at.size() * sizeof(double4), hipMemcpyHostToDevice));
+  HIP_OK(hcopy(h, h->lj, lj.data(), lj.size() * sizeof(double2), hipMemcpyHostToDevice));
+  HIP_OK(hcopy(h, h->sigh, sigh.data(), sigh.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIP_OK(hcopy(h, h->d_qv, h->qv.data(), h->V * sizeof(double), hipMemcpyHostToDevice));
+  HIP_OK(hcopy(h, h->d_rho, h->rho.data(), h->V * sizeof(double), hipMemcpyHostToDevice));
   h->t_setup = now_s() - t0;
   return RISM_OK;
 }
@@ -1015,15 +1093,21 @@ int rism_grad(rism_handle *h, double *out) {
   if (nu == 0) return RISM_OK;
   const int nblk = (int)((h->N + (long)kThreads * 16 - 1) / ((long)kThreads * 16));
   double *part = nullptr, *dout = nullptr;
-  HIP_OK(hipMalloc((void **)&part, (size_t)nblk * nu * 6 * sizeof(double)));
-  HIP_OK(hipMalloc((void **)&dout, (size_t)nu * 6 * sizeof(double)));
+  {
+    std::lock_guard<std::mutex> lk(g_device_mu);
+    HIP_OK(hipMalloc((void **)&part, (size_t)nblk * nu * 6 * sizeof(double)));
+    HIP_OK(hipMalloc((void **)&dout, (size_t)nu * 6 * sizeof(double)));
+  }
   hipError_t e = launch_grad(h->nx, h->ny, h->nz, h->dr, h->V, nu, h->atoms, h->lj, h->sigh, h->d_qv, h->d_rho,
                              h->gfield, part, nblk, dout, h->stream);
   std::vector<double> g((size_t)nu * 6);
   if (e == hipSuccess) e = hipMemcpyAsync(g.data(), dout, g.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-  (void)hipFree(part);
-  (void)hipFree(dout);
+  {
+    std::lock_guard<std::mutex> lk(g_device_mu);
+    (void)hipFree(part);
+    (void)hipFree(dout);
+  }
   if (e != hipSuccess) {
     set_error(std::string("HIP error in rism_grad: ") + hipGetErrorString(e));
     return RISM_ERR_HIP;
@@ -1125,10 +1209,10 @@ int rism_adjoint_grad(rism_handle *h, double rtol, double *dxyz, double *dq, dou
   const int nv = 1 + h->hm + h->hm * (h->hm + 1) / 2;
   int it = 0;
   for (;;) {
-    HIP_OK(launch_adj_update(a, h->hm, it, st));
+    HIP_OK(launch_adj_update(a, h->hm, st));
     HIP_OK(apply_kt(a.lam, nullptr, nullptr));
-    HIP_OK(launch_adj_resid(a, h->adj_Z, h->hm, nblk, it, st));
-    HIP_OK(launch_anderson_solve(h->adj_ctl, h->partials, nblk, nv, h->hm, h->adj_rms, it, st));
+    HIP_OK(launch_adj_resid(a, h->adj_Z, h->hm, nblk, st));
+    HIP_OK(launch_anderson_solve(h->adj_ctl, h->partials, nblk, nv, h->hm, h->adj_rms, st));
     ++it;
     if (it == 1 || it % 8 == 0 || it >= h->opts.maxstep + 1) {
       HIP_OK(hipMemcpyAsync(&c, h->adj_ctl, sizeof c, hipMemcpyDeviceToHost, st));
@@ -1151,16 +1235,22 @@ int rism_adjoint_grad(rism_handle *h, double rtol, double *dxyz, double *dq, dou
   if (nu > 0) {
     const int gblk = adj_grad_blocks(N);
     double *p6 = nullptr, *p4 = nullptr, *o = nullptr;
-    HIP_OK(hipMalloc((void **)&p6, (size_t)gblk * nu * 6 * sizeof(double)));
-    HIP_OK(hipMalloc((void **)&p4, (size_t)gblk * nu * 4 * sizeof(double)));
-    HIP_OK(hipMalloc((void **)&o, (size_t)nu * 10 * sizeof(double)));
+    {
+      std::lock_guard<std::mutex> lk(g_device_mu);
+      HIP_OK(hipMalloc((void **)&p6, (size_t)gblk * nu * 6 * sizeof(double)));
+      HIP_OK(hipMalloc((void **)&p4, (size_t)gblk * nu * 4 * sizeof(double)));
+      HIP_OK(hipMalloc((void **)&o, (size_t)nu * 10 * sizeof(double)));
+    }
     hipError_t e = launch_adj_grad(a, h->adj_Y, p6, p4, o, o + (size_t)nu * 6, st);
     std::vector<double> r((size_t)nu * 10);
     if (e == hipSuccess) e = hipMemcpyAsync(r.data(), o, r.size() * sizeof(double), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    (void)hipFree(p6);
-    (void)hipFree(p4);
-    (void)hipFree(o);
+    {
+      std::lock_guard<std::mutex> lk(g_device_mu);
+      (void)hipFree(p6);
+      (void)hipFree(p4);
+      (void)hipFree(o);
+    }
     if (e != hipSuccess) {
       set_error(std::string("HIP error in rism_adjoint_grad: ") + hipGetErrorString(e));
       return RISM_ERR_HIP;
@@ -1190,20 +1280,20 @@ int rism_get_field(rism_handle *h, int which, double *out) {
     case RISM_FIELD_T: return fetch_t(h, out);
     case RISM_FIELD_RES:
       if (h->qsz == sizeof(double)) {
-        HIP_OK(hipMemcpy(out, h->res, VN * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_OK(hcopy(h, out, h->res, VN * sizeof(double), hipMemcpyDeviceToHost));
       } else {
         std::vector<float> r(VN);
-        HIP_OK(hipMemcpy(r.data(), h->res, VN * sizeof(float), hipMemcpyDeviceToHost));
+        HIP_OK(hcopy(h, r.data(), h->res, VN * sizeof(float), hipMemcpyDeviceToHost));
         for (size_t i = 0; i < VN; ++i) out[i] = r[i];
       }
       return RISM_OK;
-    case RISM_FIELD_ULJ: HIP_OK(hipMemcpy(out, h->ulj, VN * sizeof(double), hipMemcpyDeviceToHost)); return RISM_OK;
-    case RISM_FIELD_DE: HIP_OK(hipMemcpy(out, h->de, N * sizeof(double), hipMemcpyDeviceToHost)); return RISM_OK;
-    case RISM_FIELD_FR: HIP_OK(hipMemcpy(out, h->fr, N * sizeof(double), hipMemcpyDeviceToHost)); return RISM_OK;
+    case RISM_FIELD_ULJ: HIP_OK(hcopy(h, out, h->ulj, VN * sizeof(double), hipMemcpyDeviceToHost)); return RISM_OK;
+    case RISM_FIELD_DE: HIP_OK(hcopy(h, out, h->de, N * sizeof(double), hipMemcpyDeviceToHost)); return RISM_OK;
+    case RISM_FIELD_FR: HIP_OK(hcopy(h, out, h->fr, N * sizeof(double), hipMemcpyDeviceToHost)); return RISM_OK;
     case RISM_FIELD_FK: {
       const int nx = h->nx, ny = h->ny, nz = h->nz, nyh = ny / 2;
       std::vector<double2> half((size_t)nx * nyh * nz);
-      HIP_OK(hipMemcpy(half.data(), h->fkh, half.size() * sizeof(double2), hipMemcpyDeviceToHost));
+      HIP_OK(hcopy(h, half.data(), h->fkh, half.size() * sizeof(double2), hipMemcpyDeviceToHost));
       for (int z = 0; z < nz; ++z)
         for (int y = 0; y < ny; ++y)
           for (int x = 0; x < nx; ++x) {
@@ -1249,13 +1339,13 @@ int rism_get_field(rism_handle *h, int which, double *out) {
       if (!h->evaluated)
         if (int rc = evaluate(h)) return rc;
       if (which == RISM_FIELD_G) {
-        HIP_OK(hipMemcpy(out, h->gfield, VN * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_OK(hcopy(h, out, h->gfield, VN * sizeof(double), hipMemcpyDeviceToHost));
         return RISM_OK;
       }
       std::vector<double> hh(VN);
       {
         std::vector<double2> z((size_t)h->P * N);  // FP64 evaluation result
-        HIP_OK(hipMemcpy(z.data(), h->ps[0].Z, z.size() * sizeof(double2), hipMemcpyDeviceToHost));
+        HIP_OK(hcopy(h, z.data(), h->ps[0].Z, z.size() * sizeof(double2), hipMemcpyDeviceToHost));
         for (int v = 0; v < V; ++v)
           for (long i = 0; i < N; ++i) hh[(size_t)v * N + i] = (v & 1) ? z[(size_t)(v >> 1) * N + i].y : z[(size_t)(v >> 1) * N + i].x;
       }
@@ -1317,6 +1407,8 @@ void rism_destroy(rism_handle *h) {
   if (!h) return;
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  std::lock_guard<std::mutex> lk(g_device_mu);
+  drop_graphs(h);
   for (auto &b : h->bufs) (void)hipFree(b.p);
   for (auto &e : h->ev_pool) (void)hipEventDestroy(e);
   if (h->ctl_host) (void)hipHostFree(h->ctl_host);
@@ -1386,12 +1478,11 @@ int rism_calculate(rism_handle *h, const double *t_dev, double *res_dev) {
   c.maxstep = 1;
   c.npts = (double)VN;
   c.status = -1;
-  const int it = h->iter++;
-  c.run_iter = it;
+  c.run_iter = c.cur;
   c.upd_iter = -1;
   if ((rc = put_ctl(h))) return rc;
-  rc = (h->opts.precision == RISM_PREC_MIXED) ? launch_iteration<float>(h, it, true, false)
-                                              : launch_iteration<double>(h, it, true, false);
+  rc = (h->opts.precision == RISM_PREC_MIXED) ? launch_iteration<float>(h, true, false)
+                                              : launch_iteration<double>(h, true, false);
   if (rc) return rc;
   if (h->qsz == sizeof(double))
     HIP_OK(hipMemcpyAsync(res_dev, h->res, VN * sizeof(double), hipMemcpyDeviceToDevice, h->stream));

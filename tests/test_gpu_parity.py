@@ -246,3 +246,23 @@ def test_benzene_256_matches_reference_xmu(gpu, benzene_dir, prec):
     if prec == rism3d.FP64:
         assert x["Correction_Term"] == pytest.approx(ref["Correction_Term"], abs=0.01)
     s.close()
+
+
+@pytest.mark.parametrize("prec", [rism3d.FP64, rism3d.MIXED])
+def test_graph_replay_matches_eager_launches(gpu, deckdir, prec, monkeypatch):
+    """Iteration chunks replayed as HIP graphs (many replays, with control
+    block reads between them) give the eager launches' results exactly:
+    the same residual after every step and the same Delta-mu."""
+    print(f"graph replay prec={prec}", flush=True)
+    deck = rism3d.read_deck("test1.inp")
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("RISM_GRAPHS", mode)
+        s = rism3d.Solver.from_deck(deck, precision=prec).build_potentials()
+        st, steps, rms = s.solve()
+        out[mode] = (st, steps, s.rms_history(), s.thermo()["SFE_SC"])
+        s.close()
+    assert out["0"][0] == out["1"][0] == rism3d.SOLVE_CONVERGED
+    assert out["0"][1] == out["1"][1]
+    assert np.array_equal(out["0"][2], out["1"][2])
+    assert out["0"][3] == out["1"][3]
