@@ -101,7 +101,10 @@ int main(int argc, char **argv) {
       std::printf("%s\n", rism_last_error());
       return 1;
     }
-    rism_write_xmu(&x, o.closure, (fname + (q0 ? ".xmu0" : ".xmu")).c_str());
+    if (rism_write_xmu(&x, o.closure, (fname + (q0 ? ".xmu0" : ".xmu")).c_str()) != RISM_OK) {
+      std::printf("%s\n", rism_last_error());
+      return 1;
+    }
   }
   if (outlist.find('d') != std::string::npos) {
     std::vector<double> gr((size_t)su.natom * 6);
@@ -110,7 +113,10 @@ int main(int argc, char **argv) {
       std::printf("%s\n", rism_last_error());
       return 1;
     }
-    rism_write_gra(gr.data(), su.natom, (fname + ".gra").c_str());
+    if (rism_write_gra(gr.data(), su.natom, (fname + ".gra").c_str()) != RISM_OK) {
+      std::printf("%s\n", rism_last_error());
+      return 1;
+    }
     std::printf("done.\n");
   }
   for (char c : std::string("cgheq"))

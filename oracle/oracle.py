@@ -74,6 +74,8 @@ def lib():
         L.orc_set_tr.argtypes = [vp, dp]
         L.orc_copy_hist.argtypes = [vp, C.c_int, C.c_int, dp]
         L.orc_set_atom_xyz.argtypes = [vp, C.c_int, dp]
+        L.orc_write_tuv.argtypes = [vp, C.c_char_p]
+        L.orc_read_tuv.argtypes = [vp, C.c_char_p]
         _lib = L
     return _lib
 
@@ -183,6 +185,11 @@ class Oracle:
     def set_tr(self, tr):
         tr = np.ascontiguousarray(tr, dtype=np.float64).reshape(-1)
         lib().orc_set_tr(self.s, _d(tr))
+
+    def write_tuv(self, path):
+        """write_tuv.cu format: header n[3] V, then t, 6 significant digits."""
+        if lib().orc_write_tuv(self.s, path.encode()) != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
 
     def set_atom_xyz(self, u, xyz):
         """Move atom u without recomputing the potentials (test hook)."""
