@@ -36,7 +36,8 @@ EXPORTED = [
     "rism_thermo", "rism_grad", "rism_get_field", "rism_set_t", "rism_save_tuv", "rism_load_tuv",
     "rism_write_xmu", "rism_write_gra", "rism_stats", "rism_rms_history", "rism_destroy",
     "rism_fft3d", "rism_calculate", "rism_profile", "rism_kernel_times", "rism_geometry",
-    "rism_adjoint_grad", "rism_batch_solve",
+    "rism_adjoint_grad", "rism_batch_solve", "rism_chi_default_opts", "rism_chi_nreduced",
+    "rism_chi_solve", "rism_write_xvk", "rism_xvk_read_sites",
 ]
 
 KERNELS = ["closure_fx", "fft_y_fwd", "fft_z_oz", "fft_y_inv", "resid", "solve", "potentials", "fk",
@@ -80,6 +81,18 @@ class Xmu(C.Structure):
             "SFEC_GF": list(self.sfec_gf[:v]), "SE": self.se, "SE_LJ": self.se_lj, "SE_ES": self.se_es,
             "PMV": self.pmv, "Pressure": self.pressure, "Correction_Term": self.correction,
         }
+
+
+class ChiSite(C.Structure):
+    """One .xvk site row (solvent_read.cc:42-47), include/rism3d_amd.h rism_chi_site."""
+    _fields_ = [("name", C.c_char * 8), ("species", C.c_int), ("sym", C.c_int), ("sig", C.c_double),
+                ("eps", C.c_double), ("q", C.c_double), ("xyz", C.c_double * 3), ("dens", C.c_double)]
+
+
+class ChiOpts(C.Structure):
+    _fields_ = [("ntab", C.c_int), ("dr", C.c_double), ("temper", C.c_double), ("closure", C.c_int),
+                ("tol", C.c_double), ("maxstep", C.c_int), ("nhist", C.c_int), ("mix", C.c_double),
+                ("chargeup", C.c_int)]
 
 
 class DeckC(C.Structure):
@@ -143,6 +156,13 @@ def lib():
         L.rism_profile.argtypes = [C.c_void_p, C.c_int]
         L.rism_kernel_times.argtypes = [C.c_void_p, dp, P(C.c_long)]
         L.rism_geometry.argtypes = [C.c_void_p, P(C.c_int)]
+        L.rism_chi_default_opts.argtypes = [P(ChiOpts)]
+        L.rism_chi_default_opts.restype = None
+        L.rism_chi_nreduced.argtypes = [P(ChiSite), C.c_int]
+        L.rism_chi_solve.argtypes = [P(ChiSite), C.c_int, P(ChiOpts), dp, dp, P(C.c_int), dp]
+        L.rism_write_xvk.argtypes = [C.c_char_p, P(ChiSite), C.c_int, C.c_int, C.c_double, C.c_double,
+                                     C.c_double, dp]
+        L.rism_xvk_read_sites.argtypes = [C.c_char_p, P(ChiSite), C.c_int, P(C.c_int), P(C.c_int), dp, dp]
         _lib = L
     return _lib
 
@@ -255,6 +275,60 @@ def read_xvk(path: str) -> Solvent:
             p = getattr(d, name)
             if p:
                 libc.free(C.cast(p, C.c_void_p))
+
+
+def solvent_model(path: str):
+    """The site rows, ntab, dr and temperature of an .xvk header as a model
+    for chi_solve: (sites ChiSite array, ntab, dr, temperature)."""
+    sites = (ChiSite * 64)()
+    ns, nt, dr, T = C.c_int(), C.c_int(), C.c_double(), C.c_double()
+    _check(lib().rism_xvk_read_sites(path.encode(), sites, 64, C.byref(ns), C.byref(nt), C.byref(dr), C.byref(T)),
+           "rism_xvk_read_sites")
+    out = (ChiSite * ns.value)()
+    C.memmove(out, sites, C.sizeof(ChiSite) * ns.value)
+    return out, nt.value, dr.value, T.value
+
+
+def make_sites(rows) -> C.Array:
+    """ChiSite array from rows (name, species, sym, sig A, eps J/mol, q e, (x, y, z) A, conc M)."""
+    out = (ChiSite * len(rows))()
+    for s, (name, spc, sym, sig, eps, q, xyz, dens) in zip(out, rows):
+        s.name, s.species, s.sym = name.encode()[:7], spc, sym
+        s.sig, s.eps, s.q, s.dens = sig, eps, q, dens
+        s.xyz[:] = list(xyz)
+    return out
+
+
+def chi_solve(sites, ntab: int = 16384, dr: float = 0.01, temper: float = 298.15, closure: int = KH,
+              tol: float = 1e-10, maxstep: int = 10000, nhist: int = 10, mix: float = 0.3, chargeup: int = 4):
+    """chi_vv(k) of a solvent model by the library's 1D XRISM solve
+    (rism_chi_solve): returns (Solvent, info dict with steps, rms, xt, status)."""
+    L = lib()
+    o = ChiOpts()
+    L.rism_chi_default_opts(C.byref(o))
+    o.ntab, o.dr, o.temper, o.closure, o.tol = ntab, dr, temper, closure, tol
+    o.maxstep, o.nhist, o.mix, o.chargeup = maxstep, nhist, mix, chargeup
+    V = L.rism_chi_nreduced(sites, len(sites))
+    chi = np.zeros((ntab, max(V, 1), max(V, 1)))
+    xt, rms, steps = C.c_double(), C.c_double(), C.c_int()
+    st = _check(L.rism_chi_solve(sites, len(sites), C.byref(o), _dptr(chi), C.byref(xt), C.byref(steps),
+                                 C.byref(rms)), "rism_chi_solve")
+    if st == SOLVE_DIVERGED:
+        raise RuntimeError(f"rism_chi_solve diverged: {last_error()}")
+    rho, q, sig, eps = (np.zeros(V) for _ in range(4))
+    for s in sites:  # solvent_read.cc:71-80
+        v = abs(s.sym) - 1
+        sig[v], eps[v], q[v] = s.sig, s.eps, s.q
+        rho[v] += s.dens * 6.02214076e23 * 1e-27
+    sv = Solvent(rho, q, sig, eps, chi, np.pi / (dr * ntab), temper, xt.value)
+    return sv, {"status": st, "steps": steps.value, "rms": rms.value, "xt": xt.value}
+
+
+def write_xvk(path: str, sites, solvent: Solvent, dr: float):
+    """Write `solvent` (from chi_solve on `sites`) as an .xvk file."""
+    chi = np.ascontiguousarray(solvent.chi, dtype=np.float64)
+    _check(lib().rism_write_xvk(path.encode(), sites, len(sites), chi.shape[0], dr, solvent.temper, solvent.xt,
+                                _dptr(chi)), "rism_write_xvk")
 
 
 class Solver:

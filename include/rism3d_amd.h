@@ -218,6 +218,54 @@ int rism_batch_solve(int n, const rism_grid *grids, const rism_solvent *solvent,
                      int nworkers, int *status, int *steps, double *rms, rism_xmu *xmu,
                      double *seconds);
 
+/* ---- solvent susceptibility chi_vv(k): the table an .xvk file carries ---- */
+/* The reference reads this table (solvent_read.cc:10-86) and never computes
+   it.  These entry points compute it with a 1D XRISM solve (KH or HNC) on
+   the DST-I grid r_j = j dr, k_i = i pi/(ntab dr), the scheme that reproduces
+   the reference's own example .xvk files, and write it in that format.  They
+   run on the host (a 4096-16384 point problem, once per solvent). */
+typedef struct rism_chi_site {  /* one .xvk site row (solvent_read.cc:42-47) */
+  char name[8];
+  int species;    /* molecule index, 1-based: sites of one species form one rigid molecule */
+  int sym;        /* reduced-site index, 1-based; negative = equivalent to |sym| */
+  double sig;     /* LJ sigma, Angstrom */
+  double eps;     /* LJ epsilon, J/mol */
+  double q;       /* charge, e */
+  double xyz[3];  /* position in the molecule, Angstrom */
+  double dens;    /* concentration of the species, mol/L */
+} rism_chi_site;
+
+typedef struct rism_chi_opts {
+  int ntab;       /* table length: a power of two (the examples use 4096 and 16384) */
+  double dr;      /* r spacing, Angstrom */
+  double temper;  /* K */
+  int closure;    /* RISM_CLOSURE_KH / _HNC */
+  double tol;     /* rms residual of t per charge-up stage */
+  int maxstep;    /* per stage */
+  int nhist;      /* Anderson (DIIS form) vectors */
+  double mix;     /* mixing of the residual */
+  int chargeup;   /* stages n: Coulomb scaled by 0, 1/n, .., 1 (0: one full-charge stage) */
+} rism_chi_opts;
+
+/* ntab 16384, dr 0.01, 298.15 K, KH, tol 1e-10, maxstep 10000, 10 vectors,
+   mix 0.3, 4 charge-up stages */
+void rism_chi_default_opts(rism_chi_opts *o);
+/* number of reduced sites V = max |sym| */
+int rism_chi_nreduced(const rism_chi_site *sites, int nsite);
+/* chi[ntab][V][V] in the reader's order (h_v = sum_v' chi[v][v'] c_v'),
+   xt (1/Pa), the total step count and the last rms (any output may be NULL).
+   Returns RISM_SOLVE_CONVERGED, RISM_SOLVE_MAXSTEP (tables of the last
+   iterate), RISM_SOLVE_DIVERGED or RISM_ERR_*. */
+int rism_chi_solve(const rism_chi_site *sites, int nsite, const rism_chi_opts *opts, double *chi,
+                   double *xt, int *steps, double *rms);
+/* .xvk writer: the header lines and table layout solvent_read.cc:31-58 reads */
+int rism_write_xvk(const char *path, const rism_chi_site *sites, int nsite, int ntab, double dr,
+                   double temper, double xt, const double *chi);
+/* the site rows, ntab, dr and temperature of an .xvk header, as a model to
+   solve again (solvent_read.cc:31-48) */
+int rism_xvk_read_sites(const char *path, rism_chi_site *sites, int maxsite, int *nsite, int *ntab,
+                        double *dr, double *temper);
+
 /* ---- measurement (no counterpart in the reference, which has no timers) */
 /* kernel ids for rism_kernel_times */
 #define RISM_K_CLOSURE_FX 0   /* Anderson update + closure + forward x FFT */
