@@ -500,7 +500,7 @@ extern "C" int rism_write_xvk(const char *path, const rism_chi_site *sites, int 
     const rism_chi_site &s = sites[a];
     char name[16];
     std::snprintf(name, sizeof name, "%.8s", s.name[0] ? s.name : "X");
-    std::fprintf(f, "## %-4s %4d %3d %11.5f %11.5f %11.5f %11.5f %11.5f %11.5f %11.5f\n", name, s.species, s.sym,
+    std::fprintf(f, "## %-4s %4d %3d %11.5f %11.5f %11.5f %11.5f %11.5f %11.5f %14.8f\n", name, s.species, s.sym,
                  s.sig, s.eps, s.q, s.xyz[0], s.xyz[1], s.xyz[2], s.dens);
   }
   std::fprintf(f, "##\n");

@@ -7,7 +7,9 @@
 // file, -f no centring, -z zero solute charges (writes .xmu0).  Added flags:
 // -p 0|1 FP64 (default) or mixed precision, -H Anderson history slots
 // (default 2, the reference's).  -r (RMDFT) and -e (ESP cube) are not part
-// of this build and are rejected with a message.
+// of this build and are rejected with a message.  The outlist letters of the
+// deck select the outputs as in output.cu:6-62: m .xmu, d .gra, c .cuv,
+// g .guv, h .huv, e .euv, q .qv.
 #include <unistd.h>
 
 #include <algorithm>
@@ -119,9 +121,48 @@ int main(int argc, char **argv) {
     }
     std::printf("done.\n");
   }
-  for (char c : std::string("cgheq"))
-    if (outlist.find(c) != std::string::npos)
-      std::printf("note: field writer '%c' is not part of this build\n", c);
+  // field writers in output.cu:38-61 order: c, g, h, e, q
+  const size_t N = (size_t)g.n[0] * g.n[1] * g.n[2];
+  auto fail = [&]() {
+    std::printf("%s\n", rism_last_error());
+    return 1;
+  };
+  struct Writer {
+    char key;
+    const char *label, *ext;
+    int field;
+  };
+  const Writer tw[] = {{'c', "Cuv", ".cuv", RISM_FIELD_C}, {'g', "Guv", ".guv", RISM_FIELD_G},
+                       {'h', "Huv", ".huv", RISM_FIELD_H}};
+  for (const Writer &w : tw) {
+    if (outlist.find(w.key) == std::string::npos) continue;
+    const std::string path = fname + w.ext;
+    std::printf("outputting %s to file:  %s  ...\n", w.label, path.c_str());
+    std::vector<double> f(N * sv.nsite);
+    if (rism_get_field(h, w.field, f.data()) != RISM_OK) return fail();
+    const int rc2 = w.key == 'g'
+                        ? rism_write_guv(path.c_str(), sv.nsite, g.n, g.box, centering ? deck->shift : nullptr, f.data())
+                        : rism_write_huv(path.c_str(), g.n, g.box, su.natom, su.q, su.sig, su.eps, su.xyz, sv.nsite,
+                                         f.data());
+    if (rc2 != RISM_OK) return fail();
+    std::printf("done.\n");
+  }
+  if (outlist.find('e') != std::string::npos) {
+    const std::string path = fname + ".euv";
+    std::printf("outputting euv to file:  %s  ...\n", path.c_str());
+    std::vector<double> e((size_t)2 * sv.nsite * su.natom);
+    if (rism_euv(h, e.data()) != RISM_OK || rism_write_euv(path.c_str(), sv.nsite, su.natom, e.data()) != RISM_OK)
+      return fail();
+    std::printf("done.\n");
+  }
+  if (outlist.find('q') != std::string::npos) {
+    const std::string path = fname + ".qv";
+    std::printf("outputting qv to file:  %s  ...\n", path.c_str());
+    std::vector<double> qv(N);
+    if (rism_qv(h, qv.data()) != RISM_OK || rism_write_qv(path.c_str(), g.n, g.box, qv.data()) != RISM_OK)
+      return fail();
+    std::printf("done.\n");
+  }
   rism_destroy(h);
   rism_deck_free(deck);
   return 0;

@@ -365,3 +365,129 @@ extern "C" int rism_write_gra(const double *g, int natom, const char *path) {
   std::fclose(f);
   return RISM_OK;
 }
+
+// ---- field writers (output_guv.cu, output_huv.cu, output_cuv.cu,
+// output_euv.cu, output_qv.cu).  Text values use the reference's stream
+// formats: the default (%g, 6 significant digits) for .huv/.cuv, setw(16)
+// scientific 8 digits for .euv, fixed 12.4 + scientific 16.8 for .qv.
+
+namespace {
+
+struct FileCloser {
+  FILE *f;
+  ~FileCloser() {
+    if (f) std::fclose(f);
+  }
+};
+
+int finish(FILE *&f, const char *path) {
+  const bool bad = std::ferror(f) != 0;
+  const int rc = std::fclose(f);
+  f = nullptr;
+  if (bad || rc != 0) {
+    set_error(std::string("write error on ") + path);
+    return RISM_ERR_IO;
+  }
+  return RISM_OK;
+}
+
+}  // namespace
+
+extern "C" int rism_write_guv(const char *path, int natv, const int *n, const double *box, const double *shift,
+                              const double *g) {
+  // output_guv.cu:11-21: int natv, int grid[3], double box[3], double shift[3],
+  // then g [natv][N] as doubles, native byte order
+  if (!path || !n || !box || !g || natv < 1) {
+    set_error("rism_write_guv: bad arguments");
+    return RISM_ERR_ARG;
+  }
+  FileCloser fc{std::fopen(path, "wb")};
+  if (!fc.f) {
+    set_error(std::string("cannot write ") + path);
+    return RISM_ERR_IO;
+  }
+  const double zero[3] = {0.0, 0.0, 0.0};
+  const size_t N = (size_t)n[0] * n[1] * n[2];
+  std::fwrite(&natv, sizeof(int), 1, fc.f);
+  std::fwrite(n, sizeof(int), 3, fc.f);
+  std::fwrite(box, sizeof(double), 3, fc.f);
+  std::fwrite(shift ? shift : zero, sizeof(double), 3, fc.f);
+  std::fwrite(g, sizeof(double), N * natv, fc.f);
+  return finish(fc.f, path);
+}
+
+extern "C" int rism_write_huv(const char *path, const int *n, const double *box, int natom, const double *q,
+                              const double *sig, const double *eps, const double *xyz, int natv,
+                              const double *f) {
+  // output_huv.cu:9-42 (and output_cuv.cu for c = h - t): box, grid, the
+  // solute rows q sig eps x y z, natv, then one row of natv values per grid
+  // point.  The reference prints r[3 iu .. 3 iu + 2] of its stride-4 array,
+  // which are not atom iu's coordinates for iu > 0; the coordinates are
+  // written here.
+  if (!path || !n || !box || !f || natv < 1 || natom < 0 || (natom > 0 && (!q || !sig || !eps || !xyz))) {
+    set_error("rism_write_huv: bad arguments");
+    return RISM_ERR_ARG;
+  }
+  FileCloser fc{std::fopen(path, "w")};
+  if (!fc.f) {
+    set_error(std::string("cannot write ") + path);
+    return RISM_ERR_IO;
+  }
+  std::fprintf(fc.f, "%g %g %g \n%d %d %d \n%d\n", box[0], box[1], box[2], n[0], n[1], n[2], natom);
+  for (int u = 0; u < natom; ++u)
+    std::fprintf(fc.f, "%g %g %g %g %g %g\n", q[u], sig[u], eps[u], xyz[3 * u], xyz[3 * u + 1], xyz[3 * u + 2]);
+  std::fprintf(fc.f, "%d\n", natv);
+  const size_t N = (size_t)n[0] * n[1] * n[2];
+  std::string line;
+  char buf[32];
+  for (size_t ip = 0; ip < N; ++ip) {
+    line.clear();
+    for (int v = 0; v < natv; ++v) {
+      std::snprintf(buf, sizeof buf, "%g ", f[(size_t)v * N + ip]);
+      line += buf;
+    }
+    line += '\n';
+    std::fputs(line.c_str(), fc.f);
+  }
+  return finish(fc.f, path);
+}
+
+extern "C" int rism_write_euv(const char *path, int natv, int natom, const double *euv) {
+  // output_euv.cu:14-27: per atom, the LJ energy with each site, then the
+  // Coulomb energy with each site; euv[(kind * natv + v) * natom + u], J/mol
+  if (!path || !euv || natv < 1 || natom < 0) {
+    set_error("rism_write_euv: bad arguments");
+    return RISM_ERR_ARG;
+  }
+  FileCloser fc{std::fopen(path, "w")};
+  if (!fc.f) {
+    set_error(std::string("cannot write ") + path);
+    return RISM_ERR_IO;
+  }
+  for (int u = 0; u < natom; ++u) {
+    for (int c = 0; c < 2 * natv; ++c) std::fprintf(fc.f, "%16.8e ", euv[(size_t)c * natom + u]);
+    std::fputc('\n', fc.f);
+  }
+  return finish(fc.f, path);
+}
+
+extern "C" int rism_write_qv(const char *path, const int *n, const double *box, const double *qv) {
+  // output_qv.cu:15-32: x y z (grid-point coordinates) and qv * dv per point
+  if (!path || !n || !box || !qv) {
+    set_error("rism_write_qv: bad arguments");
+    return RISM_ERR_ARG;
+  }
+  FileCloser fc{std::fopen(path, "w")};
+  if (!fc.f) {
+    set_error(std::string("cannot write ") + path);
+    return RISM_ERR_IO;
+  }
+  const double dx = box[0] / n[0], dy = box[1] / n[1], dz = box[2] / n[2], dv = dx * dy * dz;
+  const size_t N = (size_t)n[0] * n[1] * n[2];
+  for (size_t ip = 0; ip < N; ++ip) {
+    const int x = (int)(ip % n[0]), y = (int)((ip / n[0]) % n[1]), z = (int)(ip / ((size_t)n[0] * n[1]));
+    std::fprintf(fc.f, "    %12.4f%12.4f%12.4f  %16.8e\n", (x - n[0] / 2) * dx, (y - n[1] / 2) * dy,
+                 (z - n[2] / 2) * dz, qv[ip] * dv);
+  }
+  return finish(fc.f, path);
+}

@@ -193,6 +193,101 @@ __global__ __launch_bounds__(kThreads) void k_grad_sum(const double *partials, i
   out[id] = s * scale;
 }
 
+// cal_euv per (atom, site), as output_euv.cu:14-27 writes it: a workgroup owns
+// a chunk of points and walks the atoms, writing per-(chunk, atom) partials
+// [nblk][natu][2V] (LJ for each site, then ES for each site) that
+// k_euv_sum adds in chunk order.
+__global__ __launch_bounds__(kThreads) void k_euv_partial(int nx, int ny, int nz, double drx, double dry,
+                                                          double drz, int V, int natu, const double4 *atoms,
+                                                          const double2 *lj, const double *sigh,
+                                                          const double *qv, const double *rho,
+                                                          const double *g, double *partials) {
+  const long N = (long)nx * ny * nz;
+  const double cc = 4.359744722207185e-18 * 0.5291771500001 * 6.02214076e23;
+  const long cbase = (long)blockIdx.x * kThreads * kGradPPT;
+  __shared__ double red[kThreads / 64][2 * kMaxV];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int u = 0; u < natu; ++u) {
+    const double4 at = atoms[u];
+    double acc[2 * kMaxV];
+#pragma unroll
+    for (int c = 0; c < 2 * kMaxV; ++c) acc[c] = 0.0;
+    for (int k = 0; k < kGradPPT; ++k) {
+      const long ip = cbase + threadIdx.x + (long)k * kThreads;
+      if (ip >= N) break;
+      const int x = (int)(ip % nx), y = (int)((ip / nx) % ny), z = (int)(ip / ((long)nx * ny));
+      const double dx = (x - nx / 2) * drx - at.x;
+      const double dy = (y - ny / 2) * dry - at.y;
+      const double dz = (z - nz / 2) * drz - at.z;
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      const double ir1 = rsqrt_f64(fmax(r2, 1.0e-300));
+      const double r1 = r2 * ir1, ir2 = ir1 * ir1;
+#pragma unroll
+      for (int v = 0; v < kMaxV; ++v) {
+        if (v < V && r1 >= sigh[v * natu + u]) {  // r < sigma_uv / 2 excluded (cal_euv.cu:61-64)
+          const double2 p = lj[v * natu + u];
+          const double w = rho[v] * g[(long)v * N + ip];
+          const double s2 = p.x * ir2, s6 = s2 * s2 * s2;
+          acc[v] += w * p.y * s6 * (s6 - 1.0);
+          acc[kMaxV + v] += w * at.w * qv[v] * cc * ir1;
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 2 * kMaxV; ++c) {
+      const double s = wave_sum_t(acc[c]);
+      if (lane == 0) red[wave][c] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * V) {
+      const int c = threadIdx.x < V ? threadIdx.x : kMaxV + threadIdx.x - V;
+      double s = 0.0;
+      for (int w = 0; w < kThreads / 64; ++w) s += red[w][c];
+      partials[((long)blockIdx.x * natu + u) * 2 * V + threadIdx.x] = s;
+    }
+    __syncthreads();
+  }
+}
+
+// out[LJ/ES][v][u] = dv * sum over chunks (the euv array of cal_euv.cu:7-33, x dv)
+__global__ __launch_bounds__(kThreads) void k_euv_sum(const double *partials, int nblk, int natu, int V,
+                                                      double dv, double *out) {
+  const long id = (long)blockIdx.x * kThreads + threadIdx.x;  // (u, c) with c = kind * V + v
+  if (id >= (long)natu * 2 * V) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += partials[(long)b * natu * 2 * V + id];
+  const int u = (int)(id / (2 * V)), c = (int)(id % (2 * V));
+  out[(long)c * natu + u] = s * dv;
+}
+
+// qv = sum_v rho_v q_v g_v (cal_qv.cu:20-27)
+__global__ __launch_bounds__(kThreads) void k_qv(long N, int V, const double *rho, const double *qv,
+                                                 const double *g, double *out) {
+  const long ip = (long)blockIdx.x * kThreads + threadIdx.x;
+  if (ip >= N) return;
+  double s = 0.0;
+  for (int v = 0; v < V; ++v) s += rho[v] * qv[v] * g[(long)v * N + ip];
+  out[ip] = s;
+}
+
+hipError_t launch_euv_atoms(int nx, int ny, int nz, const double *dr, int V, int natu, const double4 *atoms,
+                            const double2 *lj, const double *sigh, const double *qv, const double *rho,
+                            const double *g, double *partials, int nblk, double *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_euv_partial, dim3(nblk), dim3(kThreads), 0, st, nx, ny, nz, dr[0], dr[1], dr[2], V, natu,
+                     atoms, lj, sigh, qv, rho, g, partials);
+  const double dv = dr[0] * dr[1] * dr[2];
+  hipLaunchKernelGGL(k_euv_sum, dim3((unsigned)((natu * 2 * V + kThreads - 1) / kThreads)), dim3(kThreads), 0, st,
+                     partials, nblk, natu, V, dv, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_qv(long N, int V, const double *rho, const double *qv, const double *g, double *out,
+                     hipStream_t st) {
+  hipLaunchKernelGGL(k_qv, dim3((unsigned)((N + kThreads - 1) / kThreads)), dim3(kThreads), 0, st, N, V, rho, qv,
+                     g, out);
+  return hipGetLastError();
+}
+
 template <typename R, typename S>
 hipError_t launch_gfield(const IterArgs<R, S> &a, double *g, hipStream_t st) {
   hipLaunchKernelGGL((k_gfield<R, S>), dim3((unsigned)((a.N + kThreads - 1) / kThreads)), dim3(kThreads),

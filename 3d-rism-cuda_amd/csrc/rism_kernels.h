@@ -208,6 +208,11 @@ hipError_t launch_exchem(const IterArgs<R, S> &a, const double *frd, double *par
 hipError_t launch_euv(int nx, int ny, int nz, const double *dr, int V, int natu, const double4 *atoms,
                       const double2 *lj, const double *sigh, const double *qv, const double *rho,
                       const double *g, double *partials, int nblk, hipStream_t st);
+hipError_t launch_euv_atoms(int nx, int ny, int nz, const double *dr, int V, int natu, const double4 *atoms,
+                            const double2 *lj, const double *sigh, const double *qv, const double *rho,
+                            const double *g, double *partials, int nblk, double *out, hipStream_t st);
+hipError_t launch_qv(long N, int V, const double *rho, const double *qv, const double *g, double *out,
+                     hipStream_t st);
 hipError_t launch_grad(int nx, int ny, int nz, const double *dr, int V, int natu,
                        const double4 *atoms, const double2 *lj, const double *sigh,
                        const double *qv, const double *rho, const double *g, double *partials,

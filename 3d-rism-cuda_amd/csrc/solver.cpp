@@ -1117,6 +1117,63 @@ int rism_grad(rism_handle *h, double *out) {
   return RISM_OK;
 }
 
+int rism_euv(rism_handle *h, double *out) {
+  if (!h || !out) { set_error("null argument"); return RISM_ERR_ARG; }
+  if (!h->solved) { set_error("rism_solve must run before rism_euv"); return RISM_ERR_STATE; }
+  HIP_OK(hipSetDevice(h->device));
+  if (!h->evaluated)
+    if (int rc = evaluate(h)) return rc;
+  const int nu = h->natu, V = h->V;
+  if (nu == 0) return RISM_OK;
+  const int nblk = (int)((h->N + (long)kThreads * 16 - 1) / ((long)kThreads * 16));
+  double *part = nullptr, *dout = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_device_mu);
+    HIP_OK(hipMalloc((void **)&part, (size_t)nblk * nu * 2 * V * sizeof(double)));
+    HIP_OK(hipMalloc((void **)&dout, (size_t)nu * 2 * V * sizeof(double)));
+  }
+  hipError_t e = launch_euv_atoms(h->nx, h->ny, h->nz, h->dr, V, nu, h->atoms, h->lj, h->sigh, h->d_qv, h->d_rho,
+                                  h->gfield, part, nblk, dout, h->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out, dout, (size_t)nu * 2 * V * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  {
+    std::lock_guard<std::mutex> lk(g_device_mu);
+    (void)hipFree(part);
+    (void)hipFree(dout);
+  }
+  if (e != hipSuccess) {
+    set_error(std::string("HIP error in rism_euv: ") + hipGetErrorString(e));
+    return RISM_ERR_HIP;
+  }
+  return RISM_OK;
+}
+
+int rism_qv(rism_handle *h, double *out) {
+  if (!h || !out) { set_error("null argument"); return RISM_ERR_ARG; }
+  if (!h->solved) { set_error("rism_solve must run before rism_qv"); return RISM_ERR_STATE; }
+  HIP_OK(hipSetDevice(h->device));
+  if (!h->evaluated)
+    if (int rc = evaluate(h)) return rc;
+  double *d = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_device_mu);
+    HIP_OK(hipMalloc((void **)&d, (size_t)h->N * sizeof(double)));
+  }
+  hipError_t e = launch_qv(h->N, h->V, h->d_rho, h->d_qv, h->gfield, d, h->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d, (size_t)h->N * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  {
+    std::lock_guard<std::mutex> lk(g_device_mu);
+    (void)hipFree(d);
+  }
+  if (e != hipSuccess) {
+    set_error(std::string("HIP error in rism_qv: ") + hipGetErrorString(e));
+    return RISM_ERR_HIP;
+  }
+  return RISM_OK;
+}
+
 // Adjoint of the converged fixed point (FP64 in both precisions); see
 // k_adjoint.hip for the equations.  The adjoint system
 //   lambda = (g' - 1)(K^T lambda - lambda) + b

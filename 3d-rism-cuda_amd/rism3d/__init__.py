@@ -37,7 +37,8 @@ EXPORTED = [
     "rism_write_xmu", "rism_write_gra", "rism_stats", "rism_rms_history", "rism_destroy",
     "rism_fft3d", "rism_calculate", "rism_profile", "rism_kernel_times", "rism_geometry",
     "rism_adjoint_grad", "rism_batch_solve", "rism_chi_default_opts", "rism_chi_nreduced",
-    "rism_chi_solve", "rism_write_xvk", "rism_xvk_read_sites",
+    "rism_chi_solve", "rism_write_xvk", "rism_xvk_read_sites", "rism_stream_copy",
+    "rism_euv", "rism_qv", "rism_write_guv", "rism_write_huv", "rism_write_euv", "rism_write_qv",
 ]
 
 KERNELS = ["closure_fx", "fft_y_fwd", "fft_z_oz", "fft_y_inv", "resid", "solve", "potentials", "fk",
@@ -156,6 +157,14 @@ def lib():
         L.rism_profile.argtypes = [C.c_void_p, C.c_int]
         L.rism_kernel_times.argtypes = [C.c_void_p, dp, P(C.c_long)]
         L.rism_geometry.argtypes = [C.c_void_p, P(C.c_int)]
+        L.rism_stream_copy.argtypes = [C.c_int, C.c_size_t, C.c_int, dp, dp]
+        ip = P(C.c_int)
+        L.rism_euv.argtypes = [C.c_void_p, dp]
+        L.rism_qv.argtypes = [C.c_void_p, dp]
+        L.rism_write_guv.argtypes = [C.c_char_p, C.c_int, ip, dp, dp, dp]
+        L.rism_write_huv.argtypes = [C.c_char_p, ip, dp, C.c_int, dp, dp, dp, dp, C.c_int, dp]
+        L.rism_write_euv.argtypes = [C.c_char_p, C.c_int, C.c_int, dp]
+        L.rism_write_qv.argtypes = [C.c_char_p, ip, dp, dp]
         L.rism_chi_default_opts.argtypes = [P(ChiOpts)]
         L.rism_chi_default_opts.restype = None
         L.rism_chi_nreduced.argtypes = [P(ChiSite), C.c_int]
@@ -275,6 +284,13 @@ def read_xvk(path: str) -> Solvent:
             p = getattr(d, name)
             if p:
                 libc.free(C.cast(p, C.c_void_p))
+
+
+def stream_copy(bytes_: int = 1 << 30, reps: int = 20, device: int = 0):
+    """(GB/s, us per copy) of the library's plain device copy (rism_stream_copy)."""
+    g, u = C.c_double(), C.c_double()
+    _check(lib().rism_stream_copy(device, bytes_, reps, C.byref(g), C.byref(u)), "rism_stream_copy")
+    return g.value, u.value
 
 
 def solvent_model(path: str):
@@ -403,6 +419,19 @@ class Solver:
         if rc == SOLVE_DIVERGED:
             raise RuntimeError(f"rism_adjoint_grad: adjoint solve diverged: {last_error()}")
         return {"xyz": dxyz.reshape(n, 3), "q": dq, "sig": dsig, "eps": deps, "steps": steps.value}
+
+    def euv(self) -> np.ndarray:
+        """per-(atom, site) solute-solvent energy x dv, [2 (LJ, ES)][V][natom] J/mol (rism_euv)"""
+        V, n = self.V, self.natom
+        out = np.zeros(2 * V * n)
+        _check(lib().rism_euv(self.h, _dptr(out)), "rism_euv")
+        return out.reshape(2, V, n)
+
+    def qv(self) -> np.ndarray:
+        """solvent charge density sum_v rho_v q_v g_v, [N] (rism_qv)"""
+        out = np.zeros(self.N)
+        _check(lib().rism_qv(self.h, _dptr(out)), "rism_qv")
+        return out
 
     def field(self, which: int) -> np.ndarray:
         if which == FIELD_FK:

@@ -17,6 +17,15 @@ with the reference's integer-rounded shift (src-cuda/solute.cu:13-43).
       0.5 A grids sized to the solute extent plus a 12 A margin on each
       side, rounded up to a 2,3,5-smooth multiple of 4 within 48..96.
   c4  512^3 / 256 A box, 5000 atoms, net charge -12.
+
+Solvents: chi_vv(k) comes from the library's own 1D XRISM-KH solve
+(rism_chi_solve, 16384 points at dr 0.01 A, 298.15 K):
+  water  3-site rigid water: O sigma 3.1507 A, eps 0.1521 kcal/mol,
+         q -0.834; H sigma 0.4 A, eps 0.046 kcal/mol, q +0.417; OH 0.9572 A,
+         HOH 104.52 deg; rho 0.03334 1/A^3
+  nacl   the same water plus Na+ (sigma 2.35 A, eps 0.13 kcal/mol, q +1) and
+         Cl- (sigma 4.40 A, eps 0.10 kcal/mol, q -1) at 0.15 M
+Generated tables are cached as .xvk files next to the built library.
 """
 from __future__ import annotations
 
@@ -25,7 +34,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import Solute
+from . import LIB_PATH, Solute, chi_solve, make_sites, read_xvk, write_xvk
 
 SEED0 = 20220301
 KCAL = 4184.0
@@ -38,6 +47,51 @@ class Workload:
     box: tuple
     solute: Solute
     nhist: int = 2
+
+
+WATER_RHO = 0.03334  # 1/A^3
+AVOGADRO_PER_A3 = 6.02214076e23 * 1e-27  # mol/L -> 1/A^3 (solvent_read.cc:78-80)
+
+
+def water_rows():
+    """Site rows of the 3-site water model (species 1)."""
+    half = np.radians(104.52) / 2
+    x, y = 0.9572 * np.sin(half), 0.9572 * np.cos(half)
+    conc = WATER_RHO / AVOGADRO_PER_A3
+    return [("O", 1, 1, 3.1507, 0.1521 * KCAL, -0.834, (0.0, 0.0, 0.0), conc),
+            ("H", 1, 2, 0.4, 0.046 * KCAL, 0.417, (x, y, 0.0), conc),
+            ("H", 1, -2, 0.4, 0.046 * KCAL, 0.417, (-x, y, 0.0), conc)]
+
+
+def solvent_rows(kind: str):
+    if kind == "water":
+        return water_rows()
+    if kind == "nacl":
+        return water_rows() + [("Na", 2, 3, 2.35, 0.13 * KCAL, 1.0, (0.0, 0.0, 0.0), 0.15),
+                               ("Cl", 3, 4, 4.40, 0.10 * KCAL, -1.0, (0.0, 0.0, 0.0), 0.15)]
+    raise ValueError(f"unknown solvent {kind!r}")
+
+
+def solvent_file(kind: str = "water", ntab: int = 16384, dr: float = 0.01, temper: float = 298.15) -> str:
+    """Path of the .xvk of `kind`, computed by the library's 1D solve on first
+    use and cached beside the library."""
+    d = os.path.join(os.path.dirname(LIB_PATH), "solvents")
+    path = os.path.join(d, f"{kind}_{ntab}_{dr:g}_{temper:g}.xvk")
+    if not os.path.exists(path):
+        os.makedirs(d, exist_ok=True)
+        sites = make_sites(solvent_rows(kind))
+        sv, info = chi_solve(sites, ntab, dr, temper)
+        if info["status"] != 0:
+            raise RuntimeError(f"1D RISM for {kind} did not converge: {info}")
+        tmp = f"{path}.{os.getpid()}.tmp"
+        write_xvk(tmp, sites, sv, dr)
+        os.replace(tmp, path)
+    return path
+
+
+def solvent(kind: str = "water", **kw):
+    """The Solvent of `kind` (see solvent_file)."""
+    return read_xvk(solvent_file(kind, **kw))
 
 
 def centre(xyz: np.ndarray) -> np.ndarray:

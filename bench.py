@@ -15,8 +15,9 @@ The JSON line also carries:
   cpu_baseline  the FP64 host oracle (OpenMP) on a bounded sample of the same
                 workload: potentials plus a few iterations, scaled to the
                 FP64 step count of a full solve.
-Solvent: the reference's example_rmdft/h2o_vv.xvk (16384-point TIP3P-like
-water chi, 298.15 K) committed under tests/golden.
+Solvent: 3-site water, chi_vv(k) from the library's own 1D XRISM-KH solve
+(16384 points, dr 0.01 A, 298.15 K; rism3d.configs.solvent_file), computed
+before any timing and cached.
 """
 import argparse
 import json
@@ -34,9 +35,8 @@ import numpy as np  # noqa: E402
 import torch  # noqa: E402,F401  (first: its HIP runtime then serves the solver library too)
 
 METRIC = "s-to-converge 256^3 KH, mixed vs FP64 at Δμ≤0.01 kcal/mol; % HBM roofline"
-SOLVENT = os.path.join(REPO, "tests", "golden", "reference_examples", "h2o_vv_16384.xvk")
 HBM_PEAK_GBS = 8000.0
-COPY_BYTES = 256 << 20  # calibration copy: reads and writes this many bytes
+COPY_BYTES = 1 << 30  # calibration copy: reads and writes this many bytes
 TRAFFIC = os.path.join(REPO, "profiles", "iteration_kernels_256cubed_mixed.json")
 
 
@@ -134,7 +134,7 @@ def cpu_baseline(w, fp64_iterations, iters=8, sdiv=4):
         return None
     d = tempfile.mkdtemp()
     try:
-        shutil.copy(SOLVENT, os.path.join(d, "h2o_vv.xvk"))
+        shutil.copy(configs.solvent_file("water"), os.path.join(d, "h2o_vv.xvk"))
         configs.write_inp(os.path.join(d, "c1.inp"), w, "h2o_vv.xvk")
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         r = subprocess.run([orc.EXE, "-q", "-j", "-f", "-n", str(iters), "-x", str(sdiv), "-T", str(threads),
@@ -159,7 +159,7 @@ def side_workload(which, rism3d, configs):
     """The other two measured configurations (not the headline line)."""
     import numpy as np
     if which == "batch":
-        sv = rism3d.read_xvk(os.path.join(REPO, "tests", "golden", "reference_examples", "h2o_vv.xvk"))
+        sv = configs.solvent("water")
         systems = [(w.n, w.box, w.solute) for w in configs.batch_config(256)]
         rism3d.batch_solve(systems[:8], sv, rism3d.Options(), workers=4)  # warm-up
         r64, s64 = rism3d.batch_solve(systems, sv, rism3d.Options(), workers=8)
@@ -169,11 +169,11 @@ def side_workload(which, rism3d, configs):
                 "value": round(len(systems) / smx, 2), "unit": "solutes/s", "higher_is_better": True,
                 "fp64_solutes_per_s": round(len(systems) / s64, 2), "workers": 8,
                 "converged": sum(r["status"] == 0 for r in rmx), "max_dmu_err_vs_fp64_J_mol": round(err, 4)}
-    sv = rism3d.read_xvk(os.path.join(REPO, "tests", "golden", "reference_examples", "naclaq_vv.xvk"))
+    sv = configs.solvent("nacl")
     w = configs.config(4)
     out = {"metric": "s-to-converge 512^3 / 256 A / 5000-atom KH job (potentials + solve + thermo)",
            "unit": "s", "higher_is_better": False,
-           "solvent": "reference example_rmdft/naclaq_vv.xvk (water + 0.1 M NaCl, 4 reduced sites)"}
+           "solvent": "water + 0.15 M NaCl (4 reduced sites), chi from the library's 1D XRISM-KH solve"}
     for name, prec in (("mixed", rism3d.MIXED), ("fp64", rism3d.FP64)):
         free0, _ = torch.cuda.mem_get_info()
         s = rism3d.Solver(w.n, w.box, sv, w.solute, rism3d.Options(precision=prec, nhist=w.nhist))
@@ -213,8 +213,8 @@ def main():
                     help="256 (default): the headline 256^3 job; 512: the 512^3 / 5000-atom electrolyte "
                          "job (memory, times, adjoint forces); batch: 256 small solutes (solutes/s)")
     ap.add_argument("--calibrate-copy", action="store_true",
-                    help="first run a plain device copy of 256 MiB three times (the reference kernel "
-                         "for scaling FETCH_SIZE / WRITE_SIZE counters to bytes)")
+                    help="run the 1 GiB streaming copy 3 times instead of 20 (fewer dispatches under a "
+                         "counter-collecting profiler)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -229,15 +229,12 @@ def main():
     if args.workload != "256":
         print(json.dumps(side_workload(args.workload, rism3d, configs)))
         return
-    if args.calibrate_copy:
-        src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+    # the measured HBM ceiling: the library's plain streaming copy of 1 GiB
+    # (its dispatches, rism::k_stream_copy, are also the known-size reference
+    # for scaling the FETCH_SIZE / WRITE_SIZE counters)
+    copy_gbps, copy_us = rism3d.stream_copy(COPY_BYTES, 3 if args.calibrate_copy else 20)
     w = configs.config(1)
-    solvent = rism3d.read_xvk(SOLVENT)
+    solvent = configs.solvent("water")
     V = solvent.nsite
     P = (V + 1) // 2
     N = int(np.prod(w.n))
@@ -253,7 +250,8 @@ def main():
     achieved = bytes_launch / avg_s / 1e9
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
-            "bytes_per_launch": bytes_launch, "avg_launch_us": round(avg_s * 1e6, 2)}
+            "bytes_per_launch": bytes_launch, "avg_launch_us": round(avg_s * 1e6, 2),
+            "copy_GBps": round(copy_gbps, 1), "frac_of_copy": round(achieved / copy_gbps, 4)}
     per_kernel = per_kernel_table(mixed["kernels"], kb, N)
     cpu = None
     if not args.no_cpu and fp64 is not None:
@@ -275,7 +273,7 @@ def main():
         "dtype": "f32 fields / f64 state",
         "data": "synthetic",
         "config": {"workload": w.name, "grid": list(w.n), "box_A": list(w.box), "atoms": len(w.solute.q),
-                   "solvent": "water (reference example_rmdft/h2o_vv.xvk chi)", "closure": "KH",
+                   "solvent": "3-site water, chi from the library's 1D XRISM-KH solve (16384 pts, dr 0.01 A)", "closure": "KH",
                    "tol": 1e-6, "anderson": {"nhist": w.nhist, "count": 2, "m": 0.6, "mp": 0.2},
                    "parallelism": "1 GPU"},
         "iterations_mixed": mixed["iterations"],

@@ -198,6 +198,25 @@ int rism_load_tuv(rism_handle *h, const char *path);
 /* .xmu writer (output_xmu.cu:7-111) and .gra writer (output_grad.cu:7-30) */
 int rism_write_xmu(const rism_xmu *x, int closure, const char *path);
 int rism_write_gra(const double *grad, int natom, const char *path);
+/* cal_euv per (atom, site) (cal_euv.cu:7-76) at the converged state, x dv as
+   output_euv.cu:14-27 writes it: out[(kind * V + v) * natom + u], kind 0 =
+   LJ, 1 = Coulomb, J/mol */
+int rism_euv(rism_handle *h, double *out);
+/* cal_qv (cal_qv.cu:6-27): solvent charge density sum_v rho_v q_v g_v, [N] */
+int rism_qv(rism_handle *h, double *out);
+/* field writers.  .guv binary (output_guv.cu:11-21): int V, int n[3],
+   double box[3], double shift[3] (NULL: zeros), g [V][N] */
+int rism_write_guv(const char *path, int natv, const int *n, const double *box, const double *shift,
+                   const double *g);
+/* .huv / .cuv text (output_huv.cu:9-42, output_cuv.cu:9-42): box, grid, the
+   solute rows q sig eps x y z, V, then one row of V values per grid point
+   (f = h for .huv, c = h - t for .cuv) */
+int rism_write_huv(const char *path, const int *n, const double *box, int natom, const double *q,
+                   const double *sig, const double *eps, const double *xyz, int natv, const double *f);
+/* .euv text (output_euv.cu:14-27) from rism_euv's array */
+int rism_write_euv(const char *path, int natv, int natom, const double *euv);
+/* .qv text (output_qv.cu:15-32) from rism_qv's array */
+int rism_write_qv(const char *path, const int *n, const double *box, const double *qv);
 /* per-solve statistics: steps, final rms, and timings in seconds */
 int rism_stats(rism_handle *h, int *steps, double *rms, double *t_setup,
                double *t_solve, double *t_thermo);
@@ -286,6 +305,11 @@ int rism_xvk_read_sites(const char *path, rism_chi_site *sites, int maxsite, int
 int rism_profile(rism_handle *h, int enable);
 /* accumulated milliseconds and launch counts per kernel id */
 int rism_kernel_times(rism_handle *h, double *ms, long *count);
+/* the HBM ceiling the iteration kernels are compared with: a plain copy of
+   `bytes` (a multiple of 16) between two fresh device buffers, one 16-byte
+   element per thread, timed with HIP events over `reps` launches.  gbps
+   receives (read + written bytes) / time, us the time per copy. */
+int rism_stream_copy(int device, size_t bytes, int reps, double *gbps, double *us);
 /* launch geometry: Lx, Ty, Tz, ept_x, ept_y, ept_z, xgroups, xblocks, packs, history class */
 int rism_geometry(rism_handle *h, int *out /* [10] */);
 

@@ -55,6 +55,7 @@ def lib():
         L.orc_iterate.argtypes = [vp, C.c_int, C.c_int, C.c_char_p, ip, dp]
         L.orc_thermo.argtypes = [vp, C.POINTER(XmuC)]
         L.orc_grad.argtypes = [vp, dp, dp]
+        L.orc_euv.argtypes = [vp, dp]
         L.orc_write_xmu.argtypes = [vp, C.POINTER(XmuC), C.c_char_p]
         L.orc_fft3d.argtypes = [C.c_void_p, ip, dp, C.c_int]
         L.orc_fft1d.argtypes = [C.c_void_p, C.c_int, C.c_int]
@@ -216,6 +217,13 @@ class Oracle:
         a, b = np.zeros(3 * nu), np.zeros(3 * nu)
         lib().orc_grad(self.s, _d(a), _d(b))
         return np.concatenate([a.reshape(nu, 3), b.reshape(nu, 3)], axis=1)
+
+    def euv(self) -> np.ndarray:
+        """per-(atom, site) SE x dv, [2 (LJ, ES)][V][natu] (cal_euv.cu:7-76)"""
+        nu = lib().orc_natu(self.s)
+        out = np.zeros(2 * self.V * nu)
+        lib().orc_euv(self.s, _d(out))
+        return out.reshape(2, self.V, nu)
 
     def field(self, which: int, shape=None) -> np.ndarray:
         N, V = self.N, self.V

@@ -1058,6 +1058,24 @@ void orc_thermo(orc_sys *s, orc_xmu *x) {
 }
 
 /* cal_grad.cu:8-145 with output_grad.cu:15-24 scaling (x dv) */
+/* cal_euv per (atom, site) x dv, as output_euv.cu:14-27 writes it:
+   out[(kind * V + v) * natu + u], kind 0 = LJ, 1 = Coulomb (cal_euv.cu:7-33) */
+void orc_euv(orc_sys *s, double *out) {
+  const int V = s->natv, nu = s->natu;
+  const double dv = s->dr[0] * s->dr[1] * s->dr[2];
+  for (int v = 0; v < V; ++v) {
+#pragma omp parallel for schedule(dynamic)
+    for (int u = 0; u < nu; ++u) {
+      double sg = (s->sigu[u] + s->sigv[v]) * 0.5;
+      double ep = sqrt(s->epsu[u] * s->epsv[v]);
+      double a, b;
+      euv_sum(s, v, u, sg, ep, &a, &b);
+      out[(size_t)v * nu + u] = a * s->rhov[v] * dv;
+      out[(size_t)(V + v) * nu + u] = b * s->rhov[v] * dv;
+    }
+  }
+}
+
 void orc_grad(orc_sys *s, double *dulj, double *due) {
   long N = s->ng;
   int V = s->natv, nu = s->natu;

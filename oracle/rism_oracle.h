@@ -121,7 +121,8 @@ int orc_iterate(orc_sys *s, int cu, int verbose, const char *tuv_in,
                 int *steps, double *rms);
 
 void orc_thermo(orc_sys *s, orc_xmu *x);
-void orc_grad(orc_sys *s, double *dulj, double *due);  /* [natu*3] each */
+void orc_grad(orc_sys *s, double *dulj, double *due);
+void orc_euv(orc_sys *s, double *out);  /* [2][natv][natu], x dv */  /* [natu*3] each */
 int orc_write_xmu(const orc_sys *s, const orc_xmu *x, const char *path);
 int orc_write_tuv(const orc_sys *s, const char *path);
 int orc_read_tuv(orc_sys *s, const char *path);

@@ -152,7 +152,7 @@ def test_xvk_write_read_round_trip(tmp_path):
     rism3d.write_xvk(p, sites, sv, 0.05)
     back = rism3d.read_xvk(p)
     np.testing.assert_allclose(back.chi, sv.chi, rtol=1e-14, atol=1e-300)
-    np.testing.assert_allclose(back.rho, sv.rho, rtol=1e-6)  # densities are written in M with 5 decimals
+    np.testing.assert_allclose(back.rho, sv.rho, rtol=1e-9)  # densities are written in M with 8 decimals
     assert back.xt == pytest.approx(sv.xt, rel=1e-8) and back.temper == pytest.approx(298.15)
     np.testing.assert_allclose(back.q, [-0.834, 0.417, 1.0, -1.0])
     # and the oracle's reader agrees with the header the writer produced

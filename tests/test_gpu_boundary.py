@@ -87,3 +87,72 @@ def test_tuv_restart_both_ways(gpu, deckdir):
     assert st2 == rism3d.SOLVE_CONVERGED and steps2 < steps / 2
     assert abs(s2.thermo()["SFE_SC"] - x0["SFE_SC"]) < 1.0
     s2.close()
+
+
+def test_cli_field_writers_match_oracle(gpu, deckdir):
+    """outlist "mdcgheq": .guv/.huv/.cuv/.euv/.qv (output_guv.cu, output_huv.cu,
+    output_cuv.cu, output_euv.cu, output_qv.cu) against the oracle's fields
+    at the same convergence."""
+    print("cli field writers", flush=True)
+    lines = open("test1.inp").read().split("\n")
+    lines[0] = " mdcgheq    0\t 20220301"
+    open("w1.inp", "w").write("\n".join(lines))
+    r = subprocess.run([CLI, "w1.inp"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    for ext, label in ((".cuv", "Cuv"), (".guv", "Guv"), (".huv", "Huv"), (".euv", "euv"), (".qv", "qv")):
+        assert f"outputting {label} to file:  w1{ext}  ..." in r.stdout
+    o = orc.Oracle.from_deck("w1.inp").read_xvk("h2o_vv.xvk").setup()
+    o.set_iter(1e-6, 10000, 2, 0.6, 0.2)
+    o.iterate()
+    g_ref = o.field(orc.F_G)
+    h_ref = o.field(orc.F_H)
+    c_ref = h_ref - o.field(orc.F_T)   # c = h - t, t the full iterated t
+    deck = rism3d.read_deck("w1.inp")
+    n, box = deck.n, deck.box
+    N = int(np.prod(n))
+    V = deck.solvent.nsite
+    # .guv: int V, int n[3], double box[3], double shift[3], g [V][N]
+    raw = open("w1.guv", "rb").read()
+    hdr = np.frombuffer(raw[:16], dtype=np.int32)
+    assert list(hdr) == [V, *n]
+    dbl = np.frombuffer(raw[16:], dtype=np.float64)
+    np.testing.assert_allclose(dbl[:3], box)
+    np.testing.assert_allclose(dbl[3:6], deck.shift)
+    g = dbl[6:].reshape(V, N)
+    assert np.abs(g - g_ref.reshape(V, N)).max() < 1e-6   # same FP64 trajectory as the oracle
+
+    def text_field(path):
+        rows = open(path).read().split("\n")
+        assert [float(x) for x in rows[0].split()] == list(box)
+        assert [int(x) for x in rows[1].split()] == list(n)
+        na = int(rows[2])
+        atoms = np.array([[float(x) for x in rows[3 + u].split()] for u in range(na)])
+        assert int(rows[3 + na]) == V
+        vals = np.array(" ".join(rows[4 + na:]).split(), float).reshape(N, V).T
+        return atoms, vals
+
+    atoms, h = text_field("w1.huv")
+    np.testing.assert_allclose(atoms[:, 0], deck.solute.q, atol=1e-6)
+    np.testing.assert_allclose(atoms[:, 3:6], deck.solute.xyz, atol=1e-5)
+    hr = h_ref.reshape(V, N)
+    assert np.abs(h - hr).max() < 1e-5 * max(1.0, np.abs(hr).max())   # %g: 6 significant digits
+    _, c = text_field("w1.cuv")
+    cr = c_ref.reshape(V, N)
+    assert np.abs(c - cr).max() < 1e-5 * max(1.0, np.abs(cr).max())
+    # .euv: per atom, LJ with each site then Coulomb with each site, x dv
+    e = np.loadtxt("w1.euv", ndmin=2)
+    eref = o.euv()
+    want = np.concatenate([eref[0].T, eref[1].T], axis=1)
+    np.testing.assert_allclose(e, want, rtol=1e-6, atol=1e-9)
+    x = _xmu("w1.xmu")   # SE_LJ / SE_ES of the .xmu are the sums of the .euv columns
+    assert e[:, :V].sum() == pytest.approx(x["SE_LJ"], rel=1e-5, abs=1e-4)
+    assert e[:, V:].sum() == pytest.approx(x["SE_ES"], rel=1e-5, abs=1e-4)
+    # .qv: grid coordinates and sum_v rho_v q_v g_v dv
+    q = np.loadtxt("w1.qv")
+    assert q.shape == (N, 4)
+    dv = np.prod(np.array(box) / np.array(n))
+    qref = (deck.solvent.rho[:, None] * deck.solvent.q[:, None] * g_ref.reshape(V, N)).sum(0) * dv
+    assert np.abs(q[:, 3] - qref).max() < 1e-6 * max(1e-12, np.abs(qref).max())
+    ix = np.arange(N)
+    np.testing.assert_allclose(q[:, 0], (ix % n[0] - n[0] // 2) * box[0] / n[0], atol=1e-4)
+    np.testing.assert_allclose(q[:, 2], (ix // (n[0] * n[1]) - n[2] // 2) * box[2] / n[2], atol=1e-4)
