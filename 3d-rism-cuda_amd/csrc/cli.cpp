@@ -1,13 +1,15 @@
 // cli.cpp -- rism3d-amd: drop-in for the reference executable 3drism-cuda
 // (src-cuda/main.cu:6-64 with RISM3D::output, output.cu:6-62).
 //
-//   rism3d-amd [-c N] [-d dev] [-i inp] [-s solute] [-f] [-z] [-p prec] [-H n] [inp]
+//   rism3d-amd [-c N] [-d dev] [-i inp] [-s solute] [-e cube] [-f x] [-z] [-p prec] [-H n] [inp]
 //
 // Reference flags: -c charge-up stages, -d device, -i control deck, -s solute
-// file, -f no centring, -z zero solute charges (writes .xmu0).  Added flags:
-// -p 0|1 FP64 (default) or mixed precision, -H Anderson history slots
-// (default 2, the reference's).  -r (RMDFT) and -e (ESP cube) are not part
-// of this build and are rejected with a message.  The outlist letters of the
+// file, -e ESP cube replacing the point-charge potential (it also turns
+// centring off, as main.cu:37-41 falls through to -f), -f <any> no centring
+// (the reference's getopt string "f:" makes -f take an argument), -z zero
+// solute charges (writes .xmu0).  Added flags: -p 0|1 FP64 (default) or mixed
+// precision, -H Anderson history slots (default 2, the reference's).  -r
+// (RMDFT) is not part of this build and is rejected with a message.  The outlist letters of the
 // deck select the outputs as in output.cu:6-62: m .xmu, d .gra, c .cuv,
 // g .guv, h .huv, e .euv, q .qv.
 #include <unistd.h>
@@ -23,17 +25,17 @@
 
 int main(int argc, char **argv) {
   int cu = 0, dn = 0, prec = RISM_PREC_FP64, nhist = 2;
-  std::string input, structure;
+  std::string input, structure, esp;
   bool centering = true, q0 = false;
   int ch;
-  while ((ch = getopt(argc, argv, "c:d:i:s:r:e:fzp:H:")) != -1) {
+  while ((ch = getopt(argc, argv, "c:d:i:s:r:e:f:zp:H:")) != -1) {
     switch (ch) {
       case 'c': cu = std::atoi(optarg); break;
       case 'd': dn = std::atoi(optarg); break;
       case 'i': input = optarg; break;
       case 's': structure = optarg; break;
       case 'r': std::printf("RMDFT (-r) is not supported by this build.\n"); return 1;
-      case 'e': std::printf("ESP cube input (-e) is not supported by this build.\n"); return 1;
+      case 'e': esp = optarg; centering = false; break;
       case 'f': centering = false; break;
       case 'z': q0 = true; break;
       case 'p': prec = std::atoi(optarg); break;
@@ -85,6 +87,13 @@ int main(int argc, char **argv) {
   if (rism_build_potentials(h) != RISM_OK) {
     std::printf("%s\n", rism_last_error());
     return 1;
+  }
+  if (!esp.empty()) {
+    std::vector<double> e((size_t)g.n[0] * g.n[1] * g.n[2]);
+    if (rism_read_cube(esp.c_str(), g.n, e.data()) != RISM_OK || rism_set_esp(h, e.data()) != RISM_OK) {
+      std::printf("%s\n", rism_last_error());
+      return 1;
+    }
   }
   const std::string fname = deck->fname;
   const std::string tuv = fname + ".tuv";

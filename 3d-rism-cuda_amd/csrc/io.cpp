@@ -491,3 +491,64 @@ extern "C" int rism_write_qv(const char *path, const int *n, const double *box, 
   }
   return finish(fc.f, path);
 }
+
+extern "C" int rism_read_cube(const char *path, const int *n, double *out) {
+  // The ESP cube of main.cu -e (cal_Coulomb.cu:40-96): two comment lines;
+  // natoms and origin (the reference reads a fifth value after the origin;
+  // both forms are accepted here); three axis lines "n dx dy dz" that must
+  // match the grid; natoms atom lines; then the values with z fastest,
+  // stored x-fastest as out[x + nx (y + ny z)].  Values are Hartree/e.
+  if (!path || !n || !out) {
+    set_error("rism_read_cube: bad arguments");
+    return RISM_ERR_ARG;
+  }
+  std::ifstream in(path);
+  if (!in) {
+    set_error(std::string("Error during read esp file: ") + path);
+    return RISM_ERR_IO;
+  }
+  std::string line;
+  std::getline(in, line);
+  std::getline(in, line);
+  if (!std::getline(in, line)) {
+    set_error("Error reading atom count and origin.");
+    return RISM_ERR_FORMAT;
+  }
+  int natom = 0;
+  double o[3];
+  {
+    std::istringstream iss(line);
+    if (!(iss >> natom >> o[0] >> o[1] >> o[2])) {
+      set_error("Error reading atom count and origin.");
+      return RISM_ERR_FORMAT;
+    }
+  }
+  int m[3];
+  for (int a = 0; a < 3; ++a) {
+    double d[3];
+    if (!std::getline(in, line) || !(std::istringstream(line) >> m[a] >> d[0] >> d[1] >> d[2])) {
+      set_error("malformed cube axis line");
+      return RISM_ERR_FORMAT;
+    }
+  }
+  if (m[0] != n[0] || m[1] != n[1] || m[2] != n[2]) {
+    set_error("Error. cube file doesn't match input.");
+    return RISM_ERR_FORMAT;
+  }
+  for (int a = 0; a < std::abs(natom); ++a)
+    if (!std::getline(in, line)) {
+      set_error("cube atom lines truncated");
+      return RISM_ERR_FORMAT;
+    }
+  for (int x = 0; x < n[0]; ++x)
+    for (int y = 0; y < n[1]; ++y)
+      for (int z = 0; z < n[2]; ++z) {
+        double v;
+        if (!(in >> v)) {
+          set_error("cube values truncated");
+          return RISM_ERR_FORMAT;
+        }
+        out[(size_t)x + (size_t)n[0] * ((size_t)y + (size_t)n[1] * z)] = v;
+      }
+  return RISM_OK;
+}

@@ -1062,6 +1062,26 @@ int rism_build_potentials(rism_handle *h) {
   return RISM_OK;
 }
 
+int rism_set_esp(rism_handle *h, const double *esp) {
+  // cal_Coulomb.cu:40-96: the electrostatic potential of the solute on the
+  // grid (Hartree/e) replaces the point-charge sum in de; fr and fk stay the
+  // point-charge long-range parts, so the closure exponent's de - fr changes.
+  if (!h || !esp) { set_error("null argument"); return RISM_ERR_ARG; }
+  if (!h->potentials) { set_error("rism_build_potentials must run before rism_set_esp"); return RISM_ERR_STATE; }
+  HIP_OK(hipSetDevice(h->device));
+  const double ub = kHartree / (kBoltzmann * h->temper);
+  std::vector<double> de((size_t)h->N), fr((size_t)h->N), sr((size_t)h->N);
+  HIP_OK(hcopy(h, fr.data(), h->fr, fr.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (long ip = 0; ip < h->N; ++ip) {
+    de[ip] = esp[ip] * ub;
+    sr[ip] = de[ip] - fr[ip];
+  }
+  HIP_OK(hcopy(h, h->de, de.data(), de.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIP_OK(hcopy(h, h->sr, sr.data(), sr.size() * sizeof(double), hipMemcpyHostToDevice));
+  h->solved = h->evaluated = false;
+  return RISM_OK;
+}
+
 int rism_solve(rism_handle *h, const char *tuv_path, int *steps, double *rms) {
   if (!h) { set_error("null handle"); return RISM_ERR_ARG; }
   if (!h->potentials) { set_error("rism_build_potentials must run before rism_solve"); return RISM_ERR_STATE; }

@@ -39,6 +39,7 @@ EXPORTED = [
     "rism_adjoint_grad", "rism_batch_solve", "rism_chi_default_opts", "rism_chi_nreduced",
     "rism_chi_solve", "rism_write_xvk", "rism_xvk_read_sites", "rism_stream_copy",
     "rism_euv", "rism_qv", "rism_write_guv", "rism_write_huv", "rism_write_euv", "rism_write_qv",
+    "rism_read_cube", "rism_set_esp",
 ]
 
 KERNELS = ["closure_fx", "fft_y_fwd", "fft_z_oz", "fft_y_inv", "resid", "solve", "potentials", "fk",
@@ -159,6 +160,8 @@ def lib():
         L.rism_geometry.argtypes = [C.c_void_p, P(C.c_int)]
         L.rism_stream_copy.argtypes = [C.c_int, C.c_size_t, C.c_int, dp, dp]
         ip = P(C.c_int)
+        L.rism_read_cube.argtypes = [C.c_char_p, ip, dp]
+        L.rism_set_esp.argtypes = [C.c_void_p, dp]
         L.rism_euv.argtypes = [C.c_void_p, dp]
         L.rism_qv.argtypes = [C.c_void_p, dp]
         L.rism_write_guv.argtypes = [C.c_char_p, C.c_int, ip, dp, dp, dp]
@@ -286,6 +289,14 @@ def read_xvk(path: str) -> Solvent:
                 libc.free(C.cast(p, C.c_void_p))
 
 
+def read_cube(path: str, n) -> np.ndarray:
+    """ESP cube values (Hartree/e) on the grid n, x-fastest [N] (rism_read_cube)."""
+    nn = (C.c_int * 3)(*n)
+    out = np.zeros(int(np.prod(n)))
+    _check(lib().rism_read_cube(path.encode(), nn, _dptr(out)), "rism_read_cube")
+    return out
+
+
 def stream_copy(bytes_: int = 1 << 30, reps: int = 20, device: int = 0):
     """(GB/s, us per copy) of the library's plain device copy (rism_stream_copy)."""
     g, u = C.c_double(), C.c_double()
@@ -385,6 +396,15 @@ class Solver:
 
     def build_potentials(self):
         _check(lib().rism_build_potentials(self.h), "rism_build_potentials")
+        return self
+
+    def set_esp(self, esp: np.ndarray):
+        """Replace the point-charge potential by esp[N] (Hartree/e), as the
+        reference's -e cube input (rism_set_esp); after build_potentials."""
+        e = np.ascontiguousarray(esp, dtype=np.float64).reshape(-1)
+        if e.size != self.N:
+            raise ValueError(f"esp has {e.size} values, the grid {self.N}")
+        _check(lib().rism_set_esp(self.h, _dptr(e)), "rism_set_esp")
         return self
 
     def solve(self, tuv_path: Optional[str] = None):

@@ -153,6 +153,9 @@ void rism_deck_free(rism_deck *d);
 /* views of a deck as create() arguments (pointers into the deck) */
 void rism_deck_views(const rism_deck *d, rism_grid *g, rism_solvent *sv,
                      rism_solute *su, rism_opts *o);
+/* ESP cube reader (cal_Coulomb.cu:40-96): checks the axis counts against
+   n[3] and stores the values x-fastest in out[N] (Hartree/e) */
+int rism_read_cube(const char *path, const int *n, double *out);
 /* .xvk reader alone (solvent_read.cc:10-86); arrays owned by the deck */
 int rism_xvk_read(const char *path, rism_deck *into);
 
@@ -165,6 +168,12 @@ int rism_create(const rism_grid *grid, const rism_solvent *solvent,
                 rism_handle **out);
 /* RISM3D::cal_potential (cal_potential.cu:3-14): LJ, Coulomb, fr, fk */
 int rism_build_potentials(rism_handle *h);
+/* main.cu -e with cal_Coulomb.cu:40-96: replace the solute's point-charge
+   potential de by an electrostatic potential on the grid, esp[N] in
+   Hartree/e, x-fastest (rism_read_cube gives it from a cube file).  The
+   long-range parts fr, fk stay those of the point charges.  Call after
+   rism_build_potentials. */
+int rism_set_esp(rism_handle *h, const double *esp);
 /* RISM3D::iterate (iterate.cu:6-90).  If tuv_path names an existing file
    the solve restarts from it and charge-up is skipped (iterate.cu:25-33).
    Returns RISM_SOLVE_* or RISM_ERR_*. */

@@ -2,7 +2,9 @@
 oracle's command line (same deck, same flags, the .xmu files compared), and
 the .tuv restart files exchanged between the HIP solver and the oracle."""
 import os
+import shutil
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -156,3 +158,77 @@ def test_cli_field_writers_match_oracle(gpu, deckdir):
     ix = np.arange(N)
     np.testing.assert_allclose(q[:, 0], (ix % n[0] - n[0] // 2) * box[0] / n[0], atol=1e-4)
     np.testing.assert_allclose(q[:, 2], (ix // (n[0] * n[1]) - n[2] // 2) * box[2] / n[2], atol=1e-4)
+
+
+def test_cli_esp_cube_of_the_point_charges_reproduces_the_solve(gpu, deckdir):
+    """-e with a cube that holds the solute's own point-charge potential
+    (Hartree/e) gives the point-charge solve (-e also turns centring off,
+    main.cu:37-41, so it is compared with -f)."""
+    print("cli esp cube", flush=True)
+    deck = rism3d.read_deck("test1.inp", centering=False)
+    n, box = deck.n, deck.box
+    dr = np.array(box) / np.array(n)
+    bohr = 0.5291771500001
+    ix = np.arange(n[0]) - n[0] // 2
+    X, Y, Z = np.meshgrid(ix * dr[0], (np.arange(n[1]) - n[1] // 2) * dr[1], (np.arange(n[2]) - n[2] // 2) * dr[2],
+                          indexing="ij")
+    esp = np.zeros(X.shape)
+    for q, (ax, ay, az) in zip(deck.solute.q, deck.solute.xyz):
+        r = np.sqrt((X - ax) ** 2 + (Y - ay) ** 2 + (Z - az) ** 2)
+        esp += np.where(r >= 1e-5, q * bohr / np.maximum(r, 1e-300), 0.0)
+    with open("esp.cube", "w") as f:  # values with z fastest, as the reader expects
+        f.write("esp of test1\npoint charges\n")
+        f.write(f"{len(deck.solute.q):5d} 0.0 0.0 0.0 1\n")
+        for a in range(3):
+            e = [0.0, 0.0, 0.0]
+            e[a] = dr[a] / bohr
+            f.write(f"{n[a]:5d} {e[0]:.6f} {e[1]:.6f} {e[2]:.6f}\n")
+        for q, (ax, ay, az) in zip(deck.solute.q, deck.solute.xyz):
+            f.write(f"    1 {q:.6f} {ax / bohr:.6f} {ay / bohr:.6f} {az / bohr:.6f}\n")
+        np.savetxt(f, esp.reshape(-1), fmt="%.15e")
+    for f in ("test1.xmu", "test1.tuv"):
+        if os.path.exists(f):
+            os.remove(f)
+    r = subprocess.run([CLI, "-e", "esp.cube", "test1.inp"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    x_esp = _xmu("test1.xmu")
+    os.remove("test1.xmu")
+    os.remove("test1.tuv") if os.path.exists("test1.tuv") else None
+    r = subprocess.run([CLI, "-f", "x", "test1.inp"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:]
+    x_pc = _xmu("test1.xmu")
+    for k in x_pc:
+        assert x_esp[k] == pytest.approx(x_pc[k], rel=1e-8, abs=1e-4), k
+    # a cube on another grid is an error, with the reference's message
+    lines = open("esp.cube").read().split("\n")
+    lines[3] = f"{n[0] + 2:5d} " + " ".join(lines[3].split()[1:])
+    open("bad.cube", "w").write("\n".join(lines))
+    r = subprocess.run([CLI, "-e", "bad.cube", "test1.inp"], capture_output=True, text=True, timeout=300)
+    assert r.returncode != 0 and "doesn't match" in r.stdout
+
+
+def test_milc_workflow_on_the_reference_deck(gpu, deckdir):
+    """example_milc/README: a charged run, a -z run, then the MILC tool, on
+    the reference's mobley_3053621 deck (256^3 / 64 A, 12-atom solute given
+    with -s).  Its ctip3p.xvk is not shipped, so the in-project water table
+    stands in and the values are not compared with the example's .xmu; the
+    checks are the workflow and that the report is the MILC formula of the
+    two files written."""
+    print("milc workflow", flush=True)
+    from rism3d import configs, milc
+    shutil.copy(configs.solvent_file("water"), "ctip3p.xvk")
+    shutil.copy("milc_3drism.dat", "3drism.dat")
+    for extra in ([], ["-z"]):
+        r = subprocess.run([CLI, *extra, "-i", "3drism.dat", "-s", "mobley_3053621"], capture_output=True, text=True,
+                           timeout=300)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        assert "reading solute data file : mobley_3053621" in r.stdout
+    x, x0 = _xmu("mobley_3053621.xmu"), _xmu("mobley_3053621.xmu0")
+    assert x0["SE_ES"] == 0.0 and x["SE_ES"] != 0.0  # -z zeroes the solute charges
+    assert x["Pressure"] == x0["Pressure"]
+    r = subprocess.run([sys.executable, "-m", "rism3d.milc", "mobley_3053621"], capture_output=True, text=True,
+                       timeout=60, env={**os.environ, "PYTHONPATH": PKG})
+    assert r.returncode == 0, r.stderr
+    want = milc.milc(milc.read_xmu("mobley_3053621.xmu"), milc.read_xmu("mobley_3053621.xmu0"))
+    assert f"SFE(MILC)= {want['Sub']!r} (J/mol)" in r.stdout
+    assert f"SFE(MILC)= {want['HNC']!r} (J/mol)" in r.stdout

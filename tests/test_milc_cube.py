@@ -1,0 +1,72 @@
+"""MILC post-processing (tools/milc.py) and the ESP cube reader
+(cal_Coulomb.cu:40-96), host side only."""
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+import rism3d
+from rism3d import milc
+from conftest import example
+
+
+def test_milc_reproduces_reference_output(tmp_path):
+    """example_milc/mobley_3053621.out is tools/milc.py's output for the .xmu
+    and .xmu0 beside it: the same text, byte for byte."""
+    for ext in (".xmu", ".xmu0"):
+        shutil.copy(example("mobley_3053621" + ext), tmp_path / ("mobley_3053621" + ext))
+    got = milc.report(str(tmp_path / "mobley_3053621"))
+    assert got == open(example("mobley_3053621.out")).read()
+
+
+def test_milc_weights_are_linear():
+    a = {"SFE_SC": 1.0, "SFE_SC_HNC": 0.0, "SFE_GF": 0.0, "PMV": 0.0}
+    z = {k: 0.0 for k in milc.KEYS}
+    assert milc.milc(a, z)["Sub"] == pytest.approx(-0.718451957)
+    assert milc.milc(z, {**z, "PMV": 1.0})["HNC"] == pytest.approx(-682.54621964)
+    with pytest.raises(RuntimeError):
+        milc.milc({"SFE_SC": 1.0}, z)
+
+
+def _write_cube(path, n, vals, natoms=2, fifth=True):
+    """vals x-fastest [N]; the file lists them with z fastest."""
+    with open(path, "w") as f:
+        f.write("comment 1\ncomment 2\n")
+        f.write(f"{natoms:5d} -1.0 -2.0 -3.0" + ("    1\n" if fifth else "\n"))
+        f.write(f"{n[0]:5d} 0.5 0.0 0.0\n{n[1]:5d} 0.0 0.5 0.0\n{n[2]:5d} 0.0 0.0 0.5\n")
+        for a in range(natoms):
+            f.write(f"{6:5d} 0.0 {a:.3f} 0.0 0.0\n")
+        v = vals.reshape(n[2], n[1], n[0])
+        k = 0
+        for x in range(n[0]):
+            for y in range(n[1]):
+                for z in range(n[2]):
+                    f.write(f" {v[z, y, x]:.12e}")
+                    k += 1
+                    if k % 6 == 0:
+                        f.write("\n")
+        f.write("\n")
+
+
+@pytest.mark.parametrize("fifth", [True, False])
+def test_cube_reader_order_and_header_forms(tmp_path, fifth):
+    n = (4, 6, 8)
+    vals = np.random.default_rng(5).standard_normal(int(np.prod(n)))
+    p = str(tmp_path / "e.cube")
+    _write_cube(p, n, vals, fifth=fifth)
+    np.testing.assert_array_equal(rism3d.read_cube(p, n), np.array([float(f"{v:.12e}") for v in vals]))
+
+
+def test_cube_reader_errors(tmp_path):
+    n = (4, 4, 4)
+    p = str(tmp_path / "e.cube")
+    _write_cube(p, n, np.zeros(64))
+    with pytest.raises(RuntimeError, match="doesn't match"):
+        rism3d.read_cube(p, (4, 4, 6))
+    with pytest.raises(RuntimeError, match="esp file"):
+        rism3d.read_cube(str(tmp_path / "missing.cube"), n)
+    text = open(p).read()
+    open(p, "w").write(text[: len(text) - 200])  # drop the last values
+    with pytest.raises(RuntimeError, match="truncated"):
+        rism3d.read_cube(p, n)
