@@ -7,9 +7,11 @@
 // file, -e ESP cube replacing the point-charge potential (it also turns
 // centring off, as main.cu:37-41 falls through to -f), -f <any> no centring
 // (the reference's getopt string "f:" makes -f take an argument), -z zero
-// solute charges (writes .xmu0).  Added flags: -p 0|1 FP64 (default) or mixed
-// precision, -H Anderson history slots (default 2, the reference's).  -r
-// (RMDFT) is not part of this build and is rejected with a message.  The outlist letters of the
+// solute charges (writes .xmu0), -r <file.xsv> RMDFT free energy with the
+// reference fluid's hard-sphere tables (the solvent's c(k) is read from the
+// .cvk beside its .xvk, solvent_read.cc:91-95).  Added flags: -p 0|1 FP64
+// (default) or mixed precision, -H Anderson history slots (default 2, the
+// reference's).  The outlist letters of the
 // deck select the outputs as in output.cu:6-62: m .xmu, d .gra, c .cuv,
 // g .guv, h .huv, e .euv, q .qv.
 #include <unistd.h>
@@ -25,7 +27,7 @@
 
 int main(int argc, char **argv) {
   int cu = 0, dn = 0, prec = RISM_PREC_FP64, nhist = 2;
-  std::string input, structure, esp;
+  std::string input, structure, esp, hs;
   bool centering = true, q0 = false;
   int ch;
   while ((ch = getopt(argc, argv, "c:d:i:s:r:e:f:zp:H:")) != -1) {
@@ -34,7 +36,7 @@ int main(int argc, char **argv) {
       case 'd': dn = std::atoi(optarg); break;
       case 'i': input = optarg; break;
       case 's': structure = optarg; break;
-      case 'r': std::printf("RMDFT (-r) is not supported by this build.\n"); return 1;
+      case 'r': hs = optarg; break;
       case 'e': esp = optarg; centering = false; break;
       case 'f': centering = false; break;
       case 'z': q0 = true; break;
@@ -111,6 +113,29 @@ int main(int argc, char **argv) {
     if (rism_thermo(h, &x) != RISM_OK) {
       std::printf("%s\n", rism_last_error());
       return 1;
+    }
+    if (!hs.empty()) {  // cal_rmdft (output.cu:24): c(k) from <solvent>.cvk, hard spheres from -r
+      const std::string sp = deck->solvent_path;
+      const std::string cvk = sp.substr(0, sp.rfind('.')) + ".cvk";
+      const int V = sv.nsite;
+      std::vector<double> c((size_t)sv.ntab * V * V);
+      int nh = 0;
+      if (rism_read_cvk(cvk.c_str(), V, sv.ntab, c.data()) != RISM_OK ||
+          rism_read_xsv(hs.c_str(), 0, &nh, nullptr, nullptr, nullptr, nullptr, nullptr) != RISM_OK) {
+        std::printf("%s\n", rism_last_error());
+        return 1;
+      }
+      std::vector<double> kh(nh), ch((size_t)V * nh, 0.0), wh((size_t)V * nh, 0.0), pf(V, 0.0), w0(V, 0.0);
+      if (rism_read_xsv(hs.c_str(), nh, &nh, kh.data(), ch.data(), wh.data(), &pf[0], &w0[0]) != RISM_OK) {
+        std::printf("%s\n", rism_last_error());
+        return 1;
+      }
+      rism_rmdft_input ri = {c.data(), nh, kh.data(), ch.data(), wh.data(), pf.data(), w0.data()};
+      if (rism_rmdft(h, &ri, &x.sfe_rmdft) != RISM_OK) {
+        std::printf("%s\n", rism_last_error());
+        return 1;
+      }
+      x.has_rmdft = 1;
     }
     if (rism_write_xmu(&x, o.closure, (fname + (q0 ? ".xmu0" : ".xmu")).c_str()) != RISM_OK) {
       std::printf("%s\n", rism_last_error());

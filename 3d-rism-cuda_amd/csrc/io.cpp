@@ -329,6 +329,7 @@ extern "C" int rism_write_xmu(const rism_xmu *x, int closure, const char *path) 
   }
   const int V = x->nsite;
   std::fprintf(f, " $RESULT\n");
+  if (x->has_rmdft) std::fprintf(f, "SFE_RMDFT= %.5f !(J/mol)\n\n", x->sfe_rmdft);
   std::fprintf(f, "SFE_SC= %.5f !(J/mol)\n", x->sfe_sc);
   for (int v = 0; v < V; ++v) std::fprintf(f, "  SFEC_SC(%d)= %.5f\n", v, x->sfec_sc[v]);
   std::fprintf(f, "\n");
@@ -550,5 +551,70 @@ extern "C" int rism_read_cube(const char *path, const int *n, double *out) {
         }
         out[(size_t)x + (size_t)n[0] * ((size_t)y + (size_t)n[1] * z)] = v;
       }
+  return RISM_OK;
+}
+
+extern "C" int rism_read_cvk(const char *path, int natv, int ntab, double *out) {
+  // solvent_read.cc:88-115: a .cvk file has the .xvk layout (header, site
+  // rows, then ntab records of natv*natv values); only the table is read
+  std::ifstream in(path);
+  if (!in) {
+    set_error(std::string("cvk file could not be opened! (") + path + ")");
+    return RISM_ERR_IO;
+  }
+  std::string line;
+  int num = -1;
+  for (int ln = 0; std::getline(in, line); ++ln) {
+    if (ln == 1) {
+      std::string d0;
+      int v = 0, nt = 0;
+      if (!(std::istringstream(line) >> d0 >> num >> v >> nt) || v != natv || nt != ntab) {
+        set_error("the .cvk size line does not match the solvent (" + line + ")");
+        return RISM_ERR_FORMAT;
+      }
+    }
+    if (num >= 0 && ln == num + 4) break;  // the line after the site rows
+  }
+  if (num < 0) {
+    set_error("incomplete .cvk header");
+    return RISM_ERR_FORMAT;
+  }
+  for (size_t i = 0; i < (size_t)ntab * natv * natv; ++i)
+    if (!(in >> out[i])) {
+      set_error("c(k) table truncated in .cvk");
+      return RISM_ERR_FORMAT;
+    }
+  return RISM_OK;
+}
+
+extern "C" int rism_read_xsv(const char *path, int cap, int *nhs, double *k, double *c, double *w, double *pf,
+                             double *w0) {
+  // solvent_read.cc:118-142: line 1 the table length, line 2 packing
+  // fraction and w(0), then rows "k c_hs(k) w(k)" of the reference fluid
+  std::ifstream in(path);
+  if (!in) {
+    set_error(std::string("Hardsphare file could not be opened! (") + path + ")");
+    return RISM_ERR_IO;
+  }
+  int n = 0;
+  if (!(in >> n) || n < 4) {
+    set_error("malformed .xsv length line");
+    return RISM_ERR_FORMAT;
+  }
+  if (nhs) *nhs = n;
+  if (!k) return RISM_OK;  // size query
+  if (n > cap) {
+    set_error("rism_read_xsv: table longer than the caller's arrays");
+    return RISM_ERR_ARG;
+  }
+  if (!(in >> *pf >> *w0)) {
+    set_error("malformed .xsv packing-fraction line");
+    return RISM_ERR_FORMAT;
+  }
+  for (int i = 0; i < n; ++i)
+    if (!(in >> k[i] >> c[i] >> w[i])) {
+      set_error("hard-sphere table truncated in .xsv");
+      return RISM_ERR_FORMAT;
+    }
   return RISM_OK;
 }

@@ -208,6 +208,31 @@ hipError_t launch_exchem(const IterArgs<R, S> &a, const double *frd, double *par
 hipError_t launch_euv(int nx, int ny, int nz, const double *dr, int V, int natu, const double4 *atoms,
                       const double2 *lj, const double *sigh, const double *qv, const double *rho,
                       const double *g, double *partials, int nblk, hipStream_t st);
+// RMDFT (cal_rmdft.cu): grid and spline tables (coefficients (y, c0, c1, c2)
+// per segment)
+struct RmdftGrid {
+  int nx, ny, nz;
+  long N;
+  double box[3];
+};
+struct RmdftTabs {
+  int V, ntab, nhs;
+  double dk, rho0;
+  double rho[kMaxV];
+  const double4 *cvk;  // [V*V][ntab-1] on k_i = dk (i + 1)
+  const double *khs;   // [nhs] knots of the hard-sphere tables
+  const double4 *chs;  // [V][nhs-1]
+  const double4 *whs;  // [V][nhs-1]
+};
+hipError_t launch_rm_unpack(const double2 *Z, long N, int v, double2 *out, double *partials, int nblk,
+                            hipStream_t st);
+hipError_t launch_rm_kmix(const RmdftGrid &g, const RmdftTabs &t, int v, const double2 *const *H, double2 *w1,
+                          double2 *w2, hipStream_t st);
+hipError_t launch_rm_fd(const double2 *Z, long N, int v, double a, double bex, double rho0, double2 *w2,
+                        hipStream_t st);
+hipError_t launch_rm_wmul(const RmdftGrid &g, const RmdftTabs &t, int v, double2 *w2, hipStream_t st);
+hipError_t launch_rm_reduce(const double2 *Z, long N, int v, const double2 *a, int mode, double e0,
+                            double *partials, int nblk, hipStream_t st);
 hipError_t launch_euv_atoms(int nx, int ny, int nz, const double *dr, int V, int natu, const double4 *atoms,
                             const double2 *lj, const double *sigh, const double *qv, const double *rho,
                             const double *g, double *partials, int nblk, double *out, hipStream_t st);

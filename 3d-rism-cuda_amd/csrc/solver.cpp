@@ -1194,6 +1194,148 @@ int rism_qv(rism_handle *h, double *out) {
   return RISM_OK;
 }
 
+// RMDFT (cal_rmdft.cu:5-114) on the FP64 evaluation of the final state:
+// xmu1 = -sum_v rho_v sum h_v; w = IFFT(sum_v' c~_vv' rho_v' H_v');
+// xmu2 = sum_v rho_v sum w_v, xmu3 = sum_v rho_v sum w_v h_v; the
+// hard-sphere term eda from the weighted density; result (xmu1 + xmu2 +
+// xmu3 / 2 - eda) dv RT.  Sums are per-block partials added in block order.
+int rism_rmdft(rism_handle *h, const rism_rmdft_input *in, double *sfe) {
+  if (!h || !in || !sfe || !in->cvk || !in->khs || !in->chs || !in->whs || !in->pfhs || !in->w0 || in->nhs < 4) {
+    set_error("rism_rmdft: bad arguments");
+    return RISM_ERR_ARG;
+  }
+  if (!h->solved) { set_error("rism_solve must run before rism_rmdft"); return RISM_ERR_STATE; }
+  for (int i = 1; i < in->nhs; ++i)
+    if (!(in->khs[i] > in->khs[i - 1])) { set_error("rism_rmdft: hard-sphere k table not ascending"); return RISM_ERR_ARG; }
+  HIP_OK(hipSetDevice(h->device));
+  if (!h->evaluated)
+    if (int rc = evaluate(h)) return rc;
+  const int V = h->V, nt = h->ntab, nh = in->nhs;
+  const long N = h->N;
+  // the reference stops when |k| runs beyond the tables (solvent_spline.cu:18-21, solvent_spline2.cu:16-19)
+  const double kmax = std::sqrt(std::pow(M_PI * (h->nx - 1) / h->box[0], 2) + std::pow(M_PI * (h->ny - 1) / h->box[1], 2) +
+                                std::pow(M_PI * (h->nz - 1) / h->box[2], 2));
+  if (kmax > h->dk * nt || kmax > in->khs[nh - 1]) { set_error("insufficient maximal T tabulated"); return RISM_ERR_RANGE; }
+  // spline coefficient tables (host, FP64)
+  std::vector<double4> cvk((size_t)V * V * (nt - 1)), chs((size_t)V * (nh - 1)), whs((size_t)V * (nh - 1));
+  {
+    std::vector<double> kt(nt), y(std::max(nt, nh)), c0(std::max(nt, nh)), c1(std::max(nt, nh)), c2(std::max(nt, nh));
+    for (int i = 0; i < nt; ++i) kt[i] = h->dk * (i + 1);
+    for (int v = 0; v < V; ++v)
+      for (int v1 = 0; v1 < V; ++v1) {
+        for (int i = 0; i < nt; ++i) y[i] = in->cvk[((size_t)i * V + v) * V + v1];
+        natural_spline(kt.data(), y.data(), nt, c0.data(), c1.data(), c2.data());
+        for (int i = 0; i < nt - 1; ++i) cvk[(size_t)(v * V + v1) * (nt - 1) + i] = make_double4(y[i], c0[i], c1[i], c2[i]);
+      }
+    for (int v = 0; v < V; ++v)
+      for (int which = 0; which < 2; ++which) {
+        const double *src = (which ? in->whs : in->chs) + (size_t)v * nh;
+        natural_spline(in->khs, src, nh, c0.data(), c1.data(), c2.data());
+        std::vector<double4> &dst = which ? whs : chs;
+        for (int i = 0; i < nh - 1; ++i) dst[(size_t)v * (nh - 1) + i] = make_double4(src[i], c0[i], c1[i], c2[i]);
+      }
+  }
+  RmdftGrid g;
+  g.nx = h->nx; g.ny = h->ny; g.nz = h->nz; g.N = N;
+  for (int k = 0; k < 3; ++k) g.box[k] = h->box[k];
+  RmdftTabs t;
+  std::memset(&t, 0, sizeof t);
+  t.V = V; t.ntab = nt; t.nhs = nh; t.dk = h->dk; t.rho0 = h->rho[0];
+  for (int v = 0; v < V; ++v) t.rho[v] = h->rho[v];
+  const int nblk = 1024;
+  // device scratch: V transformed h, two work fields, tables, partials
+  std::vector<void *> mem;
+  auto grab = [&](size_t bytes) -> void * {
+    void *p = nullptr;
+    std::lock_guard<std::mutex> lk(g_device_mu);
+    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    mem.push_back(p);
+    return p;
+  };
+  auto release = [&]() {
+    std::lock_guard<std::mutex> lk(g_device_mu);
+    for (void *p : mem) (void)hipFree(p);
+    mem.clear();
+  };
+  std::vector<double2 *> H(V);
+  bool ok = true;
+  for (int v = 0; v < V && ok; ++v) ok = (H[v] = (double2 *)grab((size_t)N * sizeof(double2))) != nullptr;
+  double2 *w1 = ok ? (double2 *)grab((size_t)N * sizeof(double2)) : nullptr;
+  double2 *w2 = w1 ? (double2 *)grab((size_t)N * sizeof(double2)) : nullptr;
+  double2 **dH = w2 ? (double2 **)grab(V * sizeof(double2 *)) : nullptr;
+  double4 *dcvk = dH ? (double4 *)grab(cvk.size() * sizeof(double4)) : nullptr;
+  double4 *dchs = dcvk ? (double4 *)grab(chs.size() * sizeof(double4)) : nullptr;
+  double4 *dwhs = dchs ? (double4 *)grab(whs.size() * sizeof(double4)) : nullptr;
+  double *dkhs = dwhs ? (double *)grab(nh * sizeof(double)) : nullptr;
+  double *part = dkhs ? (double *)grab(2 * nblk * sizeof(double)) : nullptr;
+  if (!part) {
+    release();
+    set_error("rism_rmdft: device allocation failed");
+    return RISM_ERR_NOMEM;
+  }
+  t.cvk = dcvk; t.chs = dchs; t.whs = dwhs; t.khs = dkhs;
+  const double2 *Z = (const double2 *)h->ps[0].Z;  // h of the FP64 evaluation, two sites per value
+  std::vector<double> hp(2 * nblk);
+  auto partial_sums = [&](double *s0, double *s1) -> hipError_t {
+    hipError_t e = hipMemcpyAsync(hp.data(), part, hp.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    double a = 0.0, b = 0.0;
+    for (int i = 0; i < nblk; ++i) {
+      a += hp[2 * i];
+      b += hp[2 * i + 1];
+    }
+    *s0 = a;
+    if (s1) *s1 = b;
+    return e;
+  };
+  auto fft = [&](double2 *d, int dir) {
+    return launch_fft3d<double>(d, h->ps[0].plan[0], h->ps[0].plan[1], h->ps[0].plan[2], h->ps[0].geom, h->nx, h->ny,
+                                h->nz, h->dv, 1.0 / h->volume, dir, h->stream);
+  };
+  hipError_t e = hipMemcpyAsync(dH, H.data(), V * sizeof(double2 *), hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dcvk, cvk.data(), cvk.size() * sizeof(double4), hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dchs, chs.data(), chs.size() * sizeof(double4), hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dwhs, whs.data(), whs.size() * sizeof(double4), hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dkhs, in->khs, nh * sizeof(double), hipMemcpyHostToDevice, h->stream);
+  double xmu1 = 0.0, xmu2 = 0.0, xmu3 = 0.0, eda = 0.0;
+  for (int v = 0; v < V && e == hipSuccess; ++v) {
+    e = launch_rm_unpack(Z, N, v, H[v], part, nblk, h->stream);  // one partial per block
+    double s = 0.0;
+    if (e == hipSuccess) e = hipMemcpyAsync(hp.data(), part, nblk * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    for (int i = 0; i < nblk; ++i) s += hp[i];
+    xmu1 -= s * h->rho[v];
+    if (e == hipSuccess) e = fft(H[v], -1);
+  }
+  for (int v = 0; v < V && e == hipSuccess; ++v) {
+    const double bex = (4.0 - 2.0 * in->pfhs[v]) / std::pow(1.0 - in->pfhs[v], 3) * (in->pfhs[v] / t.rho0);  // excp_fd.cc
+    const double av = in->pfhs[v] / t.rho0;
+    const double e0 = t.rho0 * bex * in->w0[v];
+    double s0 = 0.0, s1 = 0.0;
+    e = launch_rm_kmix(g, t, v, dH, w1, w2, h->stream);
+    if (e == hipSuccess) e = fft(w1, 1);
+    if (e == hipSuccess) e = launch_rm_reduce(Z, N, v, w1, 0, 0.0, part, nblk, h->stream);
+    if (e == hipSuccess) e = partial_sums(&s0, &s1);
+    xmu2 += s0 * h->rho[v];
+    xmu3 += s1 * h->rho[v];
+    if (e == hipSuccess) e = fft(w2, 1);
+    if (e == hipSuccess) e = launch_rm_fd(Z, N, v, av, bex, t.rho0, w2, h->stream);
+    if (e == hipSuccess) e = fft(w2, -1);
+    if (e == hipSuccess) e = launch_rm_wmul(g, t, v, w2, h->stream);
+    if (e == hipSuccess) e = fft(w2, 1);
+    if (e == hipSuccess) e = launch_rm_reduce(Z, N, v, w2, 1, e0, part, nblk, h->stream);
+    if (e == hipSuccess) e = partial_sums(&s0, nullptr);
+    eda += s0 * h->rho[v];
+  }
+  release();
+  if (e != hipSuccess) {
+    set_error(std::string("HIP error in rism_rmdft: ") + hipGetErrorString(e));
+    return RISM_ERR_HIP;
+  }
+  *sfe = kAvogadro * kBoltzmann * h->temper * (xmu1 + xmu2 + 0.5 * xmu3 - eda) * h->dv;  // output_xmu.cu:25-28
+  return RISM_OK;
+}
+
 // Adjoint of the converged fixed point (FP64 in both precisions); see
 // k_adjoint.hip for the equations.  The adjoint system
 //   lambda = (g' - 1)(K^T lambda - lambda) + b

@@ -39,7 +39,7 @@ EXPORTED = [
     "rism_adjoint_grad", "rism_batch_solve", "rism_chi_default_opts", "rism_chi_nreduced",
     "rism_chi_solve", "rism_write_xvk", "rism_xvk_read_sites", "rism_stream_copy",
     "rism_euv", "rism_qv", "rism_write_guv", "rism_write_huv", "rism_write_euv", "rism_write_qv",
-    "rism_read_cube", "rism_set_esp",
+    "rism_read_cube", "rism_set_esp", "rism_rmdft", "rism_read_cvk", "rism_read_xsv",
 ]
 
 KERNELS = ["closure_fx", "fft_y_fwd", "fft_z_oz", "fft_y_inv", "resid", "solve", "potentials", "fk",
@@ -73,16 +73,26 @@ class Xmu(C.Structure):
                 ("sfe_gf", C.c_double), ("sfec_sc", C.c_double * 16), ("sfec_sc_hnc", C.c_double * 16),
                 ("sfec_gf", C.c_double * 16), ("se", C.c_double), ("se_lj", C.c_double),
                 ("se_es", C.c_double), ("pmv", C.c_double), ("pressure", C.c_double),
-                ("correction", C.c_double)]
+                ("correction", C.c_double), ("has_rmdft", C.c_int), ("sfe_rmdft", C.c_double)]
 
     def as_dict(self):
         v = self.nsite
-        return {
+        d = {
             "SFE_SC": self.sfe_sc, "SFE_SC_HNC": self.sfe_sc_hnc, "SFE_GF": self.sfe_gf,
             "SFEC_SC": list(self.sfec_sc[:v]), "SFEC_SC_HNC": list(self.sfec_sc_hnc[:v]),
             "SFEC_GF": list(self.sfec_gf[:v]), "SE": self.se, "SE_LJ": self.se_lj, "SE_ES": self.se_es,
             "PMV": self.pmv, "Pressure": self.pressure, "Correction_Term": self.correction,
         }
+        if self.has_rmdft:
+            d["SFE_RMDFT"] = self.sfe_rmdft
+        return d
+
+
+class RmdftInput(C.Structure):
+    """include/rism3d_amd.h rism_rmdft_input (solvent_read.cc:88-143)."""
+    _fields_ = [("cvk", C.POINTER(C.c_double)), ("nhs", C.c_int), ("khs", C.POINTER(C.c_double)),
+                ("chs", C.POINTER(C.c_double)), ("whs", C.POINTER(C.c_double)),
+                ("pfhs", C.POINTER(C.c_double)), ("w0", C.POINTER(C.c_double))]
 
 
 class ChiSite(C.Structure):
@@ -161,6 +171,9 @@ def lib():
         L.rism_stream_copy.argtypes = [C.c_int, C.c_size_t, C.c_int, dp, dp]
         ip = P(C.c_int)
         L.rism_read_cube.argtypes = [C.c_char_p, ip, dp]
+        L.rism_rmdft.argtypes = [C.c_void_p, P(RmdftInput), dp]
+        L.rism_read_cvk.argtypes = [C.c_char_p, C.c_int, C.c_int, dp]
+        L.rism_read_xsv.argtypes = [C.c_char_p, C.c_int, P(C.c_int), dp, dp, dp, dp, dp]
         L.rism_set_esp.argtypes = [C.c_void_p, dp]
         L.rism_euv.argtypes = [C.c_void_p, dp]
         L.rism_qv.argtypes = [C.c_void_p, dp]
@@ -287,6 +300,38 @@ def read_xvk(path: str) -> Solvent:
             p = getattr(d, name)
             if p:
                 libc.free(C.cast(p, C.c_void_p))
+
+
+@dataclass
+class RmdftTables:
+    """The RMDFT inputs: c_vv'(k) [ntab][V][V] and the reference fluid's
+    hard-sphere tables, per reduced site (sites without data hold zeros)."""
+    cvk: np.ndarray
+    khs: np.ndarray
+    chs: np.ndarray  # [V][nhs]
+    whs: np.ndarray  # [V][nhs]
+    pfhs: np.ndarray  # [V]
+    w0: np.ndarray  # [V]
+
+
+def read_rmdft(cvk_path: str, xsv_path: str, solvent: Solvent) -> RmdftTables:
+    """.cvk and .xsv files as the reference reads them (solvent_read.cc:88-143):
+    the hard-sphere data go to site 0, zeros to the other sites."""
+    L = lib()
+    V, nt = solvent.nsite, solvent.chi.shape[0]
+    cvk = np.zeros((nt, V, V))
+    _check(L.rism_read_cvk(cvk_path.encode(), V, nt, _dptr(cvk)), "rism_read_cvk")
+    n = C.c_int()
+    _check(L.rism_read_xsv(xsv_path.encode(), 0, C.byref(n), None, None, None, None, None), "rism_read_xsv")
+    k, c, w = np.zeros(n.value), np.zeros(n.value), np.zeros(n.value)
+    pf, w0 = C.c_double(), C.c_double()
+    _check(L.rism_read_xsv(xsv_path.encode(), n.value, C.byref(n), _dptr(k), _dptr(c), _dptr(w), C.byref(pf),
+                           C.byref(w0)), "rism_read_xsv")
+    chs, whs = np.zeros((V, n.value)), np.zeros((V, n.value))
+    chs[0], whs[0] = c, w
+    pfv, w0v = np.zeros(V), np.zeros(V)
+    pfv[0], w0v[0] = pf.value, w0.value
+    return RmdftTables(cvk, k, chs, whs, pfv, w0v)
 
 
 def read_cube(path: str, n) -> np.ndarray:
@@ -439,6 +484,16 @@ class Solver:
         if rc == SOLVE_DIVERGED:
             raise RuntimeError(f"rism_adjoint_grad: adjoint solve diverged: {last_error()}")
         return {"xyz": dxyz.reshape(n, 3), "q": dq, "sig": dsig, "eps": deps, "steps": steps.value}
+
+    def rmdft(self, tabs: RmdftTables) -> float:
+        """RMDFT solvation free energy (J/mol) of the converged state (rism_rmdft)."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in
+                (tabs.cvk, tabs.khs, tabs.chs, tabs.whs, tabs.pfhs, tabs.w0)]
+        inp = RmdftInput(_dptr(arrs[0]), len(arrs[1]), _dptr(arrs[1]), _dptr(arrs[2]), _dptr(arrs[3]),
+                         _dptr(arrs[4]), _dptr(arrs[5]))
+        out = C.c_double()
+        _check(lib().rism_rmdft(self.h, C.byref(inp), C.byref(out)), "rism_rmdft")
+        return out.value
 
     def euv(self) -> np.ndarray:
         """per-(atom, site) solute-solvent energy x dv, [2 (LJ, ES)][V][natom] J/mol (rism_euv)"""

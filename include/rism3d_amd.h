@@ -116,7 +116,23 @@ typedef struct rism_xmu {
   double pmv;        /* L/mol */
   double pressure;   /* J/m^3 */
   double correction; /* J/mol */
+  int has_rmdft;     /* 1: sfe_rmdft is set and the .xmu gets its line (output_xmu.cu:25-28) */
+  double sfe_rmdft;  /* J/mol */
 } rism_xmu;
+
+/* RMDFT input (solvent_read.cc:88-143): c_vv'(k) of the solvent (.cvk) and
+   the reference hard-sphere fluid (.xsv).  The .xsv holds one site's data;
+   the reference reads it into site 0 and leaves the other sites' arrays
+   unset, which hold zeros in practice: callers pass zeros for them. */
+typedef struct rism_rmdft_input {
+  const double *cvk;   /* [ntab][V][V], the k grid and order of the solvent's chi table */
+  int nhs;             /* hard-sphere table length */
+  const double *khs;   /* [nhs] k, ascending */
+  const double *chs;   /* [V][nhs] hard-sphere c(k) per reduced site */
+  const double *whs;   /* [V][nhs] weight function w(k) per reduced site */
+  const double *pfhs;  /* [V] packing fraction */
+  const double *w0;    /* [V] w(k = 0) */
+} rism_rmdft_input;
 
 /* a parsed deck: .inp (+ optional separate solute file) and its .xvk */
 typedef struct rism_deck {
@@ -156,6 +172,13 @@ void rism_deck_views(const rism_deck *d, rism_grid *g, rism_solvent *sv,
 /* ESP cube reader (cal_Coulomb.cu:40-96): checks the axis counts against
    n[3] and stores the values x-fastest in out[N] (Hartree/e) */
 int rism_read_cube(const char *path, const int *n, double *out);
+/* RMDFT tables (solvent_read.cc:88-143).  .cvk: the table part of a file
+   with the .xvk layout, [ntab][V][V].  .xsv: call with k == NULL to get the
+   length in *nhs, then with arrays of at least cap entries: k, c_hs(k),
+   w(k), packing fraction and w(0) of the reference fluid. */
+int rism_read_cvk(const char *path, int natv, int ntab, double *out);
+int rism_read_xsv(const char *path, int cap, int *nhs, double *k, double *c, double *w, double *pf,
+                  double *w0);
 /* .xvk reader alone (solvent_read.cc:10-86); arrays owned by the deck */
 int rism_xvk_read(const char *path, rism_deck *into);
 
@@ -207,6 +230,9 @@ int rism_load_tuv(rism_handle *h, const char *path);
 /* .xmu writer (output_xmu.cu:7-111) and .gra writer (output_grad.cu:7-30) */
 int rism_write_xmu(const rism_xmu *x, int closure, const char *path);
 int rism_write_gra(const double *grad, int natom, const char *path);
+/* cal_rmdft (cal_rmdft.cu:5-114, solvent_spline2.cu:7-92) at the converged
+   state: the RMDFT solvation free energy, J/mol (the .xmu SFE_RMDFT line) */
+int rism_rmdft(rism_handle *h, const rism_rmdft_input *in, double *sfe_rmdft);
 /* cal_euv per (atom, site) (cal_euv.cu:7-76) at the converged state, x dv as
    output_euv.cu:14-27 writes it: out[(kind * V + v) * natom + u], kind 0 =
    LJ, 1 = Coulomb, J/mol */

@@ -56,6 +56,7 @@ def lib():
         L.orc_thermo.argtypes = [vp, C.POINTER(XmuC)]
         L.orc_grad.argtypes = [vp, dp, dp]
         L.orc_euv.argtypes = [vp, dp]
+        L.orc_rmdft.argtypes = [vp, dp, C.c_int, dp, dp, dp, C.c_double, C.c_double, dp]
         L.orc_write_xmu.argtypes = [vp, C.POINTER(XmuC), C.c_char_p]
         L.orc_fft3d.argtypes = [C.c_void_p, ip, dp, C.c_int]
         L.orc_fft1d.argtypes = [C.c_void_p, C.c_int, C.c_int]
@@ -217,6 +218,15 @@ class Oracle:
         a, b = np.zeros(3 * nu), np.zeros(3 * nu)
         lib().orc_grad(self.s, _d(a), _d(b))
         return np.concatenate([a.reshape(nu, 3), b.reshape(nu, 3)], axis=1)
+
+    def rmdft(self, cvk: np.ndarray, khs, chs, whs, pfhs: float, w0: float) -> float:
+        """SFE_RMDFT (J/mol) of the last calculate (cal_rmdft.cu:5-114)"""
+        cvk = np.ascontiguousarray(cvk, dtype=np.float64)
+        khs, chs, whs = (np.ascontiguousarray(a, dtype=np.float64) for a in (khs, chs, whs))
+        out = C.c_double()
+        if lib().orc_rmdft(self.s, _d(cvk), len(khs), _d(khs), _d(chs), _d(whs), pfhs, w0, C.byref(out)):
+            raise RuntimeError(lib().orc_last_error().decode())
+        return out.value
 
     def euv(self) -> np.ndarray:
         """per-(atom, site) SE x dv, [2 (LJ, ES)][V][natu] (cal_euv.cu:7-76)"""

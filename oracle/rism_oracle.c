@@ -1076,6 +1076,108 @@ void orc_euv(orc_sys *s, double *out) {
   }
 }
 
+/* ------------------------------------------------------------------ */
+/* RMDFT free energy: cal_rmdft.cu:5-114 with solvent_spline2.cu:7-92  */
+/* and the .cvk/.xsv tables of solvent_read.cc:88-143                  */
+/* ------------------------------------------------------------------ */
+/* cvk: [ntab][V][V] in file order (the .cvk table, same k grid as the .xvk);
+   khs/chs/whs: the .xsv columns (k, hard-sphere c(k), weight w(k)) of the
+   reference fluid, given for site 0 only, as the reference reads them; the
+   reference leaves the other sites' arrays unset (solvent_read.cc:90-91,
+   128-130, alloc2D.cc:4-9) and this restatement takes them as zero, what
+   the reference's fresh allocations hold.  Returns SFE_RMDFT in J/mol
+   (output_xmu.cu:25-28).  Uses huv of the last calculate. */
+static double excp_fd_o(double pf, double dens) { /* excp_fd.cc:1-6 */
+  return (4.0 - 2.0 * pf) / ((1.0 - pf) * (1.0 - pf) * (1.0 - pf)) * (pf / dens);
+}
+
+int orc_rmdft(orc_sys *s, const double *cvk, int nhs, const double *khs, const double *chs,
+              const double *whs, double pfhs0, double w00, double *out) {
+  const int V = s->natv, nt = s->ntab, nga = s->nga;
+  const long N = s->ng;
+  if (!s->huv || !s->indga) { set_err("orc_rmdft: no converged fields"); return ORC_ERR; }
+  if (s->ga[nga - 1] > s->ttab[nt - 1]) { set_err("insufficient maximal T tabulated"); return ORC_ERR; }
+  /* splines at the unique |k| (solvent_spline.cu:36-60 for cvv, spline2 for chs, wfk) */
+  double *cva = xcalloc((size_t)V * V * nga, sizeof(double));
+  double *chsa = xcalloc((size_t)V * nga, sizeof(double)), *wfka = xcalloc((size_t)V * nga, sizeof(double));
+  double *y = xcalloc(nt > nhs ? nt : nhs, sizeof(double));
+  double *c0 = xcalloc(nt > nhs ? nt : nhs, sizeof(double)), *c1 = xcalloc(nt > nhs ? nt : nhs, sizeof(double));
+  double *c2 = xcalloc(nt > nhs ? nt : nhs, sizeof(double));
+  for (int v2 = 0; v2 < V; ++v2)
+    for (int v1 = 0; v1 < V; ++v1) {
+      for (int i = 0; i < nt; ++i) y[i] = cvk[((size_t)i * V + v2) * V + v1];
+      orc_spline_coef(s->ttab, y, nt, c0, c1, c2);
+      for (int i = 0; i < nga; ++i)
+        cva[((size_t)v2 * V + v1) * nga + i] = orc_splint(s->ttab, y, c0, c1, c2, nt, s->ga[i]);
+    }
+  orc_spline_coef(khs, chs, nhs, c0, c1, c2);
+  for (int i = 0; i < nga; ++i) chsa[i] = orc_splint(khs, chs, c0, c1, c2, nhs, s->ga[i]);
+  orc_spline_coef(khs, whs, nhs, c0, c1, c2);
+  for (int i = 0; i < nga; ++i) wfka[i] = orc_splint(khs, whs, c0, c1, c2, nhs, s->ga[i]);
+  const double rho0 = s->rhov[0];
+  for (int v = 0; v < V; ++v)                                  /* solvent_spline2.cu:64-69 */
+    for (int i = 0; i < nga; ++i) cva[((size_t)v * V + v) * nga + i] -= chsa[(size_t)v * nga + i] * rho0 / s->rhov[v];
+  double *pf = xcalloc(V, sizeof(double)), *w0 = xcalloc(V, sizeof(double));
+  pf[0] = pfhs0;
+  w0[0] = w00;
+  double complex *H = xcalloc((size_t)V * N, sizeof(double complex));
+  double complex *W = xcalloc((size_t)N, sizeof(double complex));
+  double xmu1 = 0.0, xmu2 = 0.0, xmu3 = 0.0, eda = 0.0;
+  for (int v = 0; v < V; ++v) {                                /* cal_rmdft.cu:22-28 */
+    double acc = 0.0;
+    for (long ip = 0; ip < N; ++ip) acc += creal(s->huv[(size_t)v * N + ip]);
+    xmu1 -= acc * s->rhov[v];
+  }
+  for (int v = 0; v < V; ++v) {
+    memcpy(H + (size_t)v * N, s->huv + (size_t)v * N, N * sizeof(double complex));
+    orc_fft3d(H + (size_t)v * N, s->n, s->box, -1);
+  }
+  for (int v = 0; v < V; ++v) {
+    /* cal_dwork (cal_rmdft.cu:167-188): sum_v' h_v'(k) c~_vv'(k) rho_v', back to r */
+#pragma omp parallel for
+    for (long ip = 0; ip < N; ++ip) {
+      const int ig = s->indga[ip];
+      double complex w = 0.0;
+      for (int v1 = 0; v1 < V; ++v1) w += H[(size_t)v1 * N + ip] * cva[((size_t)v * V + v1) * nga + ig] * s->rhov[v1];
+      W[ip] = w;
+    }
+    orc_fft3d(W, s->n, s->box, +1);
+    double a2 = 0.0, a3 = 0.0;                                 /* cal_rmdft.cu:45-60 */
+    for (long ip = 0; ip < N; ++ip) {
+      a2 += creal(W[ip]);
+      a3 += creal(W[ip]) * creal(s->huv[(size_t)v * N + ip]);
+    }
+    xmu2 += a2 * s->rhov[v];
+    xmu3 += a3 * s->rhov[v];
+    /* rho0wfkhk, rho0, rho0guvfmbex, wfka, cal_eda (cal_rmdft.cu:66-112) */
+    const double *wf = wfka + (size_t)v * nga;
+#pragma omp parallel for
+    for (long ip = 0; ip < N; ++ip) W[ip] = rho0 * wf[s->indga[ip]] * H[(size_t)v * N + ip];
+    orc_fft3d(W, s->n, s->box, +1);
+    const double bex = excp_fd_o(pf[v], rho0), av = pf[v] / rho0;
+#pragma omp parallel for
+    for (long ip = 0; ip < N; ++ip) {
+      const double dens = creal(W[ip]) + rho0;
+      const double p = av * dens;
+      const double fd = (4.0 - 2.0 * p) / ((1.0 - p) * (1.0 - p) * (1.0 - p)) * (p / dens);
+      W[ip] = rho0 * ((creal(s->huv[(size_t)v * N + ip]) + 1.0) * fd - bex);
+    }
+    orc_fft3d(W, s->n, s->box, -1);
+#pragma omp parallel for
+    for (long ip = 0; ip < N; ++ip) W[ip] *= wf[s->indga[ip]];
+    orc_fft3d(W, s->n, s->box, +1);
+    const double e0 = rho0 * bex * w0[v];
+    double ae = 0.0;
+    for (long ip = 0; ip < N; ++ip) ae += (creal(W[ip]) + e0) * (creal(s->huv[(size_t)v * N + ip]) + 1.0) - e0;
+    eda += ae * s->rhov[v];
+  }
+  const double ibeta = ORC_NA * ORC_KB * s->temper;
+  *out = ibeta * (xmu1 + xmu2 + 0.5 * xmu3 - eda) * s->dv;
+  free(cva); free(chsa); free(wfka); free(y); free(c0); free(c1); free(c2); free(pf); free(w0);
+  free(H); free(W);
+  return 0;
+}
+
 void orc_grad(orc_sys *s, double *dulj, double *due) {
   long N = s->ng;
   int V = s->natv, nu = s->natu;

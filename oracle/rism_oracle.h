@@ -122,7 +122,10 @@ int orc_iterate(orc_sys *s, int cu, int verbose, const char *tuv_in,
 
 void orc_thermo(orc_sys *s, orc_xmu *x);
 void orc_grad(orc_sys *s, double *dulj, double *due);
-void orc_euv(orc_sys *s, double *out);  /* [2][natv][natu], x dv */  /* [natu*3] each */
+void orc_euv(orc_sys *s, double *out);  /* [2][natv][natu], x dv */
+/* RMDFT (cal_rmdft.cu): cvk [ntab][V][V], site-0 hard-sphere tables; J/mol */
+int orc_rmdft(orc_sys *s, const double *cvk, int nhs, const double *khs, const double *chs,
+              const double *whs, double pfhs0, double w00, double *out);  /* [natu*3] each */
 int orc_write_xmu(const orc_sys *s, const orc_xmu *x, const char *path);
 int orc_write_tuv(const orc_sys *s, const char *path);
 int orc_read_tuv(orc_sys *s, const char *path);

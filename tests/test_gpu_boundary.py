@@ -53,9 +53,10 @@ def test_cli_matches_oracle_cli(gpu, deckdir, flags, ext):
     assert not bad, bad
 
 
-def test_cli_rejects_rmdft_and_missing_input(gpu, deckdir):
-    r = subprocess.run([CLI, "-r", "x", "test1.inp"], capture_output=True, text=True, timeout=60)
-    assert r.returncode != 0 and "not supported" in r.stdout
+def test_cli_rmdft_without_cvk_and_missing_input(gpu, deckdir):
+    # -r needs the solvent's c(k) beside its .xvk (solvent_read.cc:91-100)
+    r = subprocess.run([CLI, "-r", "x.xsv", "test1.inp"], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "cvk file could not be opened" in r.stdout
     r = subprocess.run([CLI], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0 and "No input file" in r.stdout
 
