@@ -6,7 +6,7 @@
 //   de    = ubeta * sum_{r >= 1e-5} q_u / r
 //   fr    = ubeta * [sum_{r >= 1e-5} q_u (1 - exp(-r)) / r + sum_{r < 1e-5} q_u]
 //   sr    = de - fr, summed directly as the short-range remainder
-// Atom records are wave-uniform, so they arrive through scalar loads.
+// Atom records are staged through LDS in chunks.
 // exp(-r) is evaluated only for r < 38 Angstrom: beyond that 1 - exp(-r)
 // rounds to 1 in FP64, so fr is unchanged and the dropped sr terms are below
 // 1e-16 of q/r.
@@ -20,10 +20,17 @@
 namespace rism {
 
 // One grid point per thread; 1/r comes from rsqrt_f64 and gives r = r^2/r
-// and 1/r^2 without a division or a square root.
+// and 1/r^2 without a division or a square root.  The atoms are staged
+// through LDS in chunks (position and charge, then (sigma_uv^2, 4 eps_uv)
+// for VC sites, zero beyond V), so the inner loop reads broadcast LDS
+// values instead of waiting on a scalar load per atom and site.
+template <int VC>
 __global__ __launch_bounds__(kThreads) void k_potentials(PotArgs a) {
-  const long ip = (long)blockIdx.x * kThreads + threadIdx.x;
-  if (ip >= a.N) return;
+  __shared__ double4 s_at[kAtomChunk];
+  __shared__ double2 s_lj[kAtomChunk * VC];
+  const long ip0 = (long)blockIdx.x * kThreads + threadIdx.x;
+  const bool live = ip0 < a.N;
+  const long ip = live ? ip0 : a.N - 1;  // every thread stages atoms; only live ones store
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int x = (int)(ip % nx);
   const int y = (int)((ip / nx) % ny);
@@ -32,43 +39,53 @@ __global__ __launch_bounds__(kThreads) void k_potentials(PotArgs a) {
   const double ry = (y - ny / 2) * a.dr[1];
   const double rz = (z - nz / 2) * a.dr[2];
   const double cut2 = 1.0e-2 * 1.0e-2;  // LJ distance floor 0.01 A (cal_LJ.cu:63)
-  double lj[kMaxV];
+  double lj[VC];
 #pragma unroll
-  for (int v = 0; v < kMaxV; ++v) lj[v] = 0.0;
+  for (int v = 0; v < VC; ++v) lj[v] = 0.0;
   double e = 0.0, f = 0.0, s = 0.0;
   const int V = a.V, nu = a.natu;
-  for (int u = 0; u < nu; ++u) {
-    const double4 at = a.atoms[u];
-    const double dx = rx - at.x, dy = ry - at.y, dz = rz - at.z;
-    const double r2 = dx * dx + dy * dy + dz * dz;
-    const double rinv = rsqrt_f64(fmax(r2, 1.0e-300));
-    const double ir2 = r2 < cut2 ? 1.0 / cut2 : rinv * rinv;
+  for (int c0 = 0; c0 < nu; c0 += kAtomChunk) {
+    const int nc = min(kAtomChunk, nu - c0);
+    __syncthreads();
+    for (int i = threadIdx.x; i < nc; i += kThreads) s_at[i] = a.atoms[c0 + i];
+    for (int i = threadIdx.x; i < nc * VC; i += kThreads) {
+      const int u = i / VC, v = i - u * VC;
+      s_lj[i] = v < V ? a.lj[v * nu + c0 + u] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    for (int k = 0; k < nc; ++k) {
+      const double4 at = s_at[k];
+      const double dx = rx - at.x, dy = ry - at.y, dz = rz - at.z;
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      const double rinv = rsqrt_f64(fmax(r2, 1.0e-300));
+      const double ir2 = r2 < cut2 ? 1.0 / cut2 : rinv * rinv;
 #pragma unroll
-    for (int v = 0; v < kMaxV; ++v)
-      if (v < V) {
-        const double2 p = a.lj[v * nu + u];
+      for (int v = 0; v < VC; ++v) {
+        const double2 p = s_lj[k * VC + v];
         const double s2 = p.x * ir2;
         const double s6 = s2 * s2 * s2;
         lj[v] += p.y * s6 * (s6 - 1.0);
       }
-    const double ra = r2 * rinv;
-    if (ra >= 1.0e-5) {
-      const double qr = at.w * rinv;
-      e += qr;
-      if (ra < 38.0) {
-        const double ex = exp(-ra);
-        f += qr * (1.0 - ex);
-        s += qr * ex;
+      const double ra = r2 * rinv;
+      if (ra >= 1.0e-5) {
+        const double qr = at.w * rinv;
+        e += qr;
+        if (ra < 38.0) {
+          const double ex = exp(-ra);
+          f += qr * (1.0 - ex);
+          s += qr * ex;
+        } else {
+          f += qr;
+        }
       } else {
-        f += qr;
+        f += at.w;
+        s -= at.w;
       }
-    } else {
-      f += at.w;
-      s -= at.w;
     }
   }
+  if (!live) return;
 #pragma unroll
-  for (int v = 0; v < kMaxV; ++v)
+  for (int v = 0; v < VC; ++v)
     if (v < V) a.ulj[(long)v * a.N + ip] = lj[v] * a.ikbt;
   a.de[ip] = e * a.ubeta;
   a.fr[ip] = f * a.ubeta;
@@ -180,7 +197,12 @@ __global__ __launch_bounds__(kThreads) void k_axpy_fr(S *x, const double *fr, do
 static inline dim3 grid1(long n) { return dim3((unsigned)((n + kThreads - 1) / kThreads)); }
 
 hipError_t launch_potentials(const PotArgs &a, hipStream_t st) {
-  hipLaunchKernelGGL(k_potentials, grid1(a.N), dim3(kThreads), 0, st, a);
+  if (a.V <= 2)
+    hipLaunchKernelGGL(k_potentials<2>, grid1(a.N), dim3(kThreads), 0, st, a);
+  else if (a.V <= 4)
+    hipLaunchKernelGGL(k_potentials<4>, grid1(a.N), dim3(kThreads), 0, st, a);
+  else
+    hipLaunchKernelGGL(k_potentials<8>, grid1(a.N), dim3(kThreads), 0, st, a);
   return hipGetLastError();
 }
 

@@ -23,6 +23,7 @@
 namespace rism {
 
 constexpr int kMaxV = 8;    // reduced solvent sites
+constexpr int kAtomChunk = 128;  // solute atoms staged in LDS per pass of the grid x atom kernels
 constexpr int kHMax = 16;   // Anderson history slots
 
 // 1/sqrt(x) in FP64 for x > 0: the hardware estimate refined by two Newton
