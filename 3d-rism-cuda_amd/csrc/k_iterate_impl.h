@@ -636,6 +636,41 @@ __global__ __launch_bounds__(kThreads) void k_fx_resid(IterArgs<R, S> a) {
       const int p = f / nel, e = f - p * nel;
       zb[k] = Z[p * N + base + e];
     }
+    // points per thread: nel <= EPT * kThreads / P and P >= PMIN in this site class
+    constexpr int PMIN = VC <= 2 ? 1 : (VC <= 4 ? 2 : 3);
+    constexpr int KPT = (EPT + PMIN - 1) / PMIN;
+#ifndef RISM_RESID_CH
+#define RISM_RESID_CH 1
+#endif
+    // Operands of a point (h_ref, t_s, u, history residuals) are loaded after
+    // the line transform in batches of CH points.  Loading every point's
+    // operands before the transform instead (to overlap their latency with
+    // the LDS work) needs so many registers that occupancy halves; measured
+    // slower at 256^3.
+    constexpr int CH = KPT < RISM_RESID_CH ? KPT : RISM_RESID_CH;
+    constexpr bool PREH = CH * VC * HM <= 16;  // history residuals join the batch
+    auto load_ops = [&](unsigned e, double *hbo, double *tvo, double *uvo, Q *hqo) {
+#pragma unroll
+      for (int p = 0; p < (VC + 1) / 2; ++p) {
+        hbo[2 * p] = hbo[2 * p + 1] = 0.0;
+        if (p < P && a.delta && !a.href_only) {
+          const double2 hr2 = lane_at(href + p * N + base, e);
+          hbo[2 * p] = hr2.x;
+          hbo[2 * p + 1] = hr2.y;
+        }
+      }
+      if (!a.href_only) {
+#pragma unroll
+        for (int v = 0; v < VC; ++v)
+          if (v < V) {
+            const long vb = (long)v * N + base;
+            tvo[v] = (double)lane_at(ts + vb, e);
+            uvo[v] = (double)lane_at(uu + vb, e);
+#pragma unroll
+            for (int s = 0; s < (PREH ? HM : 0); ++s) hqo[v * (PREH ? HM : 1) + s] = lane_at(hr + (s < nf ? s : 0) * VN + vb, e);
+          }
+      }
+    };
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       const int f = tid + k * kThreads;
@@ -647,14 +682,6 @@ __global__ __launch_bounds__(kThreads) void k_fx_resid(IterArgs<R, S> a) {
     }
     __syncthreads();
     fft_lines<EPT, true>(lds, P * L, ls, a.px, ltw, tid);
-    // points per thread: nel <= EPT * kThreads / P and P >= PMIN in this site class
-    constexpr int PMIN = VC <= 2 ? 1 : (VC <= 4 ? 2 : 3);
-    constexpr int KPT = (EPT + PMIN - 1) / PMIN;
-#ifndef RISM_RESID_CH
-#define RISM_RESID_CH 1
-#endif
-    constexpr int CH = KPT < RISM_RESID_CH ? KPT : RISM_RESID_CH;
-    constexpr bool PREH = CH * VC * HM <= 16;  // history residuals join the batch
 #pragma unroll
     for (int k0 = 0; k0 < KPT; k0 += CH) {
       // batch of CH points: every load first (clamped index), then the arithmetic
@@ -666,29 +693,15 @@ __global__ __launch_bounds__(kThreads) void k_fx_resid(IterArgs<R, S> a) {
         const unsigned e = (unsigned)min(tid + (k0 + c) * kThreads, nel - 1);
         const int i = e / nx, x = e - i * nx;
         const T2 pw = pre[x];
+        double hbr[VC + 1];
+        load_ops(e, hbr, tv[c], uv[c], &hq[c][0][0]);
 #pragma unroll
         for (int p = 0; p < (VC + 1) / 2; ++p) {
           if (p < P) {
             const T2 z = cscale(cmulc(lds[(p * L + i) * ls + lpad(x)], pw), ivol);
-            hv[c][2 * p] = (double)z.x;
-            hv[c][2 * p + 1] = (double)z.y;
-            if (a.delta && !a.href_only) {
-              const double2 hb = lane_at(href + p * N + base, e);
-              hv[c][2 * p] += hb.x;
-              hv[c][2 * p + 1] += hb.y;
-            }
+            hv[c][2 * p] = (double)z.x + hbr[2 * p];
+            hv[c][2 * p + 1] = (double)z.y + hbr[2 * p + 1];
           }
-        }
-        if (!a.href_only) {
-#pragma unroll
-          for (int v = 0; v < VC; ++v)
-            if (v < V) {
-              const long vb = (long)v * N + base;
-              tv[c][v] = (double)lane_at(ts + vb, e);
-              uv[c][v] = (double)lane_at(uu + vb, e);
-#pragma unroll
-              for (int s = 0; s < (PREH ? HM : 0); ++s) hq[c][v][s] = lane_at(hr + (s < nf ? s : 0) * VN + vb, e);
-            }
         }
       }
 #pragma unroll
