@@ -1,0 +1,126 @@
+"""GPU solves against the FP64 oracle beyond the water/H = 2/cubic cases of
+test_gpu_parity.py: a four-site electrolyte solvent (example/naclaq_vv.xvk),
+5 and 10 Anderson histories, a non-cubic grid with axes n = 2 mod 4, and the
+seeded 64^3 / 16-atom / H = 5 workload with the library's own water table.
+
+FP64: the same Anderson trajectory as the oracle (step count within one),
+Delta-mu within 0.05 J/mol, g within 1e-6.  Mixed precision: against the
+oracle converged to RMS 1e-10, Delta-mu within 0.01 kcal/mol (41.84 J/mol)
+and g within 2e-4 at RMS 1e-6 (test_gpu_parity.py states why)."""
+import numpy as np
+import pytest
+
+import rism3d
+from conftest import example
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+CASES = {
+    # solvent file, grid, box, atoms, Anderson histories
+    "electrolyte_V4_H2": ("naclaq_vv.xvk", (32, 32, 32), (16.0, 16.0, 16.0), 4, 2),
+    "water_H5": ("h2o_vv.xvk", (32, 32, 32), (16.0, 16.0, 16.0), 4, 5),
+    "electrolyte_V4_H10": ("naclaq_vv.xvk", (32, 32, 32), (16.0, 16.0, 16.0), 4, 10),
+    "noncubic_36x40x30": ("h2o_vv.xvk", (36, 40, 30), (18.0, 20.0, 15.0), 4, 2),
+}
+
+
+def _atoms(k, seed=11, spread=2.5):
+    rng = np.random.default_rng(seed)
+    xyz = rng.uniform(-spread, spread, size=(k, 3))
+    sig = rng.uniform(2.6, 3.6, size=k)
+    eps = rng.uniform(200.0, 700.0, size=k)
+    q = rng.normal(0.0, 0.4, size=k)
+    q -= q.mean()
+    return sig, eps, q, xyz
+
+
+def _fields_g(s, V, n):
+    return s.field(rism3d.FIELD_G).reshape(V, -1)
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_fp64_solve_matches_oracle(gpu, case):
+    svf, n, box, natom, nhist = CASES[case]
+    print(f"variant fp64 {case}", flush=True)
+    sig, eps, q, xyz = _atoms(natom)
+    o = orc.Oracle.from_arrays(n, box, example(svf), sig, eps, q, xyz, nhist=nhist).setup()
+    ost, osteps, _ = o.iterate()
+    ox = o.thermo()
+    sv = rism3d.read_xvk(example(svf))
+    s = rism3d.Solver(n, box, sv, rism3d.Solute(sig, eps, q, xyz), rism3d.Options(nhist=nhist))
+    s.build_potentials()
+    st, steps, _ = s.solve()
+    x = s.thermo()
+    g = _fields_g(s, sv.nsite, n)
+    s.close()
+    og = o.field(orc.F_G).reshape(sv.nsite, -1)
+    print("oracle", ost, osteps, ox["SFE_SC"], "gpu", st, steps, x["SFE_SC"], flush=True)
+    assert st == rism3d.SOLVE_CONVERGED and ost == 0
+    assert abs(steps - osteps) <= 1
+    for k in ("SFE_SC", "SFE_GF", "SE_LJ", "SE_ES"):
+        assert abs(x[k] - ox[k]) < 0.05, (k, x[k], ox[k])
+    assert abs(x["PMV"] - ox["PMV"]) < 1e-8
+    assert np.abs(g - og).max() < 1e-6
+
+
+@pytest.mark.parametrize("case", ["electrolyte_V4_H2", "electrolyte_V4_H10", "noncubic_36x40x30"])
+def test_mixed_solve_matches_converged_oracle(gpu, case):
+    svf, n, box, natom, nhist = CASES[case]
+    print(f"variant mixed {case}", flush=True)
+    sig, eps, q, xyz = _atoms(natom)
+    o = orc.Oracle.from_arrays(n, box, example(svf), sig, eps, q, xyz, tol=1e-10, nhist=nhist).setup()
+    assert o.iterate()[0] == 0
+    ox = o.thermo()
+    sv = rism3d.read_xvk(example(svf))
+    s = rism3d.Solver(n, box, sv, rism3d.Solute(sig, eps, q, xyz),
+                      rism3d.Options(nhist=nhist, precision=rism3d.MIXED))
+    s.build_potentials()
+    st, steps, rms = s.solve()
+    x = s.thermo()
+    g = _fields_g(s, sv.nsite, n)
+    s.close()
+    og = o.field(orc.F_G).reshape(sv.nsite, -1)
+    print("mixed", st, steps, rms, x["SFE_SC"] - ox["SFE_SC"], np.abs(g - og).max(), flush=True)
+    assert st == rism3d.SOLVE_CONVERGED and rms <= 1e-6
+    assert abs(x["SFE_SC"] - ox["SFE_SC"]) < 41.84
+    assert np.abs(g - og).max() < 2e-4
+
+
+@pytest.mark.parametrize("prec", [rism3d.FP64, rism3d.MIXED])
+def test_config0_64_matches_oracle(gpu, prec):
+    """The seeded 64^3 / 32 A / 16-atom workload (random walk, H = 5) with the
+    library's own XRISM-KH water table, GPU against the FP64 oracle."""
+    from rism3d import configs
+    print(f"config0 prec={prec}", flush=True)
+    w = configs.config(0)
+    path = configs.solvent_file("water")
+    su = w.solute
+    tol = 1e-6 if prec == rism3d.FP64 else 1e-10
+    o = orc.Oracle.from_arrays(w.n, w.box, path, su.sig, su.eps, su.q, su.xyz, tol=tol, nhist=w.nhist).setup()
+    ost, osteps, _ = o.iterate()
+    ox = o.thermo()
+    s = rism3d.Solver(w.n, w.box, rism3d.read_xvk(path), su, rism3d.Options(nhist=w.nhist, precision=prec))
+    s.build_potentials()
+    st, steps, _ = s.solve()
+    x = s.thermo()
+    g = _fields_g(s, 2, w.n)
+    s.close()
+    og = o.field(orc.F_G).reshape(2, -1)
+    print("config0", ost, osteps, ox["SFE_SC"], st, steps, x["SFE_SC"], np.abs(g - og).max(), flush=True)
+    assert st == rism3d.SOLVE_CONVERGED and ost == 0
+    if prec == rism3d.FP64:
+        assert abs(steps - osteps) <= 1
+        assert abs(x["SFE_SC"] - ox["SFE_SC"]) < 0.05
+        assert np.abs(g - og).max() < 1e-6
+    else:
+        # the tolerance ball: how far the FP64 oracle itself stops from the
+        # fixed point at RMS 1e-6 on this system
+        o6 = orc.Oracle.from_arrays(w.n, w.box, path, su.sig, su.eps, su.q, su.xyz, tol=1e-6,
+                                    nhist=w.nhist).setup()
+        assert o6.iterate()[0] == 0
+        ball = float(np.abs(o6.field(orc.F_G).reshape(2, -1) - og).max())
+        print("config0 ball", ball, flush=True)
+        assert abs(x["SFE_SC"] - ox["SFE_SC"]) < 41.84
+        assert np.abs(g - og).max() < max(2e-4, 2.0 * ball)
