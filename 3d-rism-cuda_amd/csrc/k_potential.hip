@@ -7,9 +7,9 @@
 //   fr    = ubeta * [sum_{r >= 1e-5} q_u (1 - exp(-r)) / r + sum_{r < 1e-5} q_u]
 //   sr    = de - fr, summed directly as the short-range remainder
 // Atom records are staged through LDS in chunks.
-// exp(-r) is evaluated only for r < 38 Angstrom: beyond that 1 - exp(-r)
-// rounds to 1 in FP64, so fr is unchanged and the dropped sr terms are below
-// 1e-16 of q/r.
+// exp(-r) is used only for r < 38 Angstrom: beyond that 1 - exp(-r) rounds
+// to 1 in FP64, so fr is unchanged and the dropped sr terms are below 1e-16
+// of q/r.
 //
 // fk (cal_Coulomb.cu:125-140) = ubeta 4 pi / (k^2 (k^2 + 1)) sum_u q_u
 // exp(-i k.R_u) uses the separability exp(-i k.R) = ex(kx) ey(ky) ez(kz):
@@ -53,22 +53,29 @@ __global__ __launch_bounds__(kThreads) void k_potentials(PotArgs a) {
       s_lj[i] = v < V ? a.lj[v * nu + c0 + u] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    for (int k = 0; k < nc; ++k) {
+    // Two atoms per step: their distance and LJ terms are independent
+    // chains (rsqrt, Newton steps, powers) that the scheduler interleaves;
+    // the Coulomb terms follow, atom by atom, behind their branch on r (an
+    // exp(-r) only below 38 A).  Each sum still adds the atoms in order.
+    auto geom = [&](int k, double *tl, double &ra, double &rinv) {
       const double4 at = s_at[k];
       const double dx = rx - at.x, dy = ry - at.y, dz = rz - at.z;
       const double r2 = dx * dx + dy * dy + dz * dz;
-      const double rinv = rsqrt_f64(fmax(r2, 1.0e-300));
+      rinv = rsqrt_f64(fmax(r2, 1.0e-300));
       const double ir2 = r2 < cut2 ? 1.0 / cut2 : rinv * rinv;
 #pragma unroll
       for (int v = 0; v < VC; ++v) {
         const double2 p = s_lj[k * VC + v];
         const double s2 = p.x * ir2;
         const double s6 = s2 * s2 * s2;
-        lj[v] += p.y * s6 * (s6 - 1.0);
+        tl[v] = p.y * s6 * (s6 - 1.0);
       }
-      const double ra = r2 * rinv;
+      ra = r2 * rinv;
+    };
+    auto coulomb = [&](int k, double ra, double rinv) {
+      const double qa = s_at[k].w;
       if (ra >= 1.0e-5) {
-        const double qr = at.w * rinv;
+        const double qr = qa * rinv;
         e += qr;
         if (ra < 38.0) {
           const double ex = exp(-ra);
@@ -78,9 +85,28 @@ __global__ __launch_bounds__(kThreads) void k_potentials(PotArgs a) {
           f += qr;
         }
       } else {
-        f += at.w;
-        s -= at.w;
+        f += qa;
+        s -= qa;
       }
+    };
+    int k = 0;
+    for (; k + 1 < nc; k += 2) {
+      double la[VC], lb[VC], raa, rab, ria, rib;
+      geom(k, la, raa, ria);
+      geom(k + 1, lb, rab, rib);
+#pragma unroll
+      for (int v = 0; v < VC; ++v) lj[v] += la[v];
+#pragma unroll
+      for (int v = 0; v < VC; ++v) lj[v] += lb[v];
+      coulomb(k, raa, ria);
+      coulomb(k + 1, rab, rib);
+    }
+    if (k < nc) {
+      double la[VC], raa, ria;
+      geom(k, la, raa, ria);
+#pragma unroll
+      for (int v = 0; v < VC; ++v) lj[v] += la[v];
+      coulomb(k, raa, ria);
     }
   }
   if (!live) return;

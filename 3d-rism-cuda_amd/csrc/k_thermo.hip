@@ -112,7 +112,8 @@ __global__ __launch_bounds__(kThreads) void k_euv(int nx, int ny, int nz, double
         }
       }
       __syncthreads();
-      for (int k = 0; k < nc; ++k) {
+      // two atoms per step: independent term chains, added in atom order
+      auto terms = [&](int k, double *tl, double *te) {
         const double4 at = s_at[k];
         const double dx = rx - at.x, dy = ry - at.y, dz = rz - at.z;
         const double r2 = dx * dx + dy * dy + dz * dz;
@@ -126,8 +127,33 @@ __global__ __launch_bounds__(kThreads) void k_euv(int nx, int ny, int nz, double
           const double s2 = p.x * ir2, s6 = s2 * s2 * s2;
           double t = p.y * s6 * (s6 - 1.0);
           asm volatile("" : "+v"(t));  // computed for every lane, then selected: no divergent branch
-          alj[v] += in ? t : 0.0;
-          aes[v] += in ? qr : 0.0;
+          tl[v] = in ? t : 0.0;
+          te[v] = in ? qr : 0.0;
+        }
+      };
+      int k = 0;
+      for (; k + 1 < nc; k += 2) {
+        double la[VC], ea[VC], lb[VC], eb[VC];
+        terms(k, la, ea);
+        terms(k + 1, lb, eb);
+#pragma unroll
+        for (int v = 0; v < VC; ++v) {
+          alj[v] += la[v];
+          aes[v] += ea[v];
+        }
+#pragma unroll
+        for (int v = 0; v < VC; ++v) {
+          alj[v] += lb[v];
+          aes[v] += eb[v];
+        }
+      }
+      if (k < nc) {
+        double la[VC], ea[VC];
+        terms(k, la, ea);
+#pragma unroll
+        for (int v = 0; v < VC; ++v) {
+          alj[v] += la[v];
+          aes[v] += ea[v];
         }
       }
     }
