@@ -143,34 +143,47 @@ __global__ __launch_bounds__(kThreads) void k_phase(int nx, int ny, int nz, int 
   phase[id] = make_double2(cs, -sn);
 }
 
+// FKY rows y = yb*FKY .. yb*FKY+FKY-1 per thread at one (x, z): the x phase
+// of an atom (one per lane) is loaded once for FKY points, and the y and z
+// phases are the same across a wave of consecutive x.
+template <int FKY>
 __global__ __launch_bounds__(kThreads) void k_fk_half(int nx, int ny, int nz, int natu,
                                                       const double4 *atoms, const double2 *phase,
                                                       double bx, double by, double bz, double ubeta,
                                                       double2 *fkh) {
-  const int nyh = ny / 2;
+  const int nyh = ny / 2, nyb = nyh / FKY;
   const long id = (long)blockIdx.x * kThreads + threadIdx.x;
-  if (id >= (long)nx * nyh * nz) return;
+  if (id >= (long)nx * nyb * nz) return;
   const int x = (int)(id % nx);
-  const int y = (int)((id / nx) % nyh);
-  const int z = (int)(id / ((long)nx * nyh));
+  const int yb = (int)((id / nx) % nyb);
+  const int z = (int)(id / ((long)nx * nyb));
   const int nt = nx + ny + nz;
-  double re = 0.0, im = 0.0;
+  double re[FKY], im[FKY];
+#pragma unroll
+  for (int j = 0; j < FKY; ++j) re[j] = im[j] = 0.0;
   for (int u = 0; u < natu; ++u) {
     const double2 px = phase[(long)u * nt + x];
-    const double2 py = phase[(long)u * nt + nx + y];
     const double2 pz = phase[(long)u * nt + nx + ny + z];
-    const double ar = px.x * py.x - px.y * py.y, ai = px.x * py.y + px.y * py.x;
-    const double br = ar * pz.x - ai * pz.y, bi = ar * pz.y + ai * pz.x;
     const double q = atoms[u].w;
-    re += q * br;
-    im += q * bi;
+#pragma unroll
+    for (int j = 0; j < FKY; ++j) {
+      const double2 py = phase[(long)u * nt + nx + yb * FKY + j];
+      const double ar = px.x * py.x - px.y * py.y, ai = px.x * py.y + px.y * py.x;
+      const double br = ar * pz.x - ai * pz.y, bi = ar * pz.y + ai * pz.x;
+      re[j] += q * br;
+      im[j] += q * bi;
+    }
   }
   const double kx = 2.0 * M_PI * ((double)x - nx / 2.0 + 0.5) / bx;
-  const double ky = 2.0 * M_PI * ((double)y - ny / 2.0 + 0.5) / by;
   const double kz = 2.0 * M_PI * ((double)z - nz / 2.0 + 0.5) / bz;
-  const double rk2 = kx * kx + ky * ky + kz * kz;
-  const double sc = 4.0 * M_PI / (rk2 * (rk2 + 1.0)) * ubeta;
-  fkh[id] = make_double2(re * sc, im * sc);
+#pragma unroll
+  for (int j = 0; j < FKY; ++j) {
+    const int y = yb * FKY + j;
+    const double ky = 2.0 * M_PI * ((double)y - ny / 2.0 + 0.5) / by;
+    const double rk2 = kx * kx + ky * ky + kz * kz;
+    const double sc = 4.0 * M_PI / (rk2 * (rk2 + 1.0)) * ubeta;
+    fkh[((long)z * nyh + y) * nx + x] = make_double2(re[j] * sc, im[j] * sc);
+  }
 }
 
 template <typename R>
@@ -238,8 +251,16 @@ hipError_t launch_fk_half(int nx, int ny, int nz, int natu, const double4 *atoms
   if (nt > 0)
     hipLaunchKernelGGL(k_phase, grid1(nt), dim3(kThreads), 0, st, nx, ny, nz, natu, atoms, box[0],
                        box[1], box[2], phase);
-  hipLaunchKernelGGL(k_fk_half, grid1((long)nx * (ny / 2) * nz), dim3(kThreads), 0, st, nx, ny, nz,
-                     natu, atoms, phase, box[0], box[1], box[2], ubeta, fkh);
+  const int nyh = ny / 2;
+  if (nyh % 4 == 0)
+    hipLaunchKernelGGL(k_fk_half<4>, grid1((long)nx * (nyh / 4) * nz), dim3(kThreads), 0, st, nx, ny, nz, natu,
+                       atoms, phase, box[0], box[1], box[2], ubeta, fkh);
+  else if (nyh % 2 == 0)
+    hipLaunchKernelGGL(k_fk_half<2>, grid1((long)nx * (nyh / 2) * nz), dim3(kThreads), 0, st, nx, ny, nz, natu,
+                       atoms, phase, box[0], box[1], box[2], ubeta, fkh);
+  else
+    hipLaunchKernelGGL(k_fk_half<1>, grid1((long)nx * nyh * nz), dim3(kThreads), 0, st, nx, ny, nz, natu,
+                       atoms, phase, box[0], box[1], box[2], ubeta, fkh);
   return hipGetLastError();
 }
 
