@@ -13,7 +13,10 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <array>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "rism_host.h"
@@ -101,6 +104,7 @@ struct rism_handle {
   rism_opts opts{};
   int hm = 2;               // compiled history class >= nhist
   PrecSet ps[2];            // [0] FP64 (always), [1] FP32 (mixed precision)
+  bool tuned = false;       // rism_autotune has chosen the tile shapes
   // device memory
   std::vector<DevBuf> bufs;
   double *ulj = nullptr, *de = nullptr, *fr = nullptr, *sr = nullptr;
@@ -285,6 +289,10 @@ template <typename R> int build_plans(rism_handle *h) {
   return RISM_OK;
 }
 
+// per-workgroup partial sums of the residual pass are sized for this many
+// workgroups (the autotuner's largest candidate)
+constexpr int kMaxXBlocks = 2048;
+
 long env_long(const char *name, long dflt) {
   const char *s = std::getenv(name);
   return (s && *s) ? std::atol(s) : dflt;
@@ -294,8 +302,9 @@ long env_long(const char *name, long dflt) {
 // pass Ty consecutive-x columns, the z pass two mirrored tiles of Tz columns
 // per pack.  Every workgroup's LDS (padded lines) stays within RISM_LDS_CAP
 // bytes (64 KiB by default).  RISM_LX / RISM_TY / RISM_TZ override the
-// choices (tuning sweeps).
-int choose_geometry(rism_handle *h, LaunchGeom &g, size_t rsz) {
+// choices (tuning sweeps); `ovr` = {Lx, Ty, Tz, xblocks}, when given, takes
+// their place (the autotuner's candidates, 0 = keep the default).
+int choose_geometry(rism_handle *h, LaunchGeom &g, size_t rsz, const int *ovr = nullptr) {
   const int nx = h->nx, ny = h->ny, nz = h->nz, P = h->P;
   const long c2 = 2 * (long)rsz;
   // LDS per workgroup: 64 KiB by default (RISM_LDS_CAP raises or lowers it),
@@ -311,13 +320,15 @@ int choose_geometry(rism_handle *h, LaunchGeom &g, size_t rsz) {
   // the y and z passes also stage their twiddle and phase tables (2n complex) in LDS
   g.Ty = largest_divisor_le(nx, std::max(1L, std::min(8192L / ny, (cap - 2L * ny * c2) / (lsy * c2))));
   g.Tz = largest_divisor_le(nx, std::max(1L, std::min(8192L / (2L * P * nz), (cap - 2L * nz * c2) / (2L * P * lsz * c2))));
-  const long lx = env_long("RISM_LX", 0), ty = env_long("RISM_TY", 0), tz = env_long("RISM_TZ", 0);
+  const long lx = ovr ? ovr[0] : env_long("RISM_LX", 0), ty = ovr ? ovr[1] : env_long("RISM_TY", 0),
+             tz = ovr ? ovr[2] : env_long("RISM_TZ", 0);
   if (lx > 0 && ((long)ny * nz) % lx == 0) g.Lx = (int)lx;
   if (ty > 0 && nx % ty == 0) g.Ty = (int)ty;
   if (tz > 0 && nx % tz == 0) g.Tz = (int)tz;
   g.ept_x = ept_class((long)P * g.Lx * nx);
   g.xgroups = (int)((long)ny * nz / g.Lx);
-  g.xblocks = std::min<int>(g.xgroups, (int)env_long("RISM_XBLOCKS", 1024));
+  const long xb = (ovr && ovr[3] > 0) ? ovr[3] : env_long("RISM_XBLOCKS", 1024);
+  g.xblocks = (int)std::min<long>(g.xgroups, std::min<long>(xb, kMaxXBlocks));
   g.lds_x = ((size_t)P * g.Lx * lsx + 2 * (size_t)nx) * c2;
   g.ept_y = ept_class((long)g.Ty * ny);
   g.lds_y = ((size_t)g.Ty * lsy + 2 * (size_t)ny) * c2;
@@ -974,8 +985,7 @@ int create_impl(const rism_grid *grid, const rism_solvent *sv, const rism_solute
   }
   {  // per-workgroup partial sums: residual pass and thermodynamic reductions
     const size_t nv = 1 + h->hm + h->hm * (h->hm + 1) / 2;
-    const size_t nb = std::max(h->ps[0].geom.xblocks, h->ps[1].ready ? h->ps[1].geom.xblocks : 0);
-    const size_t npart = std::max<size_t>((size_t)1024 * 4 * kMaxV, nb * nv);
+    const size_t npart = std::max<size_t>((size_t)1024 * 4 * kMaxV, (size_t)kMaxXBlocks * nv);
     if ((rc = dalloc_t(h, &h->partials, npart))) return rc;
   }
   // solute records and Lorentz-Berthelot pair parameters (cal_LJ.cu:24-31)
@@ -1082,10 +1092,163 @@ int rism_set_esp(rism_handle *h, const double *esp) {
   return RISM_OK;
 }
 
+}  // extern "C"
+
+// ---- tile autotuner -----------------------------------------------------
+// Times one iteration (the five field passes and the solve kernel, without
+// the Anderson update, on the initial estimate t = q_v fr) for candidate
+// tile shapes and keeps the fastest: the x-line group and workgroup count
+// (update/closure and residual passes), then the y tile, then the z tile,
+// each around the current best.  Results are cached per grid shape, site
+// count, history class and precision for the life of the process, so
+// batches of solutes on a few grids tune once per grid.
+namespace {
+struct TuneKey {
+  int nx, ny, nz, V, hm, prec;
+  bool operator<(const TuneKey &o) const {
+    return std::tie(nx, ny, nz, V, hm, prec) < std::tie(o.nx, o.ny, o.nz, o.V, o.hm, o.prec);
+  }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, std::array<int, 4>> g_tune_cache;
+
+template <typename R> int time_iteration(rism_handle *h, int reps, double *ms) {
+  hipEvent_t e0, e1;
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  double tot = 0.0;
+  int rc = RISM_OK;
+  for (int r = -1; r < reps && rc == RISM_OK; ++r) {  // r = -1: warm-up
+    Ctl &c = *h->ctl_host;
+    std::memset(&c, 0, sizeof c);
+    c.nhist = h->opts.nhist;
+    c.maxstep = 1;
+    c.npts = (double)h->V * h->N;
+    c.status = -1;
+    c.upd_iter = -1;
+    if ((rc = put_ctl(h))) break;
+    if (hipEventRecord(e0, h->stream) != hipSuccess) { rc = RISM_ERR_HIP; break; }
+    if ((rc = launch_iteration<R>(h, false, true))) break;
+    if (hipEventRecord(e1, h->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { rc = RISM_ERR_HIP; break; }
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, e0, e1) != hipSuccess) { rc = RISM_ERR_HIP; break; }
+    if (r >= 0) tot += t;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc == RISM_ERR_HIP) set_error("HIP error while timing a tile candidate");
+  *ms = tot / std::max(1, reps);
+  return rc;
+}
+
+template <typename R> int autotune_impl(rism_handle *h, int reps) {
+  PrecSet &ps = h->ps[pidx<R>()];
+  const TuneKey key{h->nx, h->ny, h->nz, h->V, h->hm, (int)sizeof(R)};
+  std::array<int, 4> best{ps.geom.Lx, ps.geom.Ty, ps.geom.Tz, ps.geom.xblocks};
+  // one tuning at a time: handles on other threads that want the same shape
+  // wait here and then take the cached result (no allocation happens under
+  // this lock)
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  {
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      LaunchGeom g;
+      if (choose_geometry(h, g, sizeof(R), it->second.data()) == RISM_OK) ps.geom = g;
+      drop_graphs(h);
+      return RISM_OK;
+    }
+  }
+  HIP_OK(hipMemsetAsync(h->ts, 0, (size_t)h->V * h->N * sizeof(double), h->stream));
+  const double cf_save = h->cf;
+  const bool delta_save = h->delta;
+  h->cf = 1.0;
+  h->delta = false;
+  if (int rc = stage_potentials(h)) return rc;
+  auto trial = [&](std::array<int, 4> c, double *ms) -> int {
+    LaunchGeom g;
+    if (choose_geometry(h, g, sizeof(R), c.data()) != RISM_OK) return 1;  // does not fit: skip
+    if (g.Lx != c[0] || g.Ty != c[1] || g.Tz != c[2]) return 1;            // not a divisor: skip
+    const LaunchGeom keep = ps.geom;
+    ps.geom = g;
+    int rc = time_iteration<R>(h, reps, ms);
+    ps.geom = keep;
+    return rc < 0 ? rc : 0;
+  };
+  double tbest = 0.0;
+  if (int rc = trial(best, &tbest)) {
+    h->cf = cf_save;
+    h->delta = delta_save;
+    return rc < 0 ? rc : RISM_OK;
+  }
+  auto consider = [&](std::array<int, 4> c) -> int {
+    double t = 0.0;
+    const int rc = trial(c, &t);
+    if (rc < 0) return rc;
+    if (rc == 0 && t < tbest) {
+      tbest = t;
+      best = c;
+    }
+    return RISM_OK;
+  };
+  const std::array<int, 4> base = best;
+  for (int lf : {1, 2, 4})  // x-line group and workgroup count
+    for (int xb : {512, 1024, 2048}) {
+      if (lf == 1 && xb == base[3]) continue;
+      for (int dir : {-1, 1}) {
+        if (lf == 1 && dir > 0) continue;
+        std::array<int, 4> c = base;
+        c[0] = dir < 0 ? base[0] / lf : base[0] * lf;
+        c[3] = xb;
+        if (c[0] >= 1)
+          if (int rc = consider(c)) return rc;
+      }
+    }
+  for (int f : {2, 4}) {  // y tile
+    for (int dir : {-1, 1}) {
+      std::array<int, 4> c = best;
+      c[1] = dir < 0 ? best[1] / f : best[1] * f;
+      if (c[1] >= 1)
+        if (int rc = consider(c)) return rc;
+    }
+  }
+  for (int f : {2, 4}) {  // z tile
+    for (int dir : {-1, 1}) {
+      std::array<int, 4> c = best;
+      c[2] = dir < 0 ? best[2] / f : best[2] * f;
+      if (c[2] >= 1)
+        if (int rc = consider(c)) return rc;
+    }
+  }
+  LaunchGeom g;
+  if (int rc = choose_geometry(h, g, sizeof(R), best.data())) return rc;
+  ps.geom = g;
+  h->cf = cf_save;
+  h->delta = delta_save;
+  drop_graphs(h);
+  h->solved = h->evaluated = false;
+  g_tune_cache[key] = best;
+  return RISM_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int rism_autotune(rism_handle *h, int reps, int *geom_out) {
+  if (!h || reps < 1) { set_error("rism_autotune: bad arguments"); return RISM_ERR_ARG; }
+  if (!h->potentials) { set_error("rism_build_potentials must run before rism_autotune"); return RISM_ERR_STATE; }
+  HIP_OK(hipSetDevice(h->device));
+  const int rc = h->opts.precision == RISM_PREC_MIXED ? autotune_impl<float>(h, reps) : autotune_impl<double>(h, reps);
+  if (rc) return rc;
+  h->tuned = true;
+  return geom_out ? rism_geometry(h, geom_out) : RISM_OK;
+}
+
 int rism_solve(rism_handle *h, const char *tuv_path, int *steps, double *rms) {
   if (!h) { set_error("null handle"); return RISM_ERR_ARG; }
   if (!h->potentials) { set_error("rism_build_potentials must run before rism_solve"); return RISM_ERR_STATE; }
   HIP_OK(hipSetDevice(h->device));
+  if (!h->tuned && env_long("RISM_AUTOTUNE", 0) > 0)  // tune once per handle (cached per grid shape)
+    if (int rc = rism_autotune(h, (int)env_long("RISM_AUTOTUNE", 0), nullptr)) return rc;
   return h->opts.precision == RISM_PREC_MIXED ? solve_impl<float>(h, tuv_path, steps, rms)
                                               : solve_impl<double>(h, tuv_path, steps, rms);
 }

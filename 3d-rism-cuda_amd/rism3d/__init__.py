@@ -39,7 +39,7 @@ EXPORTED = [
     "rism_adjoint_grad", "rism_batch_solve", "rism_chi_default_opts", "rism_chi_nreduced",
     "rism_chi_solve", "rism_write_xvk", "rism_xvk_read_sites", "rism_stream_copy",
     "rism_euv", "rism_qv", "rism_write_guv", "rism_write_huv", "rism_write_euv", "rism_write_qv",
-    "rism_read_cube", "rism_set_esp", "rism_rmdft", "rism_read_cvk", "rism_read_xsv",
+    "rism_read_cube", "rism_set_esp", "rism_rmdft", "rism_read_cvk", "rism_read_xsv", "rism_autotune",
 ]
 
 KERNELS = ["closure_fx", "fft_y_fwd", "fft_z_oz", "fft_y_inv", "resid", "solve", "potentials", "fk",
@@ -171,6 +171,7 @@ def lib():
         L.rism_stream_copy.argtypes = [C.c_int, C.c_size_t, C.c_int, dp, dp]
         ip = P(C.c_int)
         L.rism_read_cube.argtypes = [C.c_char_p, ip, dp]
+        L.rism_autotune.argtypes = [C.c_void_p, C.c_int, ip]
         L.rism_rmdft.argtypes = [C.c_void_p, P(RmdftInput), dp]
         L.rism_read_cvk.argtypes = [C.c_char_p, C.c_int, C.c_int, dp]
         L.rism_read_xsv.argtypes = [C.c_char_p, C.c_int, P(C.c_int), dp, dp, dp, dp, dp]
@@ -557,6 +558,13 @@ class Solver:
         n = (C.c_long * len(KERNELS))()
         _check(lib().rism_kernel_times(self.h, ms, n), "rism_kernel_times")
         return {k: (ms[i], n[i]) for i, k in enumerate(KERNELS)}
+
+    def autotune(self, reps: int = 3) -> dict:
+        """Pick the fastest tile shapes by timing (rism_autotune); returns the geometry."""
+        g = (C.c_int * 10)()
+        _check(lib().rism_autotune(self.h, reps, g), "rism_autotune")
+        keys = ["Lx", "Ty", "Tz", "ept_x", "ept_y", "ept_z", "xgroups", "xblocks", "packs", "hclass"]
+        return dict(zip(keys, list(g)))
 
     def geometry(self) -> dict:
         g = (C.c_int * 10)()

@@ -345,6 +345,15 @@ int rism_kernel_times(rism_handle *h, double *ms, long *count);
    element per thread, timed with HIP events over `reps` launches.  gbps
    receives (read + written bytes) / time, us the time per copy. */
 int rism_stream_copy(int device, size_t bytes, int reps, double *gbps, double *us);
+/* Tile autotuner (no counterpart in the reference): times one iteration of
+   the field passes for candidate tile shapes around the defaults (x-line
+   group and workgroup count, y tile, z tile; `reps` timed launches each)
+   and keeps the fastest.  Call after rism_build_potentials and before
+   rism_solve; the state is reset to the initial estimate.  Results are
+   cached per grid, site count, history class and precision for the life of
+   the process.  Setting RISM_AUTOTUNE=<reps> makes rism_solve call it.
+   geom_out (may be NULL) receives rism_geometry's ten values. */
+int rism_autotune(rism_handle *h, int reps, int *geom_out);
 /* launch geometry: Lx, Ty, Tz, ept_x, ept_y, ept_z, xgroups, xblocks, packs, history class */
 int rism_geometry(rism_handle *h, int *out /* [10] */);
 

@@ -124,3 +124,35 @@ def test_config0_64_matches_oracle(gpu, prec):
         print("config0 ball", ball, flush=True)
         assert abs(x["SFE_SC"] - ox["SFE_SC"]) < 41.84
         assert np.abs(g - og).max() < max(2e-4, 2.0 * ball)
+
+
+@pytest.mark.parametrize("prec", [rism3d.FP64, rism3d.MIXED])
+def test_autotune_keeps_the_solution(gpu, prec):
+    """rism_autotune picks tile shapes by timing; the solve that follows
+    gives the default shapes' answer (FP64: the same trajectory up to the
+    partial-sum order), and a second handle of the same shape takes the
+    cached choice."""
+    print(f"autotune prec={prec}", flush=True)
+    svf, n, box, natom, nhist = CASES["electrolyte_V4_H2"]
+    sig, eps, q, xyz = _atoms(natom)
+    sv = rism3d.read_xvk(example(svf))
+    out = []
+    for tune in (False, True, True):
+        s = rism3d.Solver(n, box, sv, rism3d.Solute(sig, eps, q, xyz), rism3d.Options(precision=prec))
+        s.build_potentials()
+        geo = s.autotune(2) if tune else s.geometry()
+        assert (n[1] * n[2]) % geo["Lx"] == 0 and n[0] % geo["Ty"] == 0 and n[0] % geo["Tz"] == 0
+        st, steps, rms = s.solve()
+        assert st == rism3d.SOLVE_CONVERGED
+        out.append((geo, steps, s.thermo()["SFE_SC"], _fields_g(s, sv.nsite, n)))
+        s.close()
+    (g0, s0, x0, f0), (g1, s1, x1, f1), (g2, s2, x2, f2) = out
+    print("autotune", g0, g1, s0, s1, x1 - x0, flush=True)
+    assert g1 == g2  # cached per shape
+    if prec == rism3d.FP64:
+        assert abs(s1 - s0) <= 1
+        assert abs(x1 - x0) < 1e-3
+        assert np.abs(f1 - f0).max() < 1e-6
+    else:
+        assert abs(x1 - x0) < 41.84
+        assert np.abs(f1 - f0).max() < 2e-4
