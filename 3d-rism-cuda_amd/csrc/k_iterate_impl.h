@@ -61,7 +61,7 @@ __device__ __forceinline__ void closure_points(const IterArgs<R, S> &a, typename
   using Q = typename HistT<R>::T;
   constexpr bool kIncr = sizeof(Q) < sizeof(S);
   constexpr int CH = EPT < RISM_CLOS_CH ? EPT : RISM_CLOS_CH;
-  const int nx = a.nx, L = a.Lx, ls = line_stride(nx), V = a.V;
+  const int nx = a.nx, L = a.Lc, ls = line_stride(nx), V = a.V;
   const long N = a.N, VN = (long)V * N;
   const long base = (long)blockIdx.x * L * nx;
   const int nel = L * nx;
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(kThreads) void k_closure_fx(IterArgs<R, S> a) {
   if (!do_upd && !do_run) return;
   extern __shared__ unsigned char smem_raw[];
   T2 *lds = reinterpret_cast<T2 *>(smem_raw);
-  const int nx = a.nx, L = a.Lx, ls = line_stride(nx), P = a.P;
+  const int nx = a.nx, L = a.Lc, ls = line_stride(nx), P = a.P;
   const long N = a.N, VN = (long)a.V * N;
   const long base = (long)blockIdx.x * L * nx;
   const int nel = L * nx;
@@ -341,9 +341,23 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_oz(IterArgs<R, S> a) {
     const int tile = q & 1, p = q >> 1;
     return (long)p * N + (long)z * nxy + (tile ? (long)yb * nx + xb : (long)ya * nx + xa) + i;
   };
+  // Compile-time lengths: element k of a thread sits ZR z rows after element
+  // k - 1 within its tile (tile B from k = EPT/2 on), so its address is a
+  // wave-uniform offset (zofs) plus one 32-bit lane offset shared by every k.
+  // The lane offset stays below (ZR - 1) * nxy + T <= 127 * 2^20 + 64
+  // elements (ZR = 2 NL / EPT <= 128 for NL <= 512, EPT >= 8).
+  constexpr int ZR = NL > 0 ? 2 * (NL > 0 ? NL : 1) / EPT : 1;
+  const long offa = (long)ya * nx + xa, offb = (long)yb * nx + xb;
+  auto zofs = [&](int k) -> long { return (long)((k % (EPT / 2)) * ZR) * nxy + (k < EPT / 2 ? offa : offb); };
+  const unsigned lofs = NL > 0 ? (unsigned)((threadIdx.x / T) * nxy + threadIdx.x % T) : 0u;
   T2 buf[EPT];
+  if constexpr (NL > 0) {
 #pragma unroll
-  for (int k = 0; k < EPT; ++k) buf[k] = Z[gofs(min((int)threadIdx.x + k * kThreads, nel - 1))];
+    for (int k = 0; k < EPT; ++k) buf[k] = lane_at((const T2 *)Z + zofs(k), lofs);
+  } else {
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) buf[k] = Z[gofs(min((int)threadIdx.x + k * kThreads, nel - 1))];
+  }
   T2 *ltw = lds + 2 * P * T * ls, *lpre = ltw + nz;
   stage_tables(ltw, a.pz);
   __syncthreads();
@@ -358,10 +372,16 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_oz(IterArgs<R, S> a) {
       }
     }
     __syncthreads();
+    // Each phase below takes its own opaque copy of the lane index: the
+    // forward and inverse transforms (and the first and last LDS passes)
+    // share their index arithmetic, and the compiler would otherwise keep
+    // it live across the k-space phase, at the cost of occupancy.
+    int tidf = tid;
+    asm volatile("" : "+v"(tidf));
     if constexpr (NL > 0)
-      fft_lines_ct<NL, EPT, false>(lds, ls, (const T2 *)ltw);
+      fft_lines_ct<NL, EPT, false>(lds, ls, (const T2 *)ltw, tidf);
     else
-      fft_lines<EPT, false>(lds, 2 * P * T, ls, a.pz, ltw);
+      fft_lines<EPT, false>(lds, 2 * P * T, ls, a.pz, ltw, tidf);
     // k-space: pair (i, j) of tile A with (T-1-i, nz-1-j) of tile B.  |k|
     // and the spline interval are evaluated in FP64 in both precisions.
     const double tpx = 2.0 * M_PI / a.box[0], tpz = 2.0 * M_PI / a.box[2];
@@ -372,11 +392,13 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_oz(IterArgs<R, S> a) {
     const T4 *__restrict__ chi = a.chi;
     const T2 *__restrict__ fkh = a.fkh;
     const int npair = T * nz;
+    int tido = tid;
+    asm volatile("" : "+v"(tido));
 #pragma unroll
     for (int k = 0; k < (EPT + 1) / 2; ++k) {
       // clamped index: loads and arithmetic are unconditional, the LDS stores
       // below are guarded by the true index
-      const int e0 = tid + k * kThreads;
+      const int e0 = tido + k * kThreads;
       const int e = min(e0, npair - 1);
       {
         const int j = e / T;
@@ -397,8 +419,15 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_oz(IterArgs<R, S> a) {
         for (int v = 0; v < VC; ++v)
 #pragma unroll
           for (int v1 = 0; v1 < VC; ++v1)
-            if (v < V && v1 < V) cf[v][v1] = chi[(long)(v * V + v1) * nseg + lo];
-        const T2 fk = a.use_fk ? fkh[((long)j * (ny / 2) + ya) * nx + x] : mk<T2>((R)0, (R)0);
+            if (v < V && v1 < V) cf[v][v1] = lane_at(chi + (long)(v * V + v1) * nseg, (unsigned)lo);
+        T2 fk = mk<T2>((R)0, (R)0);
+        if (a.use_fk) {
+          if constexpr (NL > 0)  // j = tid / T + k ZR, x = xa + tid % T
+            fk = lane_at(fkh + ((long)k * ZR * (ny / 2) + ya) * nx + xa,
+                         (unsigned)((tido / T) * (ny / 2) * nx + tido % T));
+          else
+            fk = fkh[((long)j * (ny / 2) + ya) * nx + x];
+        }
         const R sa = (R)jsign(j, a.pz.fixsign), sb = (R)jsign(jb, a.pz.fixsign);
         T2 c[VC + 1];
 #pragma unroll
@@ -450,16 +479,25 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_oz(IterArgs<R, S> a) {
       if ((k + 1) % RISM_OZ_GROUP == 0) __builtin_amdgcn_sched_barrier(0);
     }
     __syncthreads();
+    int tidi = tid;
+    asm volatile("" : "+v"(tidi));
     if constexpr (NL > 0)
-      fft_lines_ct<NL, EPT, true>(lds, ls, (const T2 *)ltw);
+      fft_lines_ct<NL, EPT, true>(lds, ls, (const T2 *)ltw, tidi);
     else
-      fft_lines<EPT, true>(lds, 2 * P * T, ls, a.pz, ltw);
+      fft_lines<EPT, true>(lds, 2 * P * T, ls, a.pz, ltw, tidi);
+    int tids = tid;
+    asm volatile("" : "+v"(tids));
+    const unsigned lofs_s = NL > 0 ? (unsigned)((tids / T) * nxy + tids % T) : 0u;
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
-      const int e = tid + k * kThreads;
+      const int e = tids + k * kThreads;
       if (e < nel) {
         const int i = e % T, z = (e / T) % nz, line = (e / T / nz) * T + i;
-        Z[gofs(e)] = cmulc(lds[line * ls + lpad(z)], lpre[z]);
+        const T2 v = cmulc(lds[line * ls + lpad(z)], lpre[z]);
+        if constexpr (NL > 0)
+          lane_at(Z + zofs(k), lofs_s) = v;
+        else
+          Z[gofs(e)] = v;
       }
     }
   }
@@ -849,17 +887,17 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_pure(typename C2<R>::T *Z, i
 template <typename R, typename S>
 hipError_t launch_closure_fx(const IterArgs<R, S> &a, const LaunchGeom &g, bool update, int hm,
                              hipStream_t st) {
-  const dim3 grid(g.xgroups), block(kThreads);
+  const dim3 grid(g.cgroups), block(kThreads);
   if (!update) {  // closure + transform only (the FP64 source pass): no history class
     if constexpr (sizeof(R) == sizeof(double)) {
-      RISM_VC_SWITCH(a.V, RISM_EPTX_SWITCH(g.ept_x, hipLaunchKernelGGL((k_closure_fx<R, S, E, false, VC, 2>), grid, block, g.lds_x, st, a)))
+      RISM_VC_SWITCH(a.V, RISM_EPTX_SWITCH(g.ept_c, hipLaunchKernelGGL((k_closure_fx<R, S, E, false, VC, 2>), grid, block, g.lds_c, st, a)))
       return hipGetLastError();
     } else {
       return hipErrorInvalidValue;
     }
   }
 #define RISM_CLOS(HMV) \
-  RISM_VC_SWITCH(a.V, RISM_EPTX_SWITCH(g.ept_x, hipLaunchKernelGGL((k_closure_fx<R, S, E, true, VC, HMV>), grid, block, g.lds_x, st, a)))
+  RISM_VC_SWITCH(a.V, RISM_EPTX_SWITCH(g.ept_c, hipLaunchKernelGGL((k_closure_fx<R, S, E, true, VC, HMV>), grid, block, g.lds_c, st, a)))
   switch (hm) {
     case 2: RISM_CLOS(2); break;
     case 5: RISM_CLOS(5); break;

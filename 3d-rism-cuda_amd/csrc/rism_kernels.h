@@ -95,7 +95,8 @@ struct IterArgs {
   double *partials;
   int nv;           // accumulators per block in the residual pass
   AxisPlan px, py, pz;
-  int Lx;           // x lines per workgroup
+  int Lx;           // x lines per workgroup of the residual pass
+  int Lc;           // x lines per workgroup of the closure/update pass
   int Ty, Tz;       // tile widths (consecutive x) of the y / z passes
   int xblocks;      // workgroups of the x passes (grid-stride over line groups)
   int store_h;      // residual pass also writes real-space h into Z
@@ -122,6 +123,10 @@ struct LaunchGeom {
   int xgroups;              // number of x line groups (ny*nz / Lx)
   int xblocks;              // residual-pass grid size
   size_t lds_x, lds_y, lds_z;  // bytes, including any staged tables
+  // closure/update pass: its own x line group (one workgroup per group);
+  // it reduces nothing, so its shape never changes a result
+  int Lc, ept_c, cgroups;
+  size_t lds_c;
 };
 
 // ---- iteration launchers (k_iterate.hip) ----

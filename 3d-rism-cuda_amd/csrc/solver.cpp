@@ -298,12 +298,14 @@ long env_long(const char *name, long dflt) {
   return (s && *s) ? std::atol(s) : dflt;
 }
 
-// Tile shapes: x passes hold Lx whole lines (about 1024 elements), the y
+// Tile shapes: the residual pass holds Lx whole x lines (about 2048
+// elements) and the closure/update pass Lc lines (about 1024 elements: its
+// many operand streams per point want more workgroups in flight), the y
 // pass Ty consecutive-x columns, the z pass two mirrored tiles of Tz columns
 // per pack.  Every workgroup's LDS (padded lines) stays within RISM_LDS_CAP
-// bytes (64 KiB by default).  RISM_LX / RISM_TY / RISM_TZ override the
-// choices (tuning sweeps); `ovr` = {Lx, Ty, Tz, xblocks}, when given, takes
-// their place (the autotuner's candidates, 0 = keep the default).
+// bytes (64 KiB by default).  RISM_LX / RISM_LC / RISM_TY / RISM_TZ override
+// the choices (tuning sweeps); `ovr` = {Lx, Ty, Tz, xblocks, Lc}, when given,
+// takes their place (the autotuner's candidates, 0 = keep the default).
 int choose_geometry(rism_handle *h, LaunchGeom &g, size_t rsz, const int *ovr = nullptr) {
   const int nx = h->nx, ny = h->ny, nz = h->nz, P = h->P;
   const long c2 = 2 * (long)rsz;
@@ -315,14 +317,19 @@ int choose_geometry(rism_handle *h, LaunchGeom &g, size_t rsz, const int *ovr = 
   const long cap = std::min<long>(env_long("RISM_LDS_CAP", 65536), lds_max);
   const long xe = env_long("RISM_X_ELEMS", 2048);
   const long lsx = line_stride(nx), lsy = line_stride(ny), lsz = line_stride(nz);
+  const long ce = env_long("RISM_C_ELEMS", 1024);
   g.Lx = largest_divisor_le((long)ny * nz,
                             std::max(1L, std::min(xe / ((long)P * nx), (cap - 2L * nx * c2) / ((long)P * lsx * c2))));
+  g.Lc = largest_divisor_le((long)ny * nz,
+                            std::max(1L, std::min(ce / ((long)P * nx), (cap - 2L * nx * c2) / ((long)P * lsx * c2))));
   // the y and z passes also stage their twiddle and phase tables (2n complex) in LDS
   g.Ty = largest_divisor_le(nx, std::max(1L, std::min(8192L / ny, (cap - 2L * ny * c2) / (lsy * c2))));
   g.Tz = largest_divisor_le(nx, std::max(1L, std::min(8192L / (2L * P * nz), (cap - 2L * nz * c2) / (2L * P * lsz * c2))));
   const long lx = ovr ? ovr[0] : env_long("RISM_LX", 0), ty = ovr ? ovr[1] : env_long("RISM_TY", 0),
              tz = ovr ? ovr[2] : env_long("RISM_TZ", 0);
+  const long lc = (ovr && ovr[4] > 0) ? ovr[4] : env_long("RISM_LC", 0);
   if (lx > 0 && ((long)ny * nz) % lx == 0) g.Lx = (int)lx;
+  if (lc > 0 && ((long)ny * nz) % lc == 0) g.Lc = (int)lc;
   if (ty > 0 && nx % ty == 0) g.Ty = (int)ty;
   if (tz > 0 && nx % tz == 0) g.Tz = (int)tz;
   g.ept_x = ept_class((long)P * g.Lx * nx);
@@ -330,12 +337,15 @@ int choose_geometry(rism_handle *h, LaunchGeom &g, size_t rsz, const int *ovr = 
   const long xb = (ovr && ovr[3] > 0) ? ovr[3] : env_long("RISM_XBLOCKS", 1024);
   g.xblocks = (int)std::min<long>(g.xgroups, std::min<long>(xb, kMaxXBlocks));
   g.lds_x = ((size_t)P * g.Lx * lsx + 2 * (size_t)nx) * c2;
+  g.ept_c = ept_class((long)P * g.Lc * nx);
+  g.cgroups = (int)((long)ny * nz / g.Lc);
+  g.lds_c = ((size_t)P * g.Lc * lsx + 2 * (size_t)nx) * c2;
   g.ept_y = ept_class((long)g.Ty * ny);
   g.lds_y = ((size_t)g.Ty * lsy + 2 * (size_t)ny) * c2;
   g.ept_z = ept_class(2L * P * g.Tz * nz);
   g.lds_z = ((size_t)2 * P * g.Tz * lsz + 2 * (size_t)nz) * c2;
-  if (g.ept_x < 0 || g.ept_x > 16 || g.ept_y < 0 || g.ept_z < 0 || (long)g.lds_x > cap || (long)g.lds_y > cap ||
-      (long)g.lds_z > cap) {
+  if (g.ept_x < 0 || g.ept_x > 16 || g.ept_c < 0 || g.ept_c > 16 || g.ept_y < 0 || g.ept_z < 0 ||
+      (long)g.lds_x > cap || (long)g.lds_c > cap || (long)g.lds_y > cap || (long)g.lds_z > cap) {
     set_error("grid too large for the single-workgroup line transforms");
     return RISM_ERR_ARG;
   }
@@ -375,7 +385,7 @@ template <typename R> IterArgs<R, double> make_args(rism_handle *h) {
   a.partials = h->partials;
   a.nv = 1 + h->hm + h->hm * (h->hm + 1) / 2;
   a.px = ps.plan[0]; a.py = ps.plan[1]; a.pz = ps.plan[2];
-  a.Lx = ps.geom.Lx; a.Ty = ps.geom.Ty; a.Tz = ps.geom.Tz;
+  a.Lx = ps.geom.Lx; a.Lc = ps.geom.Lc; a.Ty = ps.geom.Ty; a.Tz = ps.geom.Tz;
   a.xblocks = ps.geom.xblocks;
   a.store_h = 0;
   a.use_fk = 1;
@@ -1097,9 +1107,9 @@ int rism_set_esp(rism_handle *h, const double *esp) {
 // ---- tile autotuner -----------------------------------------------------
 // Times one iteration (the five field passes and the solve kernel, without
 // the Anderson update, on the initial estimate t = q_v fr) for candidate
-// tile shapes and keeps the fastest: the x-line group and workgroup count
-// (update/closure and residual passes), then the y tile, then the z tile,
-// each around the current best.  Results are cached per grid shape, site
+// tile shapes and keeps the fastest: the residual pass's x-line group and
+// workgroup count, then the closure pass's x-line group, the y tile and the
+// z tile, each around the current best.  Results are cached per grid shape, site
 // count, history class and precision for the life of the process, so
 // batches of solutes on a few grids tune once per grid.
 namespace {
@@ -1110,7 +1120,7 @@ struct TuneKey {
   }
 };
 std::mutex g_tune_mu;
-std::map<TuneKey, std::array<int, 4>> g_tune_cache;
+std::map<TuneKey, std::array<int, 5>> g_tune_cache;
 
 template <typename R> int time_iteration(rism_handle *h, int reps, double *ms) {
   hipEvent_t e0, e1;
@@ -1144,7 +1154,7 @@ template <typename R> int time_iteration(rism_handle *h, int reps, double *ms) {
 template <typename R> int autotune_impl(rism_handle *h, int reps) {
   PrecSet &ps = h->ps[pidx<R>()];
   const TuneKey key{h->nx, h->ny, h->nz, h->V, h->hm, (int)sizeof(R)};
-  std::array<int, 4> best{ps.geom.Lx, ps.geom.Ty, ps.geom.Tz, ps.geom.xblocks};
+  std::array<int, 5> best{ps.geom.Lx, ps.geom.Ty, ps.geom.Tz, ps.geom.xblocks, ps.geom.Lc};
   // one tuning at a time: handles on other threads that want the same shape
   // wait here and then take the cached result (no allocation happens under
   // this lock)
@@ -1163,11 +1173,10 @@ template <typename R> int autotune_impl(rism_handle *h, int reps) {
   const bool delta_save = h->delta;
   h->cf = 1.0;
   h->delta = false;
-  if (int rc = stage_potentials(h)) return rc;
-  auto trial = [&](std::array<int, 4> c, double *ms) -> int {
+  auto trial = [&](std::array<int, 5> c, double *ms) -> int {
     LaunchGeom g;
-    if (choose_geometry(h, g, sizeof(R), c.data()) != RISM_OK) return 1;  // does not fit: skip
-    if (g.Lx != c[0] || g.Ty != c[1] || g.Tz != c[2]) return 1;            // not a divisor: skip
+    if (choose_geometry(h, g, sizeof(R), c.data()) != RISM_OK) return 1;         // does not fit: skip
+    if (g.Lx != c[0] || g.Ty != c[1] || g.Tz != c[2] || g.Lc != c[4]) return 1;  // not a divisor: skip
     const LaunchGeom keep = ps.geom;
     ps.geom = g;
     int rc = time_iteration<R>(h, reps, ms);
@@ -1175,12 +1184,7 @@ template <typename R> int autotune_impl(rism_handle *h, int reps) {
     return rc < 0 ? rc : 0;
   };
   double tbest = 0.0;
-  if (int rc = trial(best, &tbest)) {
-    h->cf = cf_save;
-    h->delta = delta_save;
-    return rc < 0 ? rc : RISM_OK;
-  }
-  auto consider = [&](std::array<int, 4> c) -> int {
+  auto consider = [&](std::array<int, 5> c) -> int {
     double t = 0.0;
     const int rc = trial(c, &t);
     if (rc < 0) return rc;
@@ -1190,40 +1194,44 @@ template <typename R> int autotune_impl(rism_handle *h, int reps) {
     }
     return RISM_OK;
   };
-  const std::array<int, 4> base = best;
-  for (int lf : {1, 2, 4})  // x-line group and workgroup count
-    for (int xb : {512, 1024, 2048}) {
-      if (lf == 1 && xb == base[3]) continue;
+  // one factor at a time around the best so far: {index, factors}
+  auto around = [&](int idx) -> int {
+    for (int f : {2, 4})
       for (int dir : {-1, 1}) {
-        if (lf == 1 && dir > 0) continue;
-        std::array<int, 4> c = base;
-        c[0] = dir < 0 ? base[0] / lf : base[0] * lf;
-        c[3] = xb;
-        if (c[0] >= 1)
+        std::array<int, 5> c = best;
+        c[idx] = dir < 0 ? best[idx] / f : best[idx] * f;
+        if (c[idx] >= 1)
           if (int rc = consider(c)) return rc;
       }
-    }
-  for (int f : {2, 4}) {  // y tile
-    for (int dir : {-1, 1}) {
-      std::array<int, 4> c = best;
-      c[1] = dir < 0 ? best[1] / f : best[1] * f;
-      if (c[1] >= 1)
-        if (int rc = consider(c)) return rc;
-    }
-  }
-  for (int f : {2, 4}) {  // z tile
-    for (int dir : {-1, 1}) {
-      std::array<int, 4> c = best;
-      c[2] = dir < 0 ? best[2] / f : best[2] * f;
-      if (c[2] >= 1)
-        if (int rc = consider(c)) return rc;
-    }
-  }
-  LaunchGeom g;
-  if (int rc = choose_geometry(h, g, sizeof(R), best.data())) return rc;
-  ps.geom = g;
+    return RISM_OK;
+  };
+  auto search = [&]() -> int {
+    if (int rc = stage_potentials(h)) return rc;
+    if (int rc = trial(best, &tbest)) return rc < 0 ? rc : RISM_OK;
+    const std::array<int, 5> base = best;
+    for (int lf : {1, 2, 4})  // residual-pass x-line group and workgroup count
+      for (int xb : {512, 1024, 2048}) {
+        if (lf == 1 && xb == base[3]) continue;
+        for (int dir : {-1, 1}) {
+          if (lf == 1 && dir > 0) continue;
+          std::array<int, 5> c = base;
+          c[0] = dir < 0 ? base[0] / lf : base[0] * lf;
+          c[3] = xb;
+          if (c[0] >= 1)
+            if (int rc = consider(c)) return rc;
+        }
+      }
+    for (int idx : {4, 1, 2})  // closure-pass x-line group, y tile, z tile
+      if (int rc = around(idx)) return rc;
+    return RISM_OK;
+  };
+  const int rc = search();
   h->cf = cf_save;
   h->delta = delta_save;
+  if (rc < 0) return rc;
+  LaunchGeom g;
+  if (int rc2 = choose_geometry(h, g, sizeof(R), best.data())) return rc2;
+  ps.geom = g;
   drop_graphs(h);
   h->solved = h->evaluated = false;
   g_tune_cache[key] = best;
