@@ -236,16 +236,22 @@ __device__ __forceinline__ void stage_tables(T2 *ltw, const AxisPlan &p) {
 // Run the full plan on nlines lines of length p.n held in LDS.  EPT = upper
 // bound of elements per thread (nlines * n <= kThreads * EPT); tw = the
 // plan's twiddle table staged in LDS (stage_tables); tid = the lane's
-// thread index (callers in persistent loops pass an opaque copy).
-template <int EPT, bool INV, typename T2>
+// thread index (callers in persistent loops pass an opaque copy).  MAXF < 8
+// leaves the radix-8 and radix-16 stages out of the kernel (a smaller
+// register peak); the plan must then hold no such stage.
+template <int EPT, bool INV, int MAXF = 16, typename T2>
 __device__ __forceinline__ void fft_lines(T2 *lds, int nlines, int ls, const AxisPlan &p, const T2 *tw,
                                           int tid = threadIdx.x) {
   for (int st = 0; st < p.nst; ++st) {
     const int F = p.rad[st], s = p.s[st], m = p.m[st], ts = p.tstep[st];
     const int a2 = p.ls2[st], b2 = p.lb2[st];
     switch (F) {
-      case 16: fft_stage<16, (EPT + 15) / 16, INV>(lds, nlines, ls, p.n, s, m, ts, a2, b2, tw, tid); break;
-      case 8: fft_stage<8, (EPT + 7) / 8, INV>(lds, nlines, ls, p.n, s, m, ts, a2, b2, tw, tid); break;
+      case 16:
+        if constexpr (MAXF >= 16) fft_stage<16, (EPT + 15) / 16, INV>(lds, nlines, ls, p.n, s, m, ts, a2, b2, tw, tid);
+        break;
+      case 8:
+        if constexpr (MAXF >= 8) fft_stage<8, (EPT + 7) / 8, INV>(lds, nlines, ls, p.n, s, m, ts, a2, b2, tw, tid);
+        break;
       case 4: fft_stage<4, (EPT + 3) / 4, INV>(lds, nlines, ls, p.n, s, m, ts, a2, b2, tw, tid); break;
       case 2: fft_stage<2, (EPT + 1) / 2, INV>(lds, nlines, ls, p.n, s, m, ts, a2, b2, tw, tid); break;
       case 3: fft_stage<3, (EPT + 2) / 3, INV>(lds, nlines, ls, p.n, s, m, ts, a2, b2, tw, tid); break;

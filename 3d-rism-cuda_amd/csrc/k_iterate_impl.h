@@ -659,6 +659,21 @@ __global__ __launch_bounds__(kThreads) void k_fx_resid(IterArgs<R, S> a) {
   double acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+#ifndef RISM_RESID_ZPF
+#define RISM_RESID_ZPF 1
+#endif
+  // spectrum of a line group into registers (clamped index: one batch of loads)
+  T2 zb[EPT];
+  auto load_z = [&](int g, int lane) {
+    const long gb = (long)g * nel;
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const int f = min(lane + k * kThreads, P * nel - 1);
+      const int p = f / nel, e = f - p * nel;
+      zb[k] = Z[p * N + gb + e];
+    }
+  };
+  if (RISM_RESID_ZPF) load_z(blockIdx.x, threadIdx.x);
   for (int grp = blockIdx.x; grp < groups; grp += gridDim.x) {
     const long base = (long)grp * nel;
     // the lane's index is made opaque per group: otherwise the compiler
@@ -667,13 +682,7 @@ __global__ __launch_bounds__(kThreads) void k_fx_resid(IterArgs<R, S> a) {
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
     __syncthreads();
-    T2 zb[EPT];
-#pragma unroll
-    for (int k = 0; k < EPT; ++k) {
-      const int f = min(tid + k * kThreads, P * nel - 1);
-      const int p = f / nel, e = f - p * nel;
-      zb[k] = Z[p * N + base + e];
-    }
+    if (!RISM_RESID_ZPF) load_z(grp, tid);
     // points per thread: nel <= EPT * kThreads / P and P >= PMIN in this site class
     constexpr int PMIN = VC <= 2 ? 1 : (VC <= 4 ? 2 : 3);
     constexpr int KPT = (EPT + PMIN - 1) / PMIN;
@@ -719,7 +728,24 @@ __global__ __launch_bounds__(kThreads) void k_fx_resid(IterArgs<R, S> a) {
       }
     }
     __syncthreads();
-    fft_lines<EPT, true>(lds, P * L, ls, a.px, ltw, tid);
+    // the transform and the point loop take their own opaque lane index, so
+    // the LDS indices of the first pass are not kept live through them
+    int tidf = tid;
+    asm volatile("" : "+v"(tidf));
+    // FP64: the radix-4 plan (no radix-8/16 stage compiled in) lowers the
+    // register peak from the transform enough for one more wave per SIMD;
+    // FP32 fits its radix-16 plan at the same occupancy as the point loop
+    if constexpr (sizeof(R) == sizeof(double))
+      fft_lines<EPT, true, 5>(lds, P * L, ls, a.pxr, ltw, tidf);
+    else
+      fft_lines<EPT, true>(lds, P * L, ls, a.px, ltw, tidf);
+    int tidr = tid;
+    asm volatile("" : "+v"(tidr));
+    // The next group's spectrum is loaded now, so its latency overlaps this
+    // group's point loop instead of stalling the next transform.  After the
+    // last group the load repeats this group's (discarded).  Registers: the
+    // point loop stays below the transform's peak with it.
+    if (RISM_RESID_ZPF) load_z(grp + (int)gridDim.x < groups ? grp + (int)gridDim.x : grp, tidr);
 #pragma unroll
     for (int k0 = 0; k0 < KPT; k0 += CH) {
       // batch of CH points: every load first (clamped index), then the arithmetic
@@ -728,7 +754,7 @@ __global__ __launch_bounds__(kThreads) void k_fx_resid(IterArgs<R, S> a) {
       Q hq[CH][VC][PREH ? HM : 1];
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
-        const unsigned e = (unsigned)min(tid + (k0 + c) * kThreads, nel - 1);
+        const unsigned e = (unsigned)min(tidr + (k0 + c) * kThreads, nel - 1);
         const int i = e / nx, x = e - i * nx;
         const T2 pw = pre[x];
         double hbr[VC + 1];
@@ -744,7 +770,7 @@ __global__ __launch_bounds__(kThreads) void k_fx_resid(IterArgs<R, S> a) {
       }
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
-        const unsigned e = tid + (k0 + c) * kThreads;
+        const unsigned e = tidr + (k0 + c) * kThreads;
         if (k0 + c < KPT && e < (unsigned)nel) {
           if (a.store_h) {
             const int i = e / nx, x = e - i * nx;

@@ -43,12 +43,14 @@ bool file_exists(const char *p) {
 
 // factor n into radices {16, 8, 4, 2, 3, 5}: as few power-of-two stages as
 // possible with their exponents spread evenly, then the odd radices
-bool make_radices(int n, std::vector<int> &rad) {
+// radices of an n-point plan: the power-of-two part in stages of at most
+// 2^maxbits (balanced bit counts), then the threes and fives
+bool make_radices(int n, std::vector<int> &rad, int maxbits = 4) {
   rad.clear();
   int e = 0, m = n;
   while (m % 2 == 0) { m /= 2; ++e; }
   if (e > 0) {
-    const int ns = (e + 3) / 4;
+    const int ns = (e + maxbits - 1) / maxbits;
     for (int s = 0; s < ns; ++s) {
       const int b = e / ns + (s < e % ns ? 1 : 0);
       rad.push_back(1 << b);
@@ -82,6 +84,7 @@ int largest_divisor_le(long n, long cap) {
 struct PrecSet {
   bool ready = false;
   AxisPlan plan[3];
+  AxisPlan plan_xr;  // x axis, radices <= 5 (FP64 residual pass), same tables as plan[0]
   LaunchGeom geom{};
   void *u = nullptr, *fr = nullptr, *fkh = nullptr, *chi = nullptr, *Z = nullptr, *tables = nullptr;
 };
@@ -246,15 +249,13 @@ template <typename R> int build_plans(rism_handle *h) {
   T2 *dt = nullptr;
   if (int rc = dalloc_t(h, &dt, tot)) return rc;
   std::vector<T2> host(tot);
-  size_t off = 0;
-  for (int a = 0; a < 3; ++a) {
-    const int n = ns[a];
+  // Stockham stage parameters of an n-point plan with radices up to 2^maxbits
+  auto fill = [&](AxisPlan &p, int n, int maxbits) -> int {
     std::vector<int> rad;
-    if (!make_radices(n, rad)) {
+    if (!make_radices(n, rad, maxbits)) {
       set_error("unsupported grid length " + std::to_string(n));
       return RISM_ERR_ARG;
     }
-    AxisPlan &p = ps.plan[a];
     std::memset(&p, 0, sizeof p);
     p.n = n;
     p.nst = (int)rad.size();
@@ -271,6 +272,13 @@ template <typename R> int build_plans(rism_handle *h) {
       s *= rad[k];
     }
     p.fixsign = ((n / 2) % 2) ? -1 : 1;
+    return RISM_OK;
+  };
+  size_t off = 0;
+  for (int a = 0; a < 3; ++a) {
+    const int n = ns[a];
+    AxisPlan &p = ps.plan[a];
+    if (int rc = fill(p, n, 4)) return rc;
     for (int k = 0; k < n; ++k) {
       const double ang = 2.0 * M_PI * (double)k / (double)n;
       host[off + k].x = (R)std::cos(ang);
@@ -284,6 +292,11 @@ template <typename R> int build_plans(rism_handle *h) {
     p.pre = dt + off + n;
     off += 2 * (size_t)n;
   }
+  // the FP64 residual pass's x plan: radix-4 stages at most (that kernel
+  // compiles no radix-8/16 stage), on the x axis's tables
+  if (int rc = fill(ps.plan_xr, h->nx, 2)) return rc;
+  ps.plan_xr.tw = ps.plan[0].tw;
+  ps.plan_xr.pre = ps.plan[0].pre;
   HIP_OK(hcopy(h, dt, host.data(), tot * sizeof(T2), hipMemcpyHostToDevice));
   ps.tables = dt;
   return RISM_OK;
@@ -299,7 +312,8 @@ long env_long(const char *name, long dflt) {
 }
 
 // Tile shapes: the residual pass holds Lx whole x lines (about 2048
-// elements) and the closure/update pass Lc lines (about 1024 elements: its
+// elements in FP32, 1024 in FP64) and the closure/update pass Lc lines
+// (about 1024 elements: its
 // many operand streams per point want more workgroups in flight), the y
 // pass Ty consecutive-x columns, the z pass two mirrored tiles of Tz columns
 // per pack.  Every workgroup's LDS (padded lines) stays within RISM_LDS_CAP
@@ -315,7 +329,9 @@ int choose_geometry(rism_handle *h, LaunchGeom &g, size_t rsz, const int *ovr = 
   if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device) != hipSuccess)
     lds_max = 65536;
   const long cap = std::min<long>(env_long("RISM_LDS_CAP", 65536), lds_max);
-  const long xe = env_long("RISM_X_ELEMS", 2048);
+  // residual pass: 2048 elements per workgroup in FP32, 1024 in FP64 (its
+  // registers then allow four waves per SIMD)
+  const long xe = env_long("RISM_X_ELEMS", rsz == sizeof(double) ? 1024 : 2048);
   const long lsx = line_stride(nx), lsy = line_stride(ny), lsz = line_stride(nz);
   const long ce = env_long("RISM_C_ELEMS", 1024);
   g.Lx = largest_divisor_le((long)ny * nz,
@@ -385,6 +401,7 @@ template <typename R> IterArgs<R, double> make_args(rism_handle *h) {
   a.partials = h->partials;
   a.nv = 1 + h->hm + h->hm * (h->hm + 1) / 2;
   a.px = ps.plan[0]; a.py = ps.plan[1]; a.pz = ps.plan[2];
+  a.pxr = ps.plan_xr;
   a.Lx = ps.geom.Lx; a.Lc = ps.geom.Lc; a.Ty = ps.geom.Ty; a.Tz = ps.geom.Tz;
   a.xblocks = ps.geom.xblocks;
   a.store_h = 0;
