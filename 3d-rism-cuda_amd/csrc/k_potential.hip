@@ -6,6 +6,11 @@
 //   de    = ubeta * sum_{r >= 1e-5} q_u / r
 //   fr    = ubeta * [sum_{r >= 1e-5} q_u (1 - exp(-r)) / r + sum_{r < 1e-5} q_u]
 //   sr    = de - fr, summed directly as the short-range remainder
+// and, per site, the two sums the solute-solvent energy SE needs
+// (cal_euv.cu:35-76): 4 eps_uv (s^12 - s^6) and q_u / r over the atoms with
+// r >= sigma_uv / 2 (exact r, no distance floor).  SE is then one weighted
+// sum over the converged g instead of a second pass over every atom-point
+// pair.
 // Atom records are staged through LDS in chunks.
 // exp(-r) is used only for r < 38 Angstrom: beyond that 1 - exp(-r) rounds
 // to 1 in FP64, so fr is unchanged and the dropped sr terms are below 1e-16
@@ -21,13 +26,14 @@ namespace rism {
 
 // One grid point per thread; 1/r comes from rsqrt_f64 and gives r = r^2/r
 // and 1/r^2 without a division or a square root.  The atoms are staged
-// through LDS in chunks (position and charge, then (sigma_uv^2, 4 eps_uv)
-// for VC sites, zero beyond V), so the inner loop reads broadcast LDS
-// values instead of waiting on a scalar load per atom and site.
+// through LDS in chunks (position and charge, then (sigma_uv^2, 4 eps_uv,
+// sigma_uv / 2) for VC sites; beyond V a padding record that contributes
+// nothing), so the inner loop reads broadcast LDS values instead of waiting
+// on a scalar load per atom and site.
 template <int VC>
 __global__ __launch_bounds__(kThreads) void k_potentials(PotArgs a) {
   __shared__ double4 s_at[kAtomChunk];
-  __shared__ double2 s_lj[kAtomChunk * VC];
+  __shared__ double4 s_lj[kAtomChunk * VC];
   const long ip0 = (long)blockIdx.x * kThreads + threadIdx.x;
   const bool live = ip0 < a.N;
   const long ip = live ? ip0 : a.N - 1;  // every thread stages atoms; only live ones store
@@ -39,9 +45,9 @@ __global__ __launch_bounds__(kThreads) void k_potentials(PotArgs a) {
   const double ry = (y - ny / 2) * a.dr[1];
   const double rz = (z - nz / 2) * a.dr[2];
   const double cut2 = 1.0e-2 * 1.0e-2;  // LJ distance floor 0.01 A (cal_LJ.cu:63)
-  double lj[VC];
+  double lj[VC], sel[VC], seq[VC];
 #pragma unroll
-  for (int v = 0; v < VC; ++v) lj[v] = 0.0;
+  for (int v = 0; v < VC; ++v) lj[v] = sel[v] = seq[v] = 0.0;
   double e = 0.0, f = 0.0, s = 0.0;
   const int V = a.V, nu = a.natu;
   for (int c0 = 0; c0 < nu; c0 += kAtomChunk) {
@@ -50,27 +56,48 @@ __global__ __launch_bounds__(kThreads) void k_potentials(PotArgs a) {
     for (int i = threadIdx.x; i < nc; i += kThreads) s_at[i] = a.atoms[c0 + i];
     for (int i = threadIdx.x; i < nc * VC; i += kThreads) {
       const int u = i / VC, v = i - u * VC;
-      s_lj[i] = v < V ? a.lj[v * nu + c0 + u] : make_double2(0.0, 0.0);
+      if (v < V) {
+        const double2 p = a.lj[v * nu + c0 + u];
+        s_lj[i] = make_double4(p.x, p.y, a.sigh[v * nu + c0 + u], 0.0);
+      } else {  // padding site: r >= 1e300 never holds, so its SE terms stay zero
+        s_lj[i] = make_double4(0.0, 0.0, 1.0e300, 0.0);
+      }
     }
     __syncthreads();
-    // Two atoms per step: their distance and LJ terms are independent
-    // chains (rsqrt, Newton steps, powers) that the scheduler interleaves;
-    // the Coulomb terms follow, atom by atom, behind their branch on r (an
-    // exp(-r) only below 38 A).  Each sum still adds the atoms in order.
-    auto geom = [&](int k, double *tl, double &ra, double &rinv) {
+    // Per atom: distance, LJ and SE terms, then the Coulomb terms behind
+    // their branch on r (an exp(-r) only below 38 A); every sum adds the
+    // atoms in order.  tl: the potential's LJ term (distance floored at
+    // 0.01 A); ts / tq: the SE's LJ and Coulomb terms (exact r, zero for
+    // r < sigma_uv / 2).
+    auto geom = [&](int k, double *tl, double *ts, double *tq, double &ra, double &rinv) {
       const double4 at = s_at[k];
       const double dx = rx - at.x, dy = ry - at.y, dz = rz - at.z;
       const double r2 = dx * dx + dy * dy + dz * dz;
       rinv = rsqrt_f64(fmax(r2, 1.0e-300));
-      const double ir2 = r2 < cut2 ? 1.0 / cut2 : rinv * rinv;
+      ra = r2 * rinv;
+      const double ir2 = rinv * rinv;
+      const double qr = at.w * rinv;
 #pragma unroll
       for (int v = 0; v < VC; ++v) {
-        const double2 p = s_lj[k * VC + v];
+        const double4 p = s_lj[k * VC + v];
         const double s2 = p.x * ir2;
         const double s6 = s2 * s2 * s2;
-        tl[v] = p.y * s6 * (s6 - 1.0);
+        double t = p.y * s6 * (s6 - 1.0);
+        tl[v] = t;
+        asm volatile("" : "+v"(t));  // computed for every lane, then selected: no divergent branch
+        const bool in = ra >= p.z;
+        ts[v] = in ? t : 0.0;
+        tq[v] = in ? qr : 0.0;
       }
-      ra = r2 * rinv;
+      if (r2 < cut2) {  // within 0.01 A of the atom (rare): the potential's floor
+#pragma unroll
+        for (int v = 0; v < VC; ++v) {
+          const double4 p = s_lj[k * VC + v];
+          const double s2 = p.x * (1.0 / cut2);
+          const double s6 = s2 * s2 * s2;
+          tl[v] = p.y * s6 * (s6 - 1.0);
+        }
+      }
     };
     auto coulomb = [&](int k, double ra, double rinv) {
       const double qa = s_at[k].w;
@@ -89,30 +116,29 @@ __global__ __launch_bounds__(kThreads) void k_potentials(PotArgs a) {
         s -= qa;
       }
     };
-    int k = 0;
-    for (; k + 1 < nc; k += 2) {
-      double la[VC], lb[VC], raa, rab, ria, rib;
-      geom(k, la, raa, ria);
-      geom(k + 1, lb, rab, rib);
+    // One atom per step: with the SE terms the per-atom work has enough
+    // independent chains, and two atoms per step (112 against 92 VGPRs)
+    // measured slower.
+    for (int k = 0; k < nc; ++k) {
+      double la[VC], sa[VC], qa[VC], raa, ria;
+      geom(k, la, sa, qa, raa, ria);
 #pragma unroll
-      for (int v = 0; v < VC; ++v) lj[v] += la[v];
-#pragma unroll
-      for (int v = 0; v < VC; ++v) lj[v] += lb[v];
-      coulomb(k, raa, ria);
-      coulomb(k + 1, rab, rib);
-    }
-    if (k < nc) {
-      double la[VC], raa, ria;
-      geom(k, la, raa, ria);
-#pragma unroll
-      for (int v = 0; v < VC; ++v) lj[v] += la[v];
+      for (int v = 0; v < VC; ++v) {
+        lj[v] += la[v];
+        sel[v] += sa[v];
+        seq[v] += qa[v];
+      }
       coulomb(k, raa, ria);
     }
   }
   if (!live) return;
 #pragma unroll
   for (int v = 0; v < VC; ++v)
-    if (v < V) a.ulj[(long)v * a.N + ip] = lj[v] * a.ikbt;
+    if (v < V) {
+      a.ulj[(long)v * a.N + ip] = lj[v] * a.ikbt;
+      a.sel[(long)v * a.N + ip] = sel[v];
+      a.seq[(long)v * a.N + ip] = seq[v];
+    }
   a.de[ip] = e * a.ubeta;
   a.fr[ip] = f * a.ubeta;
   a.sr[ip] = s * a.ubeta;

@@ -1,7 +1,8 @@
 // k_thermo.hip -- converged-field reductions on gfx950 (one-time, after the
 // solve): excess chemical potential densities (src-cuda/cal_exchem.cu:55-154),
 // the PMV sum of c = h - t (cal_pmv.cu:25-55), solute-solvent energy SE
-// (cal_euv.cu:35-76) and the cal_grad forces (cal_grad.cu:51-145).  All sums
+// (cal_euv.cu:35-76; total from the potentials pass's atom sums, per atom
+// for the .euv file) and the cal_grad forces (cal_grad.cu:51-145).  All sums
 // are FP64 with a fixed block order, so they are reproducible run to run.
 #include "rism_kernels.h"
 
@@ -72,99 +73,24 @@ __global__ __launch_bounds__(kThreads) void k_exchem(IterArgs<R, S> a, const dou
   }
 }
 
-// SE totals: sum over (u, v) of rho_v sum_r g 4 eps (s^12 - s^6) and
-// rho_v sum_r g q_u q_v cc / r, excluding r < sigma_uv / 2.  The atoms are
-// staged through LDS in chunks ((sigma_uv^2, 4 eps_uv, sigma_uv / 2) per
-// site, zero beyond V); the block walks the grid in block-uniform steps so
-// every thread reaches the staging barriers.
-template <int VC>
-__global__ __launch_bounds__(kThreads) void k_euv(int nx, int ny, int nz, double drx, double dry,
-                                                  double drz, int V, int natu, const double4 *atoms,
-                                                  const double2 *lj, const double *sigh,
-                                                  const double *qv, const double *rho,
-                                                  const double *g, double *partials) {
-  __shared__ double4 s_at[kAtomChunk];
-  __shared__ double4 s_p[kAtomChunk * VC];
-  const long N = (long)nx * ny * nz;
+// SE totals (cal_euv.cu:35-76): rho_v sum_r g_v sum_u 4 eps_uv (s^12 - s^6)
+// and rho_v q_v cc sum_r g_v sum_u q_u / r, both over r >= sigma_uv / 2.
+// The atom sums per point and site (sel, seq) come from the potentials
+// pass, so this is one weighted sum over the grid.  Points are walked in
+// block-uniform grid strides and reduced in a fixed order.
+__global__ __launch_bounds__(kThreads) void k_se_sum(long N, int V, const double *rho, const double *qv,
+                                                     const double *g, const double *sel, const double *seq,
+                                                     double *partials) {
   const double cc = 4.359744722207185e-18 * 0.5291771500001 * 6.02214076e23;
   double elj = 0.0, ees = 0.0;
   for (long b0 = (long)blockIdx.x * kThreads; b0 < N; b0 += (long)gridDim.x * kThreads) {
-    const long ip0 = b0 + threadIdx.x;
-    const bool live = ip0 < N;
-    const long ip = live ? ip0 : N - 1;
-    const int x = (int)(ip % nx), y = (int)((ip / nx) % ny), z = (int)(ip / ((long)nx * ny));
-    const double rx = (x - nx / 2) * drx, ry = (y - ny / 2) * dry, rz = (z - nz / 2) * drz;
-    // per-site sums over atoms; the point's factors rho_v g_v (q_v cc) are applied once
-    double alj[VC], aes[VC];
-#pragma unroll
-    for (int v = 0; v < VC; ++v) alj[v] = aes[v] = 0.0;
-    for (int c0 = 0; c0 < natu; c0 += kAtomChunk) {
-      const int nc = min(kAtomChunk, natu - c0);
-      __syncthreads();
-      for (int i = threadIdx.x; i < nc; i += kThreads) s_at[i] = atoms[c0 + i];
-      for (int i = threadIdx.x; i < nc * VC; i += kThreads) {
-        const int u = i / VC, v = i - u * VC;
-        if (v < V) {
-          const double2 p = lj[v * natu + c0 + u];
-          s_p[i] = make_double4(p.x, p.y, sigh[v * natu + c0 + u], 0.0);
-        } else {  // padding site: r1 >= 1e300 never holds, so it is excluded everywhere
-          s_p[i] = make_double4(0.0, 0.0, 1.0e300, 0.0);
-        }
-      }
-      __syncthreads();
-      // two atoms per step: independent term chains, added in atom order
-      auto terms = [&](int k, double *tl, double *te) {
-        const double4 at = s_at[k];
-        const double dx = rx - at.x, dy = ry - at.y, dz = rz - at.z;
-        const double r2 = dx * dx + dy * dy + dz * dz;
-        const double ir1 = rsqrt_f64(fmax(r2, 1.0e-300));
-        const double r1 = r2 * ir1, ir2 = ir1 * ir1;
-        const double qr = at.w * ir1;
-#pragma unroll
-        for (int v = 0; v < VC; ++v) {
-          const double4 p = s_p[k * VC + v];  // (sigma^2, 4 eps, sigma / 2)
-          const bool in = r1 >= p.z;          // r < sigma_uv / 2 is excluded
-          const double s2 = p.x * ir2, s6 = s2 * s2 * s2;
-          double t = p.y * s6 * (s6 - 1.0);
-          asm volatile("" : "+v"(t));  // computed for every lane, then selected: no divergent branch
-          tl[v] = in ? t : 0.0;
-          te[v] = in ? qr : 0.0;
-        }
-      };
-      int k = 0;
-      for (; k + 1 < nc; k += 2) {
-        double la[VC], ea[VC], lb[VC], eb[VC];
-        terms(k, la, ea);
-        terms(k + 1, lb, eb);
-#pragma unroll
-        for (int v = 0; v < VC; ++v) {
-          alj[v] += la[v];
-          aes[v] += ea[v];
-        }
-#pragma unroll
-        for (int v = 0; v < VC; ++v) {
-          alj[v] += lb[v];
-          aes[v] += eb[v];
-        }
-      }
-      if (k < nc) {
-        double la[VC], ea[VC];
-        terms(k, la, ea);
-#pragma unroll
-        for (int v = 0; v < VC; ++v) {
-          alj[v] += la[v];
-          aes[v] += ea[v];
-        }
-      }
-    }
-    if (live) {
-#pragma unroll
-      for (int v = 0; v < VC; ++v) {
-        if (v < V) {
-          const double w = rho[v] * g[(long)v * N + ip];
-          elj += w * alj[v];
-          ees += w * qv[v] * cc * aes[v];
-        }
+    const long ip = b0 + threadIdx.x;
+    if (ip < N) {
+      for (int v = 0; v < V; ++v) {
+        const long iv = (long)v * N + ip;
+        const double w = rho[v] * g[iv];
+        elj += w * sel[iv];
+        ees += w * qv[v] * cc * seq[iv];
       }
     }
   }
@@ -354,18 +280,9 @@ hipError_t launch_exchem(const IterArgs<R, S> &a, const double *frd, double *par
   return hipGetLastError();
 }
 
-hipError_t launch_euv(int nx, int ny, int nz, const double *dr, int V, int natu, const double4 *atoms,
-                      const double2 *lj, const double *sigh, const double *qv, const double *rho,
-                      const double *g, double *partials, int nblk, hipStream_t st) {
-  if (V <= 2)
-    hipLaunchKernelGGL(k_euv<2>, dim3(nblk), dim3(kThreads), 0, st, nx, ny, nz, dr[0], dr[1], dr[2], V, natu,
-                       atoms, lj, sigh, qv, rho, g, partials);
-  else if (V <= 4)
-    hipLaunchKernelGGL(k_euv<4>, dim3(nblk), dim3(kThreads), 0, st, nx, ny, nz, dr[0], dr[1], dr[2], V, natu,
-                       atoms, lj, sigh, qv, rho, g, partials);
-  else
-    hipLaunchKernelGGL(k_euv<8>, dim3(nblk), dim3(kThreads), 0, st, nx, ny, nz, dr[0], dr[1], dr[2], V, natu,
-                       atoms, lj, sigh, qv, rho, g, partials);
+hipError_t launch_se_sum(long N, int V, const double *rho, const double *qv, const double *g, const double *sel,
+                         const double *seq, double *partials, int nblk, hipStream_t st) {
+  hipLaunchKernelGGL(k_se_sum, dim3(nblk), dim3(kThreads), 0, st, N, V, rho, qv, g, sel, seq, partials);
   return hipGetLastError();
 }
 

@@ -187,8 +187,12 @@ struct PotArgs {
   double box[3];
   const double4 *atoms;  // x, y, z, q (centred coordinates)
   const double2 *lj;     // [V][natu]: (sigma_uv^2, 4*eps_uv)
+  const double *sigh;    // [V][natu]: sigma_uv / 2 (the SE's exclusion radius)
   double ikbt, ubeta;
   double *ulj, *de, *fr, *sr;  // sr = de - fr
+  // [V][N] per-site sums over the atoms for the SE (cal_euv.cu:35-76):
+  // 4 eps (s^12 - s^6) and q_u / r, both over r >= sigma_uv / 2
+  double *sel, *seq;
 };
 hipError_t launch_potentials(const PotArgs &a, hipStream_t st);
 hipError_t launch_fk_half(int nx, int ny, int nz, int natu, const double4 *atoms, const double *box,
@@ -212,9 +216,8 @@ hipError_t launch_gfield(const IterArgs<R, S> &a, double *g, hipStream_t st);
 template <typename R, typename S>
 hipError_t launch_exchem(const IterArgs<R, S> &a, const double *frd, double *partials, int nblk,
                          hipStream_t st);
-hipError_t launch_euv(int nx, int ny, int nz, const double *dr, int V, int natu, const double4 *atoms,
-                      const double2 *lj, const double *sigh, const double *qv, const double *rho,
-                      const double *g, double *partials, int nblk, hipStream_t st);
+hipError_t launch_se_sum(long N, int V, const double *rho, const double *qv, const double *g, const double *sel,
+                         const double *seq, double *partials, int nblk, hipStream_t st);
 // RMDFT (cal_rmdft.cu): grid and spline tables (coefficients (y, c0, c1, c2)
 // per segment)
 struct RmdftGrid {

@@ -111,6 +111,7 @@ struct rism_handle {
   // device memory
   std::vector<DevBuf> bufs;
   double *ulj = nullptr, *de = nullptr, *fr = nullptr, *sr = nullptr;
+  double *sel = nullptr, *seq = nullptr;  // [V][N] SE atom sums from the potentials pass
   double2 *fkh = nullptr, *phase = nullptr;
   float *cref = nullptr;    // mixed: reference c_s of the refinement
   double *ts = nullptr;
@@ -818,8 +819,7 @@ template <typename R> int thermo_impl(rism_handle *h, rism_xmu *x) {
   std::vector<double> tot(4 * V, 0.0);
   for (int b = 0; b < nblk; ++b)
     for (int k = 0; k < 4 * V; ++k) tot[k] += part[(size_t)b * 4 * V + k];
-  PROF(h, 9, launch_euv(h->nx, h->ny, h->nz, h->dr, V, h->natu, h->atoms, h->lj, h->sigh, h->d_qv, h->d_rho,
-                        h->gfield, h->partials, nblk, h->stream));
+  PROF(h, 9, launch_se_sum(h->N, V, h->d_rho, h->d_qv, h->gfield, h->sel, h->seq, h->partials, nblk, h->stream));
   std::vector<double> pe((size_t)nblk * 2);
   HIP_OK(hipMemcpyAsync(pe.data(), h->partials, pe.size() * sizeof(double), hipMemcpyDeviceToHost,
                         h->stream));
@@ -992,7 +992,8 @@ int create_impl(const rism_grid *grid, const rism_solvent *sv, const rism_solute
   const size_t VN = (size_t)h->V * h->N;
   const size_t nh = (size_t)h->nx * (h->ny / 2) * h->nz;
   int rc = 0;
-  if ((rc = dalloc_t(h, &h->ulj, VN)) || (rc = dalloc_t(h, &h->de, h->N)) || (rc = dalloc_t(h, &h->fr, h->N)) ||
+  if ((rc = dalloc_t(h, &h->ulj, VN)) || (rc = dalloc_t(h, &h->sel, VN)) || (rc = dalloc_t(h, &h->seq, VN)) ||
+      (rc = dalloc_t(h, &h->de, h->N)) || (rc = dalloc_t(h, &h->fr, h->N)) ||
       (rc = dalloc_t(h, &h->sr, h->N)) || (rc = dalloc_t(h, &h->fkh, nh)) ||
       (rc = dalloc_t(h, &h->phase, (size_t)std::max(1, h->natu) * (h->nx + h->ny + h->nz))) ||
       (rc = dalloc_t(h, &h->ts, VN)) || (rc = dalloc(h, &h->res, VN * h->qsz)) ||
@@ -1082,6 +1083,9 @@ int rism_build_potentials(rism_handle *h) {
   for (int k = 0; k < 3; ++k) { a.dr[k] = h->dr[k]; a.box[k] = h->box[k]; }
   a.atoms = h->atoms;
   a.lj = h->lj;
+  a.sigh = h->sigh;
+  a.sel = h->sel;
+  a.seq = h->seq;
   a.ikbt = 1.0 / (kAvogadro * kBoltzmann * h->temper);
   a.ubeta = kHartree * kBohr / (kBoltzmann * h->temper);
   a.ulj = h->ulj; a.de = h->de; a.fr = h->fr; a.sr = h->sr;

@@ -328,10 +328,10 @@ int rism_xvk_read_sites(const char *path, rism_chi_site *sites, int maxsite, int
 #define RISM_K_FFT_Y_INV 3    /* inverse y FFT */
 #define RISM_K_RESID 4        /* inverse x FFT + residual + Anderson dots */
 #define RISM_K_SOLVE 5        /* single-workgroup reduction + Anderson solve */
-#define RISM_K_POT 6          /* LJ + Coulomb grid potentials */
+#define RISM_K_POT 6          /* LJ + Coulomb grid potentials, with the SE atom sums */
 #define RISM_K_FK 7           /* long-range k-space term */
 #define RISM_K_EXCHEM 8       /* free-energy / PMV sums */
-#define RISM_K_EUV 9          /* solute-solvent energy */
+#define RISM_K_EUV 9          /* solute-solvent energy: weighted sum over the grid */
 #define RISM_K_REFRESH 10     /* mixed precision: FP64 passes of a refinement step */
 #define RISM_NKERNEL 11
 /* enable (1) or disable (0) HIP-event timing of every launch; resets the
