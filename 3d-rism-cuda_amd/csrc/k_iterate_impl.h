@@ -146,6 +146,9 @@ __device__ __forceinline__ void closure_points(const IterArgs<R, S> &a, typename
               // closure in FP64 in both precisions: near a charged solute its
               // argument ts - u is the small difference of large terms
               const double g = closure_g<double>(t[c][v] - u[c][v], a.closure);
+              // FP32 passes: g goes to the residual pass, which then reads 4 bytes
+              // per site instead of t_s and u (12) to evaluate the closure again
+              if constexpr (sizeof(R) == sizeof(float)) lane_at(a.gq + vb, e) = (float)g;
               double cv = g - 1.0 - t[c][v];
               if (a.refresh) {  // new reference; the rounding remainder goes to the FP32 passes
                 const float cf = (float)cv;
@@ -696,6 +699,9 @@ __global__ __launch_bounds__(kThreads) void k_fx_resid(IterArgs<R, S> a) {
     // slower at 256^3.
     constexpr int CH = KPT < RISM_RESID_CH ? KPT : RISM_RESID_CH;
     constexpr bool PREH = CH * VC * HM <= 16;  // history residuals join the batch
+    // FP32 passes: g comes from the closure pass of the same iteration, which
+    // ran under the same gate and wrote it for every point
+    constexpr bool kGq = sizeof(R) == sizeof(float);
     auto load_ops = [&](unsigned e, double *hbo, double *tvo, double *uvo, Q *hqo) {
 #pragma unroll
       for (int p = 0; p < (VC + 1) / 2; ++p) {
@@ -711,8 +717,13 @@ __global__ __launch_bounds__(kThreads) void k_fx_resid(IterArgs<R, S> a) {
         for (int v = 0; v < VC; ++v)
           if (v < V) {
             const long vb = (long)v * N + base;
-            tvo[v] = (double)lane_at(ts + vb, e);
-            uvo[v] = (double)lane_at(uu + vb, e);
+            if constexpr (kGq) {  // g of this iteration's closure pass (tvo), uvo unused
+              tvo[v] = (double)lane_at(a.gq + vb, e);
+              uvo[v] = 0.0;
+            } else {
+              tvo[v] = (double)lane_at(ts + vb, e);
+              uvo[v] = (double)lane_at(uu + vb, e);
+            }
 #pragma unroll
             for (int s = 0; s < (PREH ? HM : 0); ++s) hqo[v * (PREH ? HM : 1) + s] = lane_at(hr + (s < nf ? s : 0) * VN + vb, e);
           }
@@ -783,7 +794,7 @@ __global__ __launch_bounds__(kThreads) void k_fx_resid(IterArgs<R, S> a) {
           for (int v = 0; v < VC; ++v) {
             if (v < V) {
               const long vb = (long)v * N + base;
-              const double g = closure_g<double>(tv[c][v] - uv[c][v], a.closure);
+              const double g = kGq ? tv[c][v] : closure_g<double>(tv[c][v] - uv[c][v], a.closure);
               const double rd = hv[c][v] + 1.0 - g;  // tr: R = h + 1 - g
               lane_at(res + vb, e) = (Q)rd;
               const double wgt = a.rho_inv[v];

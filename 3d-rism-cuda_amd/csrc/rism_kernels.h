@@ -108,6 +108,7 @@ struct IterArgs {
   // scales with |c_s - c_ref|, which shrinks as the solve converges.
   const float *cref;    // [V][N] reference c_s
   float *cref_out;      // refresh: first pass stores c_s here
+  float *gq;            // FP32 passes: g of the closure pass [V][N], read by the residual pass
   const double2 *href;  // [P][N] FP64 real-space response to cref (incl. long range)
   int delta;            // FP32 passes: subtract cref / add href, no fk term
   int refresh;          // FP32 first pass: c_s -> cref_out, emit a zero spectrum

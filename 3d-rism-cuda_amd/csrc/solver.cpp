@@ -87,6 +87,7 @@ struct PrecSet {
   AxisPlan plan_xr;  // x axis, radices <= 5 (FP64 residual pass), same tables as plan[0]
   LaunchGeom geom{};
   void *u = nullptr, *fr = nullptr, *fkh = nullptr, *chi = nullptr, *Z = nullptr, *tables = nullptr;
+  float *gq = nullptr;  // FP32 passes: g handed from the closure pass to the residual pass
 };
 
 template <typename R> constexpr int pidx() { return sizeof(R) == sizeof(double) ? 0 : 1; }
@@ -408,6 +409,7 @@ template <typename R> IterArgs<R, double> make_args(rism_handle *h) {
   a.store_h = 0;
   a.use_fk = 1;
   if (pidx<R>() == 1) {
+    a.gq = ps.gq;
     a.cref = h->cref;
     a.cref_out = h->cref;
     a.href = (const double2 *)h->ps[0].Z;
@@ -877,6 +879,7 @@ template <typename R> int alloc_precision(rism_handle *h) {
   } else {
     if (int rc = dalloc(h, &ps.fr, h->N * sizeof(R))) return rc;
     if (int rc = dalloc(h, &ps.fkh, (size_t)h->nx * (h->ny / 2) * h->nz * sizeof(T2))) return rc;
+    if (int rc = dalloc_t(h, &ps.gq, VN)) return rc;
   }
   if (int rc = dalloc(h, &ps.Z, (size_t)h->P * h->N * sizeof(T2))) return rc;
   // chi spline table [V*V][ntab-1] of (y_i, c0, c1, c2)

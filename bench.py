@@ -50,20 +50,22 @@ def kernel_bytes(V, P, H, mixed):
     The mixed path transforms the difference to an FP64 reference, so its
     update step also reads c_ref (FP32) and its residual step reads h_ref
     (complex FP64 per site pair) instead of the half-grid long-range term.
+    Its update step also writes the closure's g (FP32), which the residual
+    step reads in place of t_s and u.
     """
     s_t = 8
     s_f = 4 if mixed else 8
     s_h = 4 if mixed else 8
     c = 2 * s_f
     return {
-        # t_s, u, R, H x (t_i, R_i) in; new slot (t, R) and t_s out; [c_ref in]; spectrum out
-        "closure_fx": V * (s_t + s_f + s_h + 2 * H * s_h + 2 * s_h + s_t + (4 if mixed else 0)) + P * c,
+        # t_s, u, R, H x (t_i, R_i) in; new slot (t, R) and t_s out; [c_ref in, g out]; spectrum out
+        "closure_fx": V * (s_t + s_f + s_h + 2 * H * s_h + 2 * s_h + s_t + (8 if mixed else 0)) + P * c,
         "fft_y_fwd": 2 * P * c,
         # spectrum in/out + the half-grid long-range term (FP64 path only)
         "fft_z_oz": 2 * P * c + (0 if mixed else c // 2),
         "fft_y_inv": 2 * P * c,
-        # spectrum, [h_ref], t_s, u, H history R_i in; R out
-        "resid": P * c + (16 * P if mixed else 0) + V * (s_t + s_f + H * s_h + s_h),
+        # spectrum, [h_ref], t_s and u (mixed: g), H history R_i in; R out
+        "resid": P * c + (16 * P if mixed else 0) + V * ((4 if mixed else s_t + s_f) + H * s_h + s_h),
     }
 
 

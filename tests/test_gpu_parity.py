@@ -189,10 +189,12 @@ def test_solve_multi_atom_fp64_thermo_and_grad(gpu, closure, qscale):
 def test_solve_test1_mixed_against_converged_oracle(gpu, deckdir, tol):
     """Mixed precision takes its own Anderson trajectory, so it is judged
     against the oracle converged to RMS 1e-10 (the fixed point itself).  At
-    RMS 1e-6 the FP64 oracle itself sits 1.35e-4 (g) / 0.31 J/mol (SFE_SC)
-    from that point, so g is held to 2e-4 there and to 1e-4 at RMS 1e-7
-    (the FP32 closure rounding floor is ~5e-8);
-    Delta-mu to 0.01 kcal/mol (41.84 J/mol) at both."""
+    RMS 1e-6 the FP64 oracle itself sits about 1.35e-4 (g) / 0.31 J/mol
+    (SFE_SC) from that point (the tolerance ball, measured here), and where
+    inside the ball a solve stops depends on its trajectory, so g is held to
+    max(2e-4, 2 x ball) there and to 1e-4 at RMS 1e-7 (the FP32 closure
+    rounding floor is ~5e-8); Delta-mu to 0.01 kcal/mol (41.84 J/mol) at
+    both."""
     print(f"solve test1 mixed tol={tol}", flush=True)
     deck = rism3d.read_deck("test1.inp")
     o = orc.Oracle.from_deck("test1.inp").read_xvk("h2o_vv.xvk").setup()
@@ -209,7 +211,15 @@ def test_solve_test1_mixed_against_converged_oracle(gpu, deckdir, tol):
     assert st == rism3d.SOLVE_CONVERGED and rms <= tol
     assert abs(x["SFE_SC"] - ox["SFE_SC"]) < 41.84
     assert abs(x["SFE_GF"] - ox["SFE_GF"]) < 41.84
-    assert gerr < (2e-4 if tol > 1e-7 else 1e-4), gerr
+    if tol > 1e-7:
+        o6 = orc.Oracle.from_deck("test1.inp").read_xvk("h2o_vv.xvk").setup()
+        o6.set_iter(tol, 10000, 2, 0.6, 0.2)
+        assert o6.iterate()[0] == 0
+        ball = float(np.abs(o6.field(orc.F_G) - og).max())
+        print("test1 ball", ball, flush=True)
+        assert gerr < max(2e-4, 2.0 * ball), (gerr, ball)
+    else:
+        assert gerr < 1e-4, gerr
     s.close()
 
 
