@@ -753,10 +753,10 @@ __global__ __launch_bounds__(kThreads) void k_fx_resid(IterArgs<R, S> a) {
     int tidr = tid;
     asm volatile("" : "+v"(tidr));
     // The next group's spectrum is loaded now, so its latency overlaps this
-    // group's point loop instead of stalling the next transform.  After the
-    // last group the load repeats this group's (discarded).  Registers: the
-    // point loop stays below the transform's peak with it.
-    if (RISM_RESID_ZPF) load_z(grp + (int)gridDim.x < groups ? grp + (int)gridDim.x : grp, tidr);
+    // group's point loop instead of stalling the next transform (a uniform
+    // branch: after a workgroup's last group nothing is loaded).  Registers:
+    // the point loop stays below the transform's peak with it.
+    if (RISM_RESID_ZPF && grp + (int)gridDim.x < groups) load_z(grp + (int)gridDim.x, tidr);
 #pragma unroll
     for (int k0 = 0; k0 < KPT; k0 += CH) {
       // batch of CH points: every load first (clamped index), then the arithmetic

@@ -218,12 +218,66 @@ hipError_t hcopy(rism_handle *h, void *dst, const void *src, size_t bytes, hipMe
   return e == hipSuccess ? hipStreamSynchronize(h->stream) : e;
 }
 
+// Device buffers of destroyed handles are kept for reuse, per device and
+// exact size, up to RISM_POOL_BYTES in all (default 8 GiB; 0 disables):
+// hipFree synchronises the whole device, and a batch that creates one handle
+// per solute would otherwise stall every other worker's stream at each
+// release.  An allocation that fails first returns the cache to the device.
+// Guarded by g_device_mu.
+struct CachedBuf {
+  int device;
+  size_t bytes;
+  void *p;
+};
+std::vector<CachedBuf> g_cache;
+size_t g_cache_bytes = 0;
+
+long env_long(const char *name, long dflt);
+
+size_t cache_cap() {
+  static const size_t cap = (size_t)std::max(0L, env_long("RISM_POOL_BYTES", 8L << 30));
+  return cap;
+}
+
+void cache_flush_locked(int device) {
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  for (size_t i = 0; i < g_cache.size();) {
+    if (g_cache[i].device == device) {
+      (void)hipSetDevice(device);
+      (void)hipFree(g_cache[i].p);
+      g_cache_bytes -= g_cache[i].bytes;
+      g_cache[i] = g_cache.back();
+      g_cache.pop_back();
+    } else {
+      ++i;
+    }
+  }
+  (void)hipSetDevice(cur);
+}
+
 int dalloc(rism_handle *h, void **p, size_t bytes) {
   if (bytes == 0) bytes = 16;
-  hipError_t e;
+  hipError_t e = hipSuccess;
   {
     std::lock_guard<std::mutex> lk(g_device_mu);
-    e = hipMalloc(p, bytes);
+    *p = nullptr;
+    for (size_t i = 0; i < g_cache.size(); ++i)
+      if (g_cache[i].device == h->device && g_cache[i].bytes == bytes) {
+        *p = g_cache[i].p;
+        g_cache_bytes -= bytes;
+        g_cache[i] = g_cache.back();
+        g_cache.pop_back();
+        break;
+      }
+    if (!*p) {
+      e = hipMalloc(p, bytes);
+      if (e != hipSuccess && !g_cache.empty()) {
+        (void)hipGetLastError();
+        cache_flush_locked(h->device);
+        e = hipMalloc(p, bytes);
+      }
+    }
   }
   if (e != hipSuccess) {
     set_error("device allocation of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
@@ -995,7 +1049,7 @@ int create_impl(const rism_grid *grid, const rism_solvent *sv, const rism_solute
   const size_t VN = (size_t)h->V * h->N;
   const size_t nh = (size_t)h->nx * (h->ny / 2) * h->nz;
   int rc = 0;
-  if ((rc = dalloc_t(h, &h->ulj, VN)) || (rc = dalloc_t(h, &h->sel, VN)) || (rc = dalloc_t(h, &h->seq, VN)) ||
+  if ((rc = dalloc_t(h, &h->ulj, VN)) || (rc = dalloc_t(h, &h->sel, 2 * VN)) ||
       (rc = dalloc_t(h, &h->de, h->N)) || (rc = dalloc_t(h, &h->fr, h->N)) ||
       (rc = dalloc_t(h, &h->sr, h->N)) || (rc = dalloc_t(h, &h->fkh, nh)) ||
       (rc = dalloc_t(h, &h->phase, (size_t)std::max(1, h->natu) * (h->nx + h->ny + h->nz))) ||
@@ -1008,6 +1062,7 @@ int create_impl(const rism_grid *grid, const rism_solvent *sv, const rism_solute
       (rc = dalloc_t(h, &h->sigh, (size_t)h->V * std::max(1, h->natu))) || (rc = dalloc_t(h, &h->d_qv, h->V)) ||
       (rc = dalloc_t(h, &h->d_rho, h->V)))
     return rc;
+  h->seq = h->sel + VN;  // the two SE atom sums share one allocation
   std::memset(h->ctl_host, 0, sizeof(Ctl));
   if ((rc = alloc_precision<double>(h))) return rc;
   if (o->precision == RISM_PREC_MIXED) {
@@ -1823,7 +1878,14 @@ void rism_destroy(rism_handle *h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   std::lock_guard<std::mutex> lk(g_device_mu);
   drop_graphs(h);
-  for (auto &b : h->bufs) (void)hipFree(b.p);
+  for (auto &b : h->bufs) {
+    if (g_cache_bytes + b.bytes <= cache_cap()) {
+      g_cache.push_back({h->device, b.bytes, b.p});
+      g_cache_bytes += b.bytes;
+    } else {
+      (void)hipFree(b.p);
+    }
+  }
   for (auto &e : h->ev_pool) (void)hipEventDestroy(e);
   if (h->ctl_host) (void)hipHostFree(h->ctl_host);
   if (h->stream) (void)hipStreamDestroy(h->stream);

@@ -156,3 +156,30 @@ def test_autotune_keeps_the_solution(gpu, prec):
     else:
         assert abs(x1 - x0) < 41.84
         assert np.abs(f1 - f0).max() < 2e-4
+
+
+@pytest.mark.parametrize("prec", [rism3d.FP64, rism3d.MIXED])
+def test_reused_buffers_leave_the_solution_unchanged(gpu, prec):
+    """Destroyed handles return their device buffers to a cache that later
+    handles of the same shape draw from, with the old contents in place.
+    Solving A, then another solute B, then A again (on B's buffers) gives
+    A's answer bit for bit."""
+    print(f"buffer reuse prec={prec}", flush=True)
+    svf, n, box, natom, nhist = CASES["water_H5"]
+    sv = rism3d.read_xvk(example(svf))
+    out = []
+    for seed in (11, 23, 11):
+        sig, eps, q, xyz = _atoms(natom, seed=seed)
+        s = rism3d.Solver(n, box, sv, rism3d.Solute(sig, eps, q, xyz), rism3d.Options(precision=prec))
+        s.build_potentials()
+        st, steps, rms = s.solve()
+        assert st == rism3d.SOLVE_CONVERGED
+        out.append((steps, s.thermo(), _fields_g(s, sv.nsite, n)))
+        s.close()
+    (s0, x0, g0), (_, xb, _), (s2, x2, g2) = out
+    print("reuse", s0, s2, x0["SFE_SC"], xb["SFE_SC"], x2["SFE_SC"], flush=True)
+    assert xb["SFE_SC"] != x0["SFE_SC"]  # B is a different system
+    assert s2 == s0
+    for k in ("SFE_SC", "SFE_GF", "PMV", "SE"):
+        assert x2[k] == x0[k], k
+    assert np.array_equal(g2, g0)
