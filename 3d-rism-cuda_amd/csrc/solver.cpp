@@ -239,6 +239,17 @@ size_t cache_cap() {
   return cap;
 }
 
+// A destroyed handle's stream (synchronised, graphs dropped) and pinned
+// control block are kept for the next handle on the same device, up to 64:
+// creating and destroying them also synchronises.  Guarded by g_device_mu.
+struct SpareCtx {
+  int device;
+  hipStream_t stream;
+  Ctl *ctl_host;
+};
+std::vector<SpareCtx> g_spare;
+constexpr size_t kMaxSpare = 64;
+
 void cache_flush_locked(int device) {
   int cur = 0;
   (void)hipGetDevice(&cur);
@@ -1006,8 +1017,16 @@ int create_impl(const rism_grid *grid, const rism_solvent *sv, const rism_solute
   HIP_OK(hipSetDevice(device));
   {
     std::lock_guard<std::mutex> lk(g_device_mu);
-    HIP_OK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-    HIP_OK(hipHostMalloc((void **)&h->ctl_host, sizeof(Ctl), hipHostMallocDefault));
+    for (size_t i = 0; i < g_spare.size(); ++i)
+      if (g_spare[i].device == device) {
+        h->stream = g_spare[i].stream;
+        h->ctl_host = g_spare[i].ctl_host;
+        g_spare[i] = g_spare.back();
+        g_spare.pop_back();
+        break;
+      }
+    if (!h->stream) HIP_OK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    if (!h->ctl_host) HIP_OK(hipHostMalloc((void **)&h->ctl_host, sizeof(Ctl), hipHostMallocDefault));
   }
   h->nx = grid->n[0]; h->ny = grid->n[1]; h->nz = grid->n[2];
   h->N = (long)h->nx * h->ny * h->nz;
@@ -1887,8 +1906,12 @@ void rism_destroy(rism_handle *h) {
     }
   }
   for (auto &e : h->ev_pool) (void)hipEventDestroy(e);
-  if (h->ctl_host) (void)hipHostFree(h->ctl_host);
-  if (h->stream) (void)hipStreamDestroy(h->stream);
+  if (h->stream && h->ctl_host && g_spare.size() < kMaxSpare) {
+    g_spare.push_back({h->device, h->stream, h->ctl_host});
+  } else {
+    if (h->ctl_host) (void)hipHostFree(h->ctl_host);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+  }
   delete h;
 }
 
