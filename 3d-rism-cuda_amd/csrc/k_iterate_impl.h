@@ -232,7 +232,10 @@ __global__ __launch_bounds__(kThreads) void k_closure_fx(IterArgs<R, S> a) {
   T2 *ltw = lds + P * L * ls;
   stage_tables(ltw, a.px);
   __syncthreads();
-  fft_lines<EPT, false>(lds, P * L, ls, a.px, ltw);
+  // radix-4 plan: at about 1024 elements per workgroup a radix-16 stage has
+  // work for one wave in four; four radix-4 stages keep every wave busy
+  // (measured 0-1% faster, and no radix-8/16 stage is compiled in)
+  fft_lines<EPT, false, 5>(lds, P * L, ls, a.pxr, ltw);
   const R dv = (R)a.dv;
   T2 *__restrict__ Z = a.Z;
 #pragma unroll

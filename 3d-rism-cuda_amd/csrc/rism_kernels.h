@@ -95,7 +95,7 @@ struct IterArgs {
   double *partials;
   int nv;           // accumulators per block in the residual pass
   AxisPlan px, py, pz;
-  AxisPlan pxr;     // x plan with radices <= 5 (FP64 residual pass: smaller register peak)
+  AxisPlan pxr;     // x plan with radices <= 5 (closure pass, FP64 residual pass)
   int Lx;           // x lines per workgroup of the residual pass
   int Lc;           // x lines per workgroup of the closure/update pass
   int Ty, Tz;       // tile widths (consecutive x) of the y / z passes

@@ -84,7 +84,7 @@ int largest_divisor_le(long n, long cap) {
 struct PrecSet {
   bool ready = false;
   AxisPlan plan[3];
-  AxisPlan plan_xr;  // x axis, radices <= 5 (FP64 residual pass), same tables as plan[0]
+  AxisPlan plan_xr;  // x axis, radices <= 5 (closure pass, FP64 residual pass), same tables as plan[0]
   LaunchGeom geom{};
   void *u = nullptr, *fr = nullptr, *fkh = nullptr, *chi = nullptr, *Z = nullptr, *tables = nullptr;
   float *gq = nullptr;  // FP32 passes: g handed from the closure pass to the residual pass
@@ -359,8 +359,9 @@ template <typename R> int build_plans(rism_handle *h) {
     p.pre = dt + off + n;
     off += 2 * (size_t)n;
   }
-  // the FP64 residual pass's x plan: radix-4 stages at most (that kernel
-  // compiles no radix-8/16 stage), on the x axis's tables
+  // the closure pass's and the FP64 residual pass's x plan: radix-4 stages
+  // at most (those kernels compile no radix-8/16 stage), on the x axis's
+  // tables
   if (int rc = fill(ps.plan_xr, h->nx, 2)) return rc;
   ps.plan_xr.tw = ps.plan[0].tw;
   ps.plan_xr.pre = ps.plan[0].pre;
