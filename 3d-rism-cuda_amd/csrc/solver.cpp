@@ -1916,6 +1916,27 @@ void rism_destroy(rism_handle *h) {
   delete h;
 }
 
+long rism_release_cache(void) {
+  std::lock_guard<std::mutex> lk(g_device_mu);
+  const long freed = (long)g_cache_bytes;
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  for (auto &c : g_cache) {
+    (void)hipSetDevice(c.device);
+    (void)hipFree(c.p);
+  }
+  g_cache.clear();
+  g_cache_bytes = 0;
+  for (auto &sp : g_spare) {
+    (void)hipSetDevice(sp.device);
+    (void)hipHostFree(sp.ctl_host);
+    (void)hipStreamDestroy(sp.stream);
+  }
+  g_spare.clear();
+  (void)hipSetDevice(cur);
+  return freed;
+}
+
 int rism_profile(rism_handle *h, int enable) {
   if (!h) { set_error("null handle"); return RISM_ERR_ARG; }
   HIP_OK(hipSetDevice(h->device));

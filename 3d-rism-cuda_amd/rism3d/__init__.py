@@ -34,7 +34,7 @@ EXPORTED = [
     "rism_last_error", "rism_version", "rism_device_count", "rism_deck_read", "rism_deck_free",
     "rism_deck_views", "rism_xvk_read", "rism_create", "rism_build_potentials", "rism_solve",
     "rism_thermo", "rism_grad", "rism_get_field", "rism_set_t", "rism_save_tuv", "rism_load_tuv",
-    "rism_write_xmu", "rism_write_gra", "rism_stats", "rism_rms_history", "rism_destroy",
+    "rism_write_xmu", "rism_write_gra", "rism_stats", "rism_rms_history", "rism_destroy", "rism_release_cache",
     "rism_fft3d", "rism_calculate", "rism_profile", "rism_kernel_times", "rism_geometry",
     "rism_adjoint_grad", "rism_batch_solve", "rism_chi_default_opts", "rism_chi_nreduced",
     "rism_chi_solve", "rism_write_xvk", "rism_xvk_read_sites", "rism_stream_copy",
@@ -159,6 +159,8 @@ def lib():
         L.rism_rms_history.argtypes = [C.c_void_p, dp, C.c_int]
         L.rism_destroy.argtypes = [C.c_void_p]
         L.rism_destroy.restype = None
+        L.rism_release_cache.argtypes = []
+        L.rism_release_cache.restype = C.c_long
         L.rism_fft3d.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
         L.rism_calculate.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.rism_adjoint_grad.argtypes = [C.c_void_p, C.c_double, dp, dp, dp, dp, C.POINTER(C.c_int)]
@@ -348,6 +350,12 @@ def stream_copy(bytes_: int = 1 << 30, reps: int = 20, device: int = 0):
     g, u = C.c_double(), C.c_double()
     _check(lib().rism_stream_copy(device, bytes_, reps, C.byref(g), C.byref(u)), "rism_stream_copy")
     return g.value, u.value
+
+
+def release_cache() -> int:
+    """Free the device buffers, streams and pinned blocks that destroyed
+    handles left for reuse (rism_release_cache); returns the bytes freed."""
+    return int(lib().rism_release_cache())
 
 
 def solvent_model(path: str):

@@ -257,7 +257,16 @@ int rism_stats(rism_handle *h, int *steps, double *rms, double *t_setup,
                double *t_solve, double *t_thermo);
 /* rms after each step of the last solve (n <= maxstep); returns count */
 int rism_rms_history(rism_handle *h, double *out, int n);
+/* release the handle.  Its device buffers, stream and pinned control block
+   go to a process-wide cache that later handles of the same sizes reuse
+   (RISM_POOL_BYTES bytes of buffers at most, 8 GiB by default; 0 frees at
+   once): releasing them synchronises the device, which stalls every other
+   handle's stream when many handles come and go. */
 void rism_destroy(rism_handle *h);
+/* free everything that cache holds, on every device (no counterpart in the
+   reference); returns the number of buffer bytes freed.  Handles in use are
+   not affected. */
+long rism_release_cache(void);
 
 /* ---- many solutes (no counterpart in the reference: one solve per run) ---- */
 /* n independent solves sharing one solvent, each solute on its own grid

@@ -183,3 +183,26 @@ def test_reused_buffers_leave_the_solution_unchanged(gpu, prec):
     for k in ("SFE_SC", "SFE_GF", "PMV", "SE"):
         assert x2[k] == x0[k], k
     assert np.array_equal(g2, g0)
+
+
+def test_release_cache_frees_and_solving_continues(gpu):
+    """rism_release_cache frees what destroyed handles left for reuse (a
+    second call finds nothing), and handles created afterwards allocate
+    afresh and solve as before."""
+    print("release cache", flush=True)
+    svf, n, box, natom, nhist = CASES["water_H5"]
+    sv = rism3d.read_xvk(example(svf))
+    sig, eps, q, xyz = _atoms(natom)
+    res = []
+    for release in (False, True):
+        s = rism3d.Solver(n, box, sv, rism3d.Solute(sig, eps, q, xyz), rism3d.Options())
+        s.build_potentials()
+        assert s.solve()[0] == rism3d.SOLVE_CONVERGED
+        res.append(s.thermo()["SFE_SC"])
+        s.close()
+        if release:
+            freed = rism3d.release_cache()
+            print("freed", freed, flush=True)
+            assert freed > 0
+            assert rism3d.release_cache() == 0
+    assert res[0] == res[1]
