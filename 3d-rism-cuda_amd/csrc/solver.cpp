@@ -140,6 +140,7 @@ struct rism_handle {
   // replayable chunks of iterations: [precision][delta mode]
   hipGraphExec_t gexec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   int gchunk[2][2] = {{0, 0}, {0, 0}};
+  double gcf[2][2] = {{0, 0}, {0, 0}};  // charge-up factor the graph was captured with
   bool graph_ok = true;  // false after a capture failed on this handle
   int steps = 0;
   double rms = 0.0;
@@ -643,9 +644,11 @@ static void drop_graphs(rism_handle *h) {
 }
 
 // n iterations as one HIP graph: captured on first use for the current
-// precision and refinement mode, then replayed.  The launches of an
-// iteration do not depend on its index (the kernels read it from the
-// control block), and they hold no memset or copy nodes.
+// precision and refinement mode, then replayed, also by later solves on the
+// handle.  The launches of an iteration do not depend on its index (the
+// kernels read it from the control block), and they hold no memset or copy
+// nodes; of the handle's state they carry only q_v cf (captured again when
+// cf differs) and the tile shapes (the autotuner drops the graphs).
 template <typename R> int launch_chunk_graph(rism_handle *h, int n) {
   hipGraphExec_t &ex = h->gexec[pidx<R>()][h->delta ? 1 : 0];
   int &gn = h->gchunk[pidx<R>()][h->delta ? 1 : 0];
@@ -654,7 +657,8 @@ template <typename R> int launch_chunk_graph(rism_handle *h, int n) {
       if (int rc = launch_iteration<R>(h, false, true)) return rc;
     return RISM_OK;
   }
-  if (!ex || gn != n) {
+  double &gc = h->gcf[pidx<R>()][h->delta ? 1 : 0];
+  if (!ex || gn != n || gc != h->cf) {
     if (ex) (void)hipGraphExecDestroy(ex);
     ex = nullptr;
     // relaxed mode: other host threads (other handles) keep allocating and
@@ -680,6 +684,7 @@ template <typename R> int launch_chunk_graph(rism_handle *h, int n) {
       return RISM_OK;
     }
     gn = n;
+    gc = h->cf;
   }
   HIP_OK(hipGraphLaunch(ex, h->stream));
   return RISM_OK;
@@ -728,7 +733,6 @@ template <typename R> int solve_impl(rism_handle *h, const char *tuv_path, int *
   // step (RISM_GRAPHS=0 turns them off)
   const char *genv = std::getenv("RISM_GRAPHS");
   const bool graphs = !(genv && genv[0] == '0') && !h->opts.verbose && !h->prof && h->opts.ksave <= 0;
-  drop_graphs(h);
   // iterations enqueued per control-block read; 1 when timing kernels so no
   // gated (empty) launches enter the averages
   const int chunk = (h->opts.verbose || h->prof) ? 1 : (N >= (1L << 21) ? 4 : 16);
@@ -759,7 +763,6 @@ template <typename R> int solve_impl(rism_handle *h, const char *tuv_path, int *
       }
     }
     if (int rc = stage_potentials(h)) return rc;
-    drop_graphs(h);    // the captured launches carry q_v cf
     h->delta = false;  // a new stage (new cf) starts without a reference
     double ref_rms = -1.0, best_rms = HUGE_VAL;
     int best_at = 0, since_ref = 0;
