@@ -29,4 +29,8 @@ int table_interval(double k, double dk, int ntab);
 
 bool is_supported_length(int n);
 
+// A new solute on an existing handle of the same grid, solvent and options
+// (the batched solves reuse a worker's handle per grid shape).
+int set_solute(rism_handle *h, const rism_solute *su);
+
 }  // namespace rism

@@ -104,6 +104,7 @@ struct rism_handle {
   int ntab = 0;
   std::vector<double> chi;  // [ntab][V][V]
   int natu = 0;
+  int natu_cap = 1;  // atoms the solute-sized device arrays hold
   std::vector<double> usig, ueps, uq, uxyz;
   rism_opts opts{};
   int hm = 2;               // compiled history class >= nhist
@@ -974,6 +975,8 @@ template <typename R> int alloc_precision(rism_handle *h) {
   return RISM_OK;
 }
 
+int upload_solute(rism_handle *h);
+
 int create_impl(const rism_grid *grid, const rism_solvent *sv, const rism_solute *su, const rism_opts *o,
                 int device, rism_handle *h) {
   const double t0 = now_s();
@@ -1097,7 +1100,14 @@ int create_impl(const rism_grid *grid, const rism_solvent *sv, const rism_solute
     const size_t npart = std::max<size_t>((size_t)1024 * 4 * kMaxV, (size_t)kMaxXBlocks * nv);
     if ((rc = dalloc_t(h, &h->partials, npart))) return rc;
   }
-  // solute records and Lorentz-Berthelot pair parameters (cal_LJ.cu:24-31)
+  h->natu_cap = std::max(1, h->natu);
+  if ((rc = upload_solute(h))) return rc;
+  h->t_setup = now_s() - t0;
+  return RISM_OK;
+}
+
+// solute records and Lorentz-Berthelot pair parameters (cal_LJ.cu:24-31)
+int upload_solute(rism_handle *h) {
   std::vector<double4> at(std::max(1, h->natu));
   std::vector<double2> lj((size_t)h->V * std::max(1, h->natu));
   std::vector<double> sigh((size_t)h->V * std::max(1, h->natu));
@@ -1114,11 +1124,41 @@ int create_impl(const rism_grid *grid, const rism_solvent *sv, const rism_solute
   HIP_OK(hcopy(h, h->sigh, sigh.data(), sigh.size() * sizeof(double), hipMemcpyHostToDevice));
   HIP_OK(hcopy(h, h->d_qv, h->qv.data(), h->V * sizeof(double), hipMemcpyHostToDevice));
   HIP_OK(hcopy(h, h->d_rho, h->rho.data(), h->V * sizeof(double), hipMemcpyHostToDevice));
-  h->t_setup = now_s() - t0;
   return RISM_OK;
 }
 
 }  // namespace
+
+// A new solute on an existing handle (same grid, solvent and options): the
+// field buffers and the captured iteration graphs stay; the solute-sized
+// arrays grow when needed (the adjoint's are then allocated afresh on its
+// next use).  Potentials must be built again before the next solve.
+int rism::set_solute(rism_handle *h, const rism_solute *su) {
+  if (!h || !su || su->natom < 0 || (su->natom > 0 && (!su->sig || !su->eps || !su->q || !su->xyz))) {
+    set_error("set_solute: bad arguments");
+    return RISM_ERR_ARG;
+  }
+  HIP_OK(hipSetDevice(h->device));
+  const int nu = su->natom;
+  if (std::max(1, nu) > h->natu_cap) {
+    const int cap = std::max(1, nu);
+    int rc = 0;
+    if ((rc = dalloc_t(h, &h->atoms, cap)) || (rc = dalloc_t(h, &h->lj, (size_t)h->V * cap)) ||
+        (rc = dalloc_t(h, &h->sigh, (size_t)h->V * cap)) ||
+        (rc = dalloc_t(h, &h->phase, (size_t)cap * (h->nx + h->ny + h->nz))))
+      return rc;
+    h->natu_cap = cap;
+    h->adj_f = nullptr;  // its per-atom arrays are sized by the solute
+  }
+  h->natu = nu;
+  h->usig.assign(su->sig, su->sig + nu);
+  h->ueps.assign(su->eps, su->eps + nu);
+  h->uq.assign(su->q, su->q + nu);
+  h->uxyz.assign(su->xyz, su->xyz + 3 * (size_t)nu);
+  if (int rc = upload_solute(h)) return rc;
+  h->potentials = h->solved = h->evaluated = false;
+  return RISM_OK;
+}
 
 // =====================================================================
 // C ABI
