@@ -180,6 +180,9 @@ def side_workload(which, rism3d, configs):
            "unit": "s", "higher_is_better": False,
            "solvent": "water + 0.15 M NaCl (4 reduced sites), chi from the library's 1D XRISM-KH solve"}
     for name, prec in (("mixed", rism3d.MIXED), ("fp64", rism3d.FP64)):
+        # buffers a destroyed handle left for reuse would not show as new
+        # device memory: return them first, so device_GB is this path's own
+        rism3d.release_cache()
         free0, _ = torch.cuda.mem_get_info()
         s = rism3d.Solver(w.n, w.box, sv, w.solute, rism3d.Options(precision=prec, nhist=w.nhist))
         free1, _ = torch.cuda.mem_get_info()
