@@ -112,6 +112,8 @@ struct rism_handle {
   bool tuned = false;       // rism_autotune has chosen the tile shapes
   // device memory
   std::vector<DevBuf> bufs;
+  void *scratch = nullptr;  // one-off reductions (see scratch())
+  size_t scratch_bytes = 0;
   double *ulj = nullptr, *de = nullptr, *fr = nullptr, *sr = nullptr;
   double *sel = nullptr, *seq = nullptr;  // [V][N] SE atom sums from the potentials pass
   double2 *fkh = nullptr, *phase = nullptr;
@@ -130,7 +132,8 @@ struct rism_handle {
   double *adj_hist = nullptr;    // hist_l, hist_r: 2 x [nhist][V][N]
   double *adj_n = nullptr;       // debar, frbar: 2 x [N]
   double2 *adj_Z = nullptr, *adj_Y = nullptr;
-  double2 *adj_ljd = nullptr;
+  double2 *adj_ljd = nullptr;   // [V][natu]: sized by the solute (adj_ljd_cap atoms)
+  int adj_ljd_cap = 0;
   double *adj_rms = nullptr;
   Ctl *adj_ctl = nullptr;
   // state
@@ -301,12 +304,52 @@ int dalloc(rism_handle *h, void **p, size_t bytes) {
   return RISM_OK;
 }
 
+// a buffer to the cache, or back to the device when the cache is full
+void release_locked(int device, const DevBuf &b) {
+  if (g_cache_bytes + b.bytes <= cache_cap()) {
+    g_cache.push_back({device, b.bytes, b.p});
+    g_cache_bytes += b.bytes;
+  } else {
+    (void)hipFree(b.p);
+  }
+}
+
+// return one of the handle's buffers once its stream's queued work is done
+void dfree(rism_handle *h, void *p) {
+  if (!p) return;
+  (void)hipStreamSynchronize(h->stream);
+  std::lock_guard<std::mutex> lk(g_device_mu);
+  for (size_t i = 0; i < h->bufs.size(); ++i)
+    if (h->bufs[i].p == p) {
+      release_locked(h->device, h->bufs[i]);
+      h->bufs[i] = h->bufs.back();
+      h->bufs.pop_back();
+      return;
+    }
+}
+
 template <typename T> int dalloc_t(rism_handle *h, T **p, size_t count) {
   void *q = nullptr;
   int rc = dalloc(h, &q, count * sizeof(T));
   *p = (T *)q;
   return rc;
 }
+
+// per-handle scratch for the one-off reductions (grad, euv, qv): grown on
+// demand through the buffer cache, never hipMalloc/hipFree per call
+int scratch(rism_handle *h, size_t bytes, void **p) {
+  bytes = (bytes + 255) & ~(size_t)255;
+  if (h->scratch_bytes < bytes) {
+    dfree(h, h->scratch);
+    h->scratch = nullptr;
+    h->scratch_bytes = 0;
+    if (int rc = dalloc(h, &h->scratch, bytes)) return rc;
+    h->scratch_bytes = bytes;
+  }
+  *p = h->scratch;
+  return RISM_OK;
+}
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // per-axis plan + twiddle tables in the iteration precision
 template <typename R> int build_plans(rism_handle *h) {
@@ -1142,13 +1185,22 @@ int rism::set_solute(rism_handle *h, const rism_solute *su) {
   const int nu = su->natom;
   if (std::max(1, nu) > h->natu_cap) {
     const int cap = std::max(1, nu);
+    // the superseded arrays go back to the cache
+    dfree(h, h->atoms);
+    dfree(h, h->lj);
+    dfree(h, h->sigh);
+    dfree(h, h->phase);
+    h->atoms = nullptr;
+    h->lj = nullptr;
+    h->sigh = nullptr;
+    h->phase = nullptr;
+    h->natu_cap = 0;
     int rc = 0;
     if ((rc = dalloc_t(h, &h->atoms, cap)) || (rc = dalloc_t(h, &h->lj, (size_t)h->V * cap)) ||
         (rc = dalloc_t(h, &h->sigh, (size_t)h->V * cap)) ||
         (rc = dalloc_t(h, &h->phase, (size_t)cap * (h->nx + h->ny + h->nz))))
       return rc;
     h->natu_cap = cap;
-    h->adj_f = nullptr;  // its per-atom arrays are sized by the solute
   }
   h->natu = nu;
   h->usig.assign(su->sig, su->sig + nu);
@@ -1425,22 +1477,15 @@ int rism_grad(rism_handle *h, double *out) {
   const int nu = h->natu;
   if (nu == 0) return RISM_OK;
   const int nblk = (int)((h->N + (long)kThreads * 16 - 1) / ((long)kThreads * 16));
-  double *part = nullptr, *dout = nullptr;
-  {
-    std::lock_guard<std::mutex> lk(g_device_mu);
-    HIP_OK(hipMalloc((void **)&part, (size_t)nblk * nu * 6 * sizeof(double)));
-    HIP_OK(hipMalloc((void **)&dout, (size_t)nu * 6 * sizeof(double)));
-  }
+  const size_t bpart = align256((size_t)nblk * nu * 6 * sizeof(double));
+  void *sp = nullptr;
+  if (int rc = scratch(h, bpart + (size_t)nu * 6 * sizeof(double), &sp)) return rc;
+  double *part = (double *)sp, *dout = (double *)((char *)sp + bpart);
   hipError_t e = launch_grad(h->nx, h->ny, h->nz, h->dr, h->V, nu, h->atoms, h->lj, h->sigh, h->d_qv, h->d_rho,
                              h->gfield, part, nblk, dout, h->stream);
   std::vector<double> g((size_t)nu * 6);
   if (e == hipSuccess) e = hipMemcpyAsync(g.data(), dout, g.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-  {
-    std::lock_guard<std::mutex> lk(g_device_mu);
-    (void)hipFree(part);
-    (void)hipFree(dout);
-  }
   if (e != hipSuccess) {
     set_error(std::string("HIP error in rism_grad: ") + hipGetErrorString(e));
     return RISM_ERR_HIP;
@@ -1459,22 +1504,15 @@ int rism_euv(rism_handle *h, double *out) {
   const int nu = h->natu, V = h->V;
   if (nu == 0) return RISM_OK;
   const int nblk = (int)((h->N + (long)kThreads * 16 - 1) / ((long)kThreads * 16));
-  double *part = nullptr, *dout = nullptr;
-  {
-    std::lock_guard<std::mutex> lk(g_device_mu);
-    HIP_OK(hipMalloc((void **)&part, (size_t)nblk * nu * 2 * V * sizeof(double)));
-    HIP_OK(hipMalloc((void **)&dout, (size_t)nu * 2 * V * sizeof(double)));
-  }
+  const size_t bpart = align256((size_t)nblk * nu * 2 * V * sizeof(double));
+  void *sp = nullptr;
+  if (int rc = scratch(h, bpart + (size_t)nu * 2 * V * sizeof(double), &sp)) return rc;
+  double *part = (double *)sp, *dout = (double *)((char *)sp + bpart);
   hipError_t e = launch_euv_atoms(h->nx, h->ny, h->nz, h->dr, V, nu, h->atoms, h->lj, h->sigh, h->d_qv, h->d_rho,
                                   h->gfield, part, nblk, dout, h->stream);
   if (e == hipSuccess)
     e = hipMemcpyAsync(out, dout, (size_t)nu * 2 * V * sizeof(double), hipMemcpyDeviceToHost, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-  {
-    std::lock_guard<std::mutex> lk(g_device_mu);
-    (void)hipFree(part);
-    (void)hipFree(dout);
-  }
   if (e != hipSuccess) {
     set_error(std::string("HIP error in rism_euv: ") + hipGetErrorString(e));
     return RISM_ERR_HIP;
@@ -1488,18 +1526,12 @@ int rism_qv(rism_handle *h, double *out) {
   HIP_OK(hipSetDevice(h->device));
   if (!h->evaluated)
     if (int rc = evaluate(h)) return rc;
-  double *d = nullptr;
-  {
-    std::lock_guard<std::mutex> lk(g_device_mu);
-    HIP_OK(hipMalloc((void **)&d, (size_t)h->N * sizeof(double)));
-  }
+  void *sp = nullptr;
+  if (int rc = scratch(h, (size_t)h->N * sizeof(double), &sp)) return rc;
+  double *d = (double *)sp;
   hipError_t e = launch_qv(h->N, h->V, h->d_rho, h->d_qv, h->gfield, d, h->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(out, d, (size_t)h->N * sizeof(double), hipMemcpyDeviceToHost, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-  {
-    std::lock_guard<std::mutex> lk(g_device_mu);
-    (void)hipFree(d);
-  }
   if (e != hipSuccess) {
     set_error(std::string("HIP error in rism_qv: ") + hipGetErrorString(e));
     return RISM_ERR_HIP;
@@ -1668,12 +1700,19 @@ int rism_adjoint_grad(rism_handle *h, double rtol, double *dxyz, double *dq, dou
   const size_t VN = (size_t)V * N;
   PrecSet &ps = h->ps[0];
   int rc = 0;
-  if (!h->adj_f) {
+  if (!h->adj_f) {  // grid-sized: kept for the life of the handle
     if ((rc = dalloc_t(h, &h->adj_f, 7 * VN)) || (rc = dalloc_t(h, &h->adj_hist, 2 * (size_t)nh * VN)) ||
         (rc = dalloc_t(h, &h->adj_n, 2 * (size_t)N)) || (rc = dalloc_t(h, &h->adj_Z, (size_t)P * N)) ||
-        (rc = dalloc_t(h, &h->adj_Y, (size_t)N)) || (rc = dalloc_t(h, &h->adj_ljd, (size_t)V * std::max(1, nu))) ||
-        (rc = dalloc_t(h, &h->adj_rms, (size_t)h->opts.maxstep + 4)) || (rc = dalloc_t(h, &h->adj_ctl, 1)))
+        (rc = dalloc_t(h, &h->adj_Y, (size_t)N)) || (rc = dalloc_t(h, &h->adj_rms, (size_t)h->opts.maxstep + 4)) ||
+        (rc = dalloc_t(h, &h->adj_ctl, 1)))
       return rc;
+  }
+  if (h->adj_ljd_cap < std::max(1, nu)) {  // sized by the solute
+    dfree(h, h->adj_ljd);
+    h->adj_ljd = nullptr;
+    h->adj_ljd_cap = 0;
+    if ((rc = dalloc_t(h, &h->adj_ljd, (size_t)V * std::max(1, nu)))) return rc;
+    h->adj_ljd_cap = std::max(1, nu);
   }
   // (1 / (2 sigma_uv), 4 eps_uv / (2 eps_u)): Lorentz-Berthelot derivatives
   std::vector<double2> ljd((size_t)V * std::max(1, nu));
@@ -1766,23 +1805,15 @@ int rism_adjoint_grad(rism_handle *h, double rtol, double *dxyz, double *dq, dou
   HIP_OK(launch_adj_bar(a, h->adj_Z, st));
   if (nu > 0) {
     const int gblk = adj_grad_blocks(N);
-    double *p6 = nullptr, *p4 = nullptr, *o = nullptr;
-    {
-      std::lock_guard<std::mutex> lk(g_device_mu);
-      HIP_OK(hipMalloc((void **)&p6, (size_t)gblk * nu * 6 * sizeof(double)));
-      HIP_OK(hipMalloc((void **)&p4, (size_t)gblk * nu * 4 * sizeof(double)));
-      HIP_OK(hipMalloc((void **)&o, (size_t)nu * 10 * sizeof(double)));
-    }
+    const size_t b6 = align256((size_t)gblk * nu * 6 * sizeof(double)),
+                 b4 = align256((size_t)gblk * nu * 4 * sizeof(double));
+    void *sp = nullptr;
+    if (int rc2 = scratch(h, b6 + b4 + (size_t)nu * 10 * sizeof(double), &sp)) return rc2;
+    double *p6 = (double *)sp, *p4 = (double *)((char *)sp + b6), *o = (double *)((char *)sp + b6 + b4);
     hipError_t e = launch_adj_grad(a, h->adj_Y, p6, p4, o, o + (size_t)nu * 6, st);
     std::vector<double> r((size_t)nu * 10);
     if (e == hipSuccess) e = hipMemcpyAsync(r.data(), o, r.size() * sizeof(double), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    {
-      std::lock_guard<std::mutex> lk(g_device_mu);
-      (void)hipFree(p6);
-      (void)hipFree(p4);
-      (void)hipFree(o);
-    }
     if (e != hipSuccess) {
       set_error(std::string("HIP error in rism_adjoint_grad: ") + hipGetErrorString(e));
       return RISM_ERR_HIP;
@@ -1941,14 +1972,7 @@ void rism_destroy(rism_handle *h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   std::lock_guard<std::mutex> lk(g_device_mu);
   drop_graphs(h);
-  for (auto &b : h->bufs) {
-    if (g_cache_bytes + b.bytes <= cache_cap()) {
-      g_cache.push_back({h->device, b.bytes, b.p});
-      g_cache_bytes += b.bytes;
-    } else {
-      (void)hipFree(b.p);
-    }
-  }
+  for (auto &b : h->bufs) release_locked(h->device, b);
   for (auto &e : h->ev_pool) (void)hipEventDestroy(e);
   if (h->stream && h->ctl_host && g_spare.size() < kMaxSpare) {
     g_spare.push_back({h->device, h->stream, h->ctl_host});

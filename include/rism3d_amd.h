@@ -100,7 +100,7 @@ typedef struct rism_opts {
   int count;      /* Anderson Picard warm-up steps (ma->count) */
   double m;       /* Anderson mixing (ma->m) */
   double mp;      /* extra weight of the most recent slot (ma->mp) */
-  int nhist;      /* Anderson history slots; 2 = the reference's AN2; <= 16 */
+  int nhist;      /* Anderson history slots, 1..10; 2 = the reference's AN2 */
   int precision;  /* RISM_PREC_FP64 / RISM_PREC_MIXED */
   int chargeup;   /* charge-up stages (main.cu -c); 0 = start from t = q_v*fr */
   int verbose;    /* print the reference's progress lines on stdout */

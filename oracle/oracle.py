@@ -12,7 +12,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "_build", "liboracle.so")
+# RISM_ORACLE_LIB names another build of the same sources (the SE pin tests
+# load builds with a planted mistake this way, in a subprocess of their own)
+LIB = os.environ.get("RISM_ORACLE_LIB") or os.path.join(HERE, "_build", "liboracle.so")
 EXE = os.path.join(HERE, "_build", "rism_oracle")
 
 
@@ -74,6 +76,7 @@ def lib():
         L.orc_set_iter.argtypes = [vp, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double]
         L.orc_alloc_test.argtypes = [vp, C.c_int, C.c_long, C.c_int, dp]
         L.orc_set_tr.argtypes = [vp, dp]
+        L.orc_set_g.argtypes = [vp, dp]
         L.orc_copy_hist.argtypes = [vp, C.c_int, C.c_int, dp]
         L.orc_set_atom_xyz.argtypes = [vp, C.c_int, dp]
         L.orc_write_tuv.argtypes = [vp, C.c_char_p]
@@ -187,6 +190,11 @@ class Oracle:
     def set_tr(self, tr):
         tr = np.ascontiguousarray(tr, dtype=np.float64).reshape(-1)
         lib().orc_set_tr(self.s, _d(tr))
+
+    def set_g(self, g):
+        """Overwrite g (test hook) for the thermodynamic sums."""
+        g = np.ascontiguousarray(g, dtype=np.float64).reshape(-1)
+        lib().orc_set_g(self.s, _d(g))
 
     def write_tuv(self, path):
         """write_tuv.cu format: header n[3] V, then t, 6 significant digits."""

@@ -29,6 +29,13 @@
 
 #define ORC_BLK 4096L
 
+/* ORC_MUTANT (tests only, never in the shipped build): a nonzero value plants
+   one known mistake in the SE sums, so the closed-form pins can be shown to
+   reject it */
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+
 static __thread char g_err[1024];
 
 static void set_err(const char *fmt, ...) {
@@ -989,11 +996,19 @@ static void euv_sum(const orc_sys *s, int v, int u, double sig, double eps,
       double dy = (y - ny / 2) * s->dr[1] - s->ru[3 * u + 1];
       double dz = (z - s->n[2] / 2) * s->dr[2] - s->ru[3 * u + 2];
       double r2 = dx * dx + dy * dy + dz * dz, r1 = sqrt(r2);
+#if ORC_MUTANT == 1 /* planted mistake (test of the SE pins): exclusion at sigma */
+      if (r1 < sig) continue;
+#else
       if (r1 < sig * 0.5) continue;
+#endif
       double rs2i = sig * sig / r2, rs6i = rs2i * rs2i * rs2i;
       double gg = creal(g[ip]);
       al += eps * 4.0 * rs6i * (rs6i - 1.0) * gg;
+#if ORC_MUTANT == 4 /* planted mistake: cc without Avogadro's number */
+      ae += qq / r1 * (ORC_HARTREE * ORC_BOHR) * gg;
+#else
       ae += qq / r1 * cc * gg;
+#endif
     }
     pl[b] = al; pe[b] = ae;
   }
@@ -1030,12 +1045,21 @@ void orc_thermo(orc_sys *s, orc_xmu *x) {
     double lj_v = 0.0, es_v = 0.0;
 #pragma omp parallel for reduction(+ : lj_v, es_v) schedule(dynamic)
     for (int u = 0; u < s->natu; ++u) {
+#if ORC_MUTANT == 2
+      double sg = sqrt(s->sigu[u] * s->sigv[v]);
+#else
       double sg = (s->sigu[u] + s->sigv[v]) * 0.5;
+#endif
       double ep = sqrt(s->epsu[u] * s->epsv[v]);
       double a, b;
       euv_sum(s, v, u, sg, ep, &a, &b);
+#if ORC_MUTANT == 3
+      lj_v += a;
+      es_v += b;
+#else
       lj_v += a * s->rhov[v];
       es_v += b * s->rhov[v];
+#endif
     }
     selj += lj_v; sees += es_v;
   }
@@ -1066,12 +1090,21 @@ void orc_euv(orc_sys *s, double *out) {
   for (int v = 0; v < V; ++v) {
 #pragma omp parallel for schedule(dynamic)
     for (int u = 0; u < nu; ++u) {
+#if ORC_MUTANT == 2 /* planted mistake: geometric sigma mixing */
+      double sg = sqrt(s->sigu[u] * s->sigv[v]);
+#else
       double sg = (s->sigu[u] + s->sigv[v]) * 0.5;
+#endif
       double ep = sqrt(s->epsu[u] * s->epsv[v]);
       double a, b;
       euv_sum(s, v, u, sg, ep, &a, &b);
+#if ORC_MUTANT == 3 /* planted mistake: the rho_v factor dropped */
+      out[(size_t)v * nu + u] = a * dv;
+      out[(size_t)(V + v) * nu + u] = b * dv;
+#else
       out[(size_t)v * nu + u] = a * s->rhov[v] * dv;
       out[(size_t)(V + v) * nu + u] = b * s->rhov[v] * dv;
+#endif
     }
   }
 }
@@ -1329,6 +1362,14 @@ int orc_alloc_test(orc_sys *s, int natv, long ng, int nhist, const double *rho) 
   ensure_fields(s);
   orc_anderson_reset(s);
   return 0;
+}
+
+/* test hook: g (guv, real) for the thermodynamic sums, e.g. g = 1 for the SE
+   closed forms */
+void orc_set_g(orc_sys *s, const double *g) {
+  ensure_fields(s);
+  const size_t VN = (size_t)s->natv * s->ng;
+  for (size_t i = 0; i < VN; ++i) s->guv[i] = g[i];
 }
 
 void orc_set_tr(orc_sys *s, const double *tr) {

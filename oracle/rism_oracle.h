@@ -154,6 +154,8 @@ void orc_set_nhist(orc_sys *s, int nhist);
 /* test hooks: a bare field set for exercising the Anderson step alone */
 int orc_alloc_test(orc_sys *s, int natv, long ng, int nhist, const double *rho);
 void orc_set_tr(orc_sys *s, const double *tr);
+/* test hook: set g (real) for the thermodynamic sums */
+void orc_set_g(orc_sys *s, const double *g);
 void orc_copy_hist(const orc_sys *s, int which, int slot, double *out);
 void orc_set_iter(orc_sys *s, double tol, int maxstep, int count, double m, double mp);
 /* test hook: move solute atom u (its grid potentials are not recomputed) */

@@ -1,8 +1,9 @@
 """Many small solutes solved concurrently (rism_batch_solve), the batched
 workload of seeded 8-40-atom molecules on 60^3-72^3 grids.
 
-Checks: every solve converges; each batched FP64 result equals the same
-solute solved alone (same arithmetic, so to 1e-12 relative); one solute
+Checks: every solve converges; every batched FP64 result equals the same
+solute solved alone (same arithmetic, so to 1e-12 relative), with one
+worker (each same-grid neighbour through the solute swap) and with six; one solute
 against the FP64 oracle (SFE terms within 0.05 J/mol, as for single
 solves); mixed precision against FP64 within 0.01 kcal/mol (41.84 J/mol)
 for every solute.
@@ -31,19 +32,31 @@ def batch():
 
 
 def test_batch_fp64_matches_single_solves_and_oracle(gpu, batch):
+    """Every batched FP64 result equals its standalone solve.  With one
+    worker the solutes run in order on one handle per grid, so every
+    consecutive pair on the same grid goes through the solute swap
+    (rism::set_solute, which keeps the field buffers and captured graphs),
+    including swaps to a larger solute, which grow the per-atom arrays."""
     sv, ws, systems = batch
-    print(f"batch fp64 n={len(systems)}", flush=True)
-    res, sec = rism3d.batch_solve(systems, sv, rism3d.Options(), workers=6)
-    print("seconds", sec, "solutes/s", len(res) / sec, flush=True)
-    assert all(r["status"] == rism3d.SOLVE_CONVERGED for r in res), [r["status"] for r in res]
-    for i in (0, 5):
-        s = rism3d.Solver(ws[i].n, ws[i].box, sv, ws[i].solute, rism3d.Options())
+    swaps = [(ws[i - 1].n == ws[i].n, len(ws[i].solute.q) > len(ws[i - 1].solute.q)) for i in range(1, len(ws))]
+    assert any(same and grow for same, grow in swaps), "workload has no same-grid swap to a larger solute"
+    single = []
+    for w in ws:
+        s = rism3d.Solver(w.n, w.box, sv, w.solute, rism3d.Options())
         s.build_potentials()
         st, steps, _ = s.solve()
-        x = s.thermo()
+        single.append((st, steps, s.thermo()))
         s.close()
-        assert steps == res[i]["steps"]
-        assert x["SFE_SC"] == pytest.approx(res[i]["SFE_SC"], rel=1e-12, abs=1e-9)
+    for workers in (1, 6):
+        print(f"batch fp64 n={len(systems)} workers={workers}", flush=True)
+        res, sec = rism3d.batch_solve(systems, sv, rism3d.Options(), workers=workers)
+        print("seconds", sec, "solutes/s", len(res) / sec, flush=True)
+        assert all(r["status"] == rism3d.SOLVE_CONVERGED for r in res), [r["status"] for r in res]
+        for i, (st, steps, x) in enumerate(single):
+            assert st == rism3d.SOLVE_CONVERGED
+            assert steps == res[i]["steps"], (workers, i)
+            for k in ("SFE_SC", "SFE_GF", "PMV"):
+                assert x[k] == pytest.approx(res[i][k], rel=1e-12, abs=1e-9), (workers, i, k)
     w = ws[0]
     su = w.solute
     o = orc.Oracle.from_arrays(w.n, w.box, example("h2o_vv.xvk"), su.sig, su.eps, su.q, su.xyz).setup()

@@ -323,3 +323,122 @@ def test_grad_is_derivative_of_solute_solvent_energy():
             d_es = (e[0][1] - e[1][1]) / (2 * h)
             assert d_lj == pytest.approx(g[u, k], rel=1e-5, abs=1e-6 * np.abs(g[:, :3]).max())
             assert d_es == pytest.approx(g[u, 3 + k], rel=1e-5, abs=1e-6 * np.abs(g[:, 3:]).max())
+
+
+# ------------------------------------------------------ SE closed forms ----
+def _box_inv_r(lo, hi):
+    """Integral of dV / |r| over the box [lo, hi] (corners relative to the
+    point): the Newtonian potential of a uniform rectangular prism, in the
+    closed form of its corner function
+    F = yz ln(x+r) + xz ln(y+r) + xy ln(z+r)
+        - x^2/2 atan(yz/(xr)) - y^2/2 atan(xz/(yr)) - z^2/2 atan(xy/(zr))."""
+    def F(x, y, z):
+        r = math.sqrt(x * x + y * y + z * z)
+        return (y * z * math.log(x + r) + x * z * math.log(y + r) + x * y * math.log(z + r)
+                - 0.5 * x * x * math.atan(y * z / (x * r)) - 0.5 * y * y * math.atan(x * z / (y * r))
+                - 0.5 * z * z * math.atan(x * y / (z * r)))
+    s = 0.0
+    for i, x in enumerate((lo[0], hi[0])):
+        for j, y in enumerate((lo[1], hi[1])):
+            for k, z in enumerate((lo[2], hi[2])):
+                s += (-1) ** (i + j + k + 1) * F(x, y, z)
+    return s
+
+
+def test_box_inv_r_closed_form():
+    # the potential at the centre of a unit cube, 2.3800774... (a tabulated constant)
+    assert _box_inv_r((-0.5,) * 3, (0.5,) * 3) == pytest.approx(2.380077363979553, rel=1e-12)
+    # additivity: two halves make the whole
+    a = _box_inv_r((-1.0, -2.0, -0.5), (3.0, 1.0, 2.0))
+    b = _box_inv_r((-1.0, -2.0, -0.5), (1.0, 1.0, 2.0)) + _box_inv_r((1.0, -2.0, -0.5), (3.0, 1.0, 2.0))
+    assert a == pytest.approx(b, rel=1e-12)
+
+
+def _xvk_reduced_sites(path):
+    """(sigma, eps, q, density 1/A^3) per reduced site of an .xvk header,
+    equivalent sites (negative symmetry index) merged by summing densities."""
+    rows = [ln.split() for ln in open(path).read().splitlines()[4:] if ln.startswith("##") and len(ln.split()) > 9]
+    sites = {}
+    for r in rows:
+        sym = abs(int(r[3]))
+        sig, eps, q, conc = float(r[4]), float(r[5]), float(r[6]), float(r[10])
+        if sym in sites:
+            sites[sym][3] += conc * NA * 1e-27
+        else:
+            sites[sym] = [sig, eps, q, conc * NA * 1e-27]
+    return [sites[k] for k in sorted(sites)]
+
+
+def se_closed_form_errors():
+    """Relative deviations of the oracle's SE sums (cal_euv.cu:35-76, per
+    atom and site through orc_euv, totals through orc_thermo) from closed
+    forms, for g = 1 on a 128^3 / 12.8 A grid around a handful of LJ atoms
+    placed near the centre, off the lattice.  With g = 1 each (atom u, site
+    v) term is an integral of its pair potential alone:
+      LJ  rho_v * int_{r >= s/2} 4 eps_uv [(s/r)^12 - (s/r)^6] dV
+          = 16 pi rho_v eps_uv s^3 (2^9/9 - 2^3/3)       (s = sigma_uv)
+      ES  rho_v q_u q_v E_h a0 N_A [int_cell-box dV/r - 2 pi (s/2)^2]
+    with sigma_uv = (sigma_u + sigma_v)/2 and eps_uv = sqrt(eps_u eps_v)
+    (Lorentz-Berthelot, cal_LJ.cu:24-31).  The lattice sum meets the LJ
+    integral to about 1% (the r^-12 wall at s/2 cuts through the lattice)
+    and the ES integral to about 1e-5."""
+    n, box = (128, 128, 128), (12.8, 12.8, 12.8)
+    dr = box[0] / n[0]
+    xvk = example("h2o_vv.xvk")
+    sites = _xvk_reduced_sites(xvk)
+    rng = np.random.default_rng(11)
+    sig = np.array([2.0, 2.4, 2.8, 3.1, 3.4])
+    eps = rng.uniform(200.0, 900.0, sig.size)
+    q = rng.uniform(-1.0, 1.0, sig.size)
+    xyz = rng.uniform(-0.05, 0.05, (sig.size, 3))
+    o = orc.Oracle.from_arrays(n, box, xvk, sig, eps, q, xyz).setup()
+    V = o.V
+    o.set_g(np.ones(V * o.N))
+    e = o.euv()
+    x = o.thermo()
+    # constants from CODATA (the reference's bohr differs from CODATA 2018 in the 7th digit)
+    cc = 4.3597447222071e-18 * 0.529177210903 * 6.02214076e23
+    lo = np.full(3, (-(n[0] // 2) - 0.5) * dr)
+    hi = np.full(3, (n[0] - 1 - n[0] // 2 + 0.5) * dr)
+    err_lj, err_es = [], []
+    for v, (sv, ev, qv, rho) in enumerate(sites):
+        for u in range(sig.size):
+            s = 0.5 * (sig[u] + sv)
+            euv = math.sqrt(eps[u] * ev)
+            lj = 16 * math.pi * rho * euv * s ** 3 * (2 ** 9 / 9 - 2 ** 3 / 3)
+            es = rho * q[u] * qv * cc * (_box_inv_r(lo - xyz[u], hi - xyz[u]) - 2 * math.pi * (s / 2) ** 2)
+            err_lj.append(e[0, v, u] / lj - 1)
+            err_es.append(e[1, v, u] / es - 1)
+    tot = {"lj": x["SE_LJ"] / e[0].sum() - 1, "es": x["SE_ES"] / e[1].sum() - 1}
+    return {"lj": float(np.abs(err_lj).max()), "es": float(np.abs(err_es).max()),
+            "lj_mean": float(np.mean(err_lj)), "tot_lj": tot["lj"], "tot_es": tot["es"]}
+
+
+SE_TOL = {"lj": 0.03, "es": 2e-4, "tot": 1e-12}
+
+
+def _se_within(r):
+    return r["lj"] < SE_TOL["lj"] and r["es"] < SE_TOL["es"] and \
+        abs(r["tot_lj"]) < SE_TOL["tot"] and abs(r["tot_es"]) < SE_TOL["tot"]
+
+
+def test_se_matches_closed_forms():
+    r = se_closed_form_errors()
+    assert _se_within(r), r
+
+
+@pytest.mark.parametrize("mutant", [1, 2, 3, 4],
+                         ids=["exclusion_at_sigma", "geometric_sigma", "rho_dropped", "cc_without_NA"])
+def test_se_closed_forms_reject_planted_mistakes(mutant, tmp_path):
+    """The same checks on oracle builds with one planted mistake each
+    (ORC_MUTANT in rism_oracle.c) must fail."""
+    here = os.path.dirname(orc.HERE)
+    so = tmp_path / f"liboracle_m{mutant}.so"
+    subprocess.run(["gcc", "-O2", "-fopenmp", "-fPIC", "-std=gnu11", "-shared", f"-DORC_MUTANT={mutant}",
+                    "-o", str(so), os.path.join(orc.HERE, "rism_oracle.c"), "-lm"], check=True)
+    code = ("import json, sys; sys.path[:0] = [%r, %r]; import test_oracle as t; "
+            "print(json.dumps(t.se_closed_form_errors()))" % (os.path.join(here, "tests"), here))
+    env = dict(os.environ, RISM_ORACLE_LIB=str(so))
+    out = subprocess.run([os.sys.executable, "-c", code], env=env, capture_output=True, text=True, check=True)
+    r = __import__("json").loads(out.stdout.strip().splitlines()[-1])
+    assert not _se_within(r), r
