@@ -257,10 +257,11 @@ __global__ __launch_bounds__(kThreads) void k_closure_fx(IterArgs<R, S> a) {
 // ---------------------------------------------------------------------
 template <typename R, int EPT, bool INV, int NL = 0>
 __global__ __launch_bounds__(kThreads) void k_fft_y(typename C2<R>::T *Z, const Ctl *ctl, int nx,
-                                                     int ny_rt, int nz, long N, int T_rt, int P, AxisPlan py) {
+                                                     int ny_rt, int nz, long N, int T_rt, int P, AxisPlan py,
+                                                     int gate) {
   // NL > 0: compile-time line length (power of two) with T = EPT * kThreads / NL
   using T2 = typename C2<R>::T;
-  if (ctl && ctl->run_iter != ctl->cur) return;
+  if (ctl && !gate_open(ctl, gate)) return;
   extern __shared__ unsigned char smem_raw[];
   T2 *lds = reinterpret_cast<T2 *>(smem_raw);
   const int ny = NL > 0 ? NL : ny_rt;
@@ -323,7 +324,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_oz(IterArgs<R, S> a) {
   // NL > 0: compile-time z length (power of two), one pack (VC == 2), T = EPT * kThreads / (2 NL)
   using T2 = typename C2<R>::T;
   using T4 = typename R4T<R>::T;
-  if (a.ctl->run_iter != a.ctl->cur) return;
+  if (!gate_open(a.ctl, a.gate)) return;
   extern __shared__ unsigned char smem_raw[];
   T2 *lds = reinterpret_cast<T2 *>(smem_raw);
   const int nx = a.nx, ny = a.ny, V = a.V;
@@ -573,6 +574,7 @@ static __device__ void anderson_solve_block(Ctl *ctl, const double *partials, in
   if (threadIdx.x != 0) return;
   const int iter = ctl->cur;
   ctl->cur = iter + 1;
+  if (ctl->ring) ctl->rlast = ctl->rcur;
   const double rms = sqrt(tot[0] / ctl->npts);  // cal_rms.cu:18-20
   ctl->step += 1;
   ctl->total += 1;
@@ -592,6 +594,23 @@ static __device__ void anderson_solve_block(Ctl *ctl, const double *partials, in
   }
   const int H = ctl->nhist;
   const int w = ctl->write;
+  if (ctl->ring) {  // mixed precision: refinement schedule (see Ctl)
+    if (ctl->refresh_iter == iter) {
+      ctl->ref_rms = rms;
+      ctl->best_rms = rms;
+      ctl->best_at = ctl->total;
+      ctl->since_ref = 0;
+    } else {
+      ctl->since_ref += 1;
+      if (rms < ctl->best_rms) {
+        ctl->best_rms = rms;
+        ctl->best_at = ctl->total;
+      }
+    }
+    if (ctl->step < ctl->maxstep &&
+        (rms < 0.1 * ctl->ref_rms || (ctl->since_ref >= 24 && ctl->total - ctl->best_at >= 24)))
+      ctl->refresh_iter = iter + 1;
+  }
   if (ctl->count > 0) {  // Picard warm-up, anderson.cu:43-49
     ctl->count -= 1;
     ctl->upd_nf = 0;
@@ -616,6 +635,13 @@ static __device__ void anderson_solve_block(Ctl *ctl, const double *partials, in
     ctl->upd_mix = ctl->m;
   }
   ctl->upd_w = w;
+  if (ctl->ring) {  // file R into slot w by relabelling; the update reads the old map
+    ctl->upd_rcur = ctl->rcur;
+    for (int s = 0; s < H; ++s) ctl->upd_rmap[s] = ctl->rmap[s];
+    const int freed = ctl->rmap[w];
+    ctl->rmap[w] = ctl->rcur;
+    ctl->rcur = freed;
+  }
   ctl->write = (w + 1) % H;
   if (ctl->filled < H) ctl->filled += 1;
   ctl->upd_iter = iter + 1;
@@ -629,8 +655,8 @@ static __device__ void anderson_solve_block(Ctl *ctl, const double *partials, in
 
 template <int HM>
 __global__ __launch_bounds__(kThreads) void k_anderson_solve(Ctl *ctl, const double *partials, int nblk,
-                                                             double *rms_hist) {
-  if (ctl->run_iter != ctl->cur) return;
+                                                             double *rms_hist, int gate) {
+  if (!gate_open(ctl, gate)) return;
   anderson_solve_block<HM>(ctl, partials, nblk, rms_hist);
 }
 
@@ -974,15 +1000,15 @@ hipError_t launch_fft_y(const IterArgs<R, S> &a, const LaunchGeom &g, bool inver
   if (inverse) {
     RISM_CT_SWITCH(nl, g.ept_y,
                    RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, true>), grid, block, g.lds_y, st, a.Z,
-                                                               a.ctl, a.nx, a.ny, a.nz, a.N, g.Ty, a.P, a.py)),
+                                                               a.ctl, a.nx, a.ny, a.nz, a.N, g.Ty, a.P, a.py, a.gate)),
                    hipLaunchKernelGGL((k_fft_y<R, E, true, NLV>), grid, block, g.lds_y, st, a.Z, a.ctl, a.nx,
-                                      a.ny, a.nz, a.N, g.Ty, a.P, a.py))
+                                      a.ny, a.nz, a.N, g.Ty, a.P, a.py, a.gate))
   } else {
     RISM_CT_SWITCH(nl, g.ept_y,
                    RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, false>), grid, block, g.lds_y, st, a.Z,
-                                                               a.ctl, a.nx, a.ny, a.nz, a.N, g.Ty, a.P, a.py)),
+                                                               a.ctl, a.nx, a.ny, a.nz, a.N, g.Ty, a.P, a.py, a.gate)),
                    hipLaunchKernelGGL((k_fft_y<R, E, false, NLV>), grid, block, g.lds_y, st, a.Z, a.ctl, a.nx,
-                                      a.ny, a.nz, a.N, g.Ty, a.P, a.py))
+                                      a.ny, a.nz, a.N, g.Ty, a.P, a.py, a.gate))
   }
   return hipGetLastError();
 }
@@ -1014,11 +1040,11 @@ hipError_t launch_fx_resid(const IterArgs<R, S> &a, const LaunchGeom &g, int hm,
 }
 
 inline hipError_t launch_anderson_solve_impl(Ctl *ctl, const double *partials, int nblk, int nv, int hm,
-                                 double *rms_hist, hipStream_t st) {
+                                 double *rms_hist, hipStream_t st, int gate) {
   switch (hm) {
-    case 2: hipLaunchKernelGGL(k_anderson_solve<2>, dim3(1), dim3(kThreads), 0, st, ctl, partials, nblk, rms_hist); break;
-    case 5: hipLaunchKernelGGL(k_anderson_solve<5>, dim3(1), dim3(kThreads), 0, st, ctl, partials, nblk, rms_hist); break;
-    case 10: hipLaunchKernelGGL(k_anderson_solve<10>, dim3(1), dim3(kThreads), 0, st, ctl, partials, nblk, rms_hist); break;
+    case 2: hipLaunchKernelGGL(k_anderson_solve<2>, dim3(1), dim3(kThreads), 0, st, ctl, partials, nblk, rms_hist, gate); break;
+    case 5: hipLaunchKernelGGL(k_anderson_solve<5>, dim3(1), dim3(kThreads), 0, st, ctl, partials, nblk, rms_hist, gate); break;
+    case 10: hipLaunchKernelGGL(k_anderson_solve<10>, dim3(1), dim3(kThreads), 0, st, ctl, partials, nblk, rms_hist, gate); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -1039,11 +1065,11 @@ hipError_t launch_fft3d(typename C2<R>::T *data, const AxisPlan &px, const AxisP
   const int ez = (g.Tz * nz <= 4 * kThreads) ? 4 : (g.Tz * nz <= 8 * kThreads) ? 8 : (g.Tz * nz <= 16 * kThreads) ? 16 : 32;
   if (dir < 0) {
     RISM_EPTX_SWITCH(g.ept_x, hipLaunchKernelGGL((k_fft_x_pure<R, E, false>), gx, block, ldsx, st, data, nx, L, px, (R)dv))
-    RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, false>), gy, block, ldsy, st, data, (const Ctl *)nullptr, nx, ny, nz, N, g.Ty, 1, py))
+    RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, false>), gy, block, ldsy, st, data, (const Ctl *)nullptr, nx, ny, nz, N, g.Ty, 1, py, 0))
     RISM_EPT_SWITCH(ez, hipLaunchKernelGGL((k_fft_z_pure<R, E, false>), gz, block, ldsz, st, data, nx, ny, nz, g.Tz, pz))
   } else {
     RISM_EPT_SWITCH(ez, hipLaunchKernelGGL((k_fft_z_pure<R, E, true>), gz, block, ldsz, st, data, nx, ny, nz, g.Tz, pz))
-    RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, true>), gy, block, ldsy, st, data, (const Ctl *)nullptr, nx, ny, nz, N, g.Ty, 1, py))
+    RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, true>), gy, block, ldsy, st, data, (const Ctl *)nullptr, nx, ny, nz, N, g.Ty, 1, py, 0))
     RISM_EPTX_SWITCH(g.ept_x, hipLaunchKernelGGL((k_fft_x_pure<R, E, true>), gx, block, ldsx, st, data, nx, L, px, (R)ivol))
   }
   return hipGetLastError();

@@ -1,0 +1,395 @@
+// k_mixed_impl.h -- the closure and residual passes of the mixed-precision
+// iteration on gfx950: FP32 fields relative to an FP64 reference.
+//
+// The fixed point is the one of the FP64 path (src-cuda/calculate.cu:4-58,
+// anderson.cu:35-84); only the representation differs.  With ts the
+// short-range state and u the closure potential,
+//   c_s = G(ts - u) - 1 - ts,   h = K c (FFT, chi, inverse FFT),
+//   R = h + 1 - G(ts - u).
+// The state is ts = T + dl: T (FP64) is ts at the last refresh and dl (FP32)
+// the change since then; A = T - u (FP64) is the closure argument at the
+// refresh.  (ts itself is kept: near the solute's core u reaches 1e29, and
+// ts - u cannot carry ts there, while c_s = -1 - ts needs it.)  A refresh
+// step (R = double, gate 2) folds dl into T and evaluates the map in FP64,
+// storing R_ref = R (FP32).  Every other step (R = float, gate 1)
+// transforms only the change of c_s since the refresh,
+//   dc = G(A + dl) - G(A) - dl,
+// whose long-range part is zero, in FP32; the residual is then
+//   R = K dc + B,   B = R_ref - (G(A + dl) - G(A)),
+// with G, dc and B formed in FP64 and B handed to the residual pass in
+// FP32.  Rounding thus scales with |dc| and |B|, which shrink as the solve
+// converges, not with |c_s| or |h|.
+//
+// Anderson state (FP32 in both chains): hx [H][V][N] holds the step
+// increments of ts by slot (age order as in the FP64 update, written by the
+// pending update), rring [H+1][V][N] the residuals, addressed through the
+// control block's ring map (Ctl::rcur, rmap): the residual pass writes the
+// current R into ring slot rcur and the solve kernel files it into history
+// by relabelling, so no pass copies a residual.
+#pragma once
+#include "k_iterate_impl.h"
+
+namespace rism {
+
+// ---------------------------------------------------------------------
+// 1. pending Anderson update of x + closure + c_s (FP64) or dc (FP32),
+// packed two sites per complex value, forward FFT along x
+// ---------------------------------------------------------------------
+template <typename R, int EPT, int VC, int HM, bool UPD>
+__device__ __forceinline__ void mx_points(const IterArgs<R, double> &a, typename C2<R>::T *lds, bool do_run,
+                                          const double *th, const long *toff, const long *roff, long rcur_off,
+                                          double mix, int w, int nf) {
+  using T2 = typename C2<R>::T;
+  constexpr bool F64 = sizeof(R) == sizeof(double);  // the refresh chain
+  constexpr int CH = EPT < RISM_CLOS_CH ? EPT : RISM_CLOS_CH;
+  const int nx = a.nx, L = a.Lc, ls = line_stride(nx), V = a.V;
+  const long N = a.N, VN = (long)V * N;
+  const long base = (long)blockIdx.x * L * nx;
+  const int nel = L * nx;
+  const T2 *__restrict__ pre = (const T2 *)a.px.pre;
+  double *__restrict__ Ab = a.A;
+  double *__restrict__ Tb = a.ts;  // the refresh chain's T
+  float *__restrict__ dlb = a.dl;
+  const float *__restrict__ hx = a.hx;
+  float *__restrict__ hxw = a.hx;
+  const float *__restrict__ rr = a.rring;
+  const float *__restrict__ rref = a.rref;
+  const double *__restrict__ uu = (const double *)a.u;
+#pragma unroll
+  for (int k0 = 0; k0 < EPT; k0 += CH) {
+    double A[CH][VC], d0[CH][VC], u[CH][VC], rf[CH][VC];  // A: T in the refresh chain
+    float r[CH][VC], hti[CH][VC][UPD ? HM : 1], hri[CH][VC][UPD ? HM : 1];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const unsigned e = (unsigned)min((int)threadIdx.x + (k0 + c) * kThreads, nel - 1);
+#pragma unroll
+      for (int v = 0; v < VC; ++v) {
+        if (v < V) {
+          const long vb = (long)v * N + base;
+          A[c][v] = F64 ? lane_at(Tb + vb, e) : lane_at(Ab + vb, e);
+          d0[c][v] = (double)lane_at(dlb + vb, e);
+          if constexpr (F64) u[c][v] = lane_at(uu + vb, e);
+          else rf[c][v] = (double)lane_at(rref + vb, e);
+          if (UPD) {
+            r[c][v] = lane_at(rr + rcur_off + vb, e);
+#pragma unroll
+            for (int s = 0; s < (UPD ? HM : 0); ++s) {
+              hti[c][v][s] = lane_at(hx + toff[s] + vb, e);
+              hri[c][v][s] = lane_at(rr + roff[s] + vb, e);
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const unsigned e = threadIdx.x + (k0 + c) * kThreads;
+      const bool valid = e < (unsigned)nel;
+      R cs[VC + 1];
+#pragma unroll
+      for (int v = 0; v < VC; ++v) {
+        cs[v] = (R)0;
+        if (v < V) {
+          const long vb = (long)v * N + base;
+          // the update as a change of ts: ts_new - ts = sum theta_s (ts_s - ts) + m Rbar
+          double d = d0[c][v];
+          if (UPD) {  // newdt / newdt0 (anderson_cuda.cu:1-23) on the increment history
+            const double rv = (double)r[c][v];
+            double tb = 0.0, rb = rv, back = 0.0;
+#pragma unroll
+            for (int s = 0; s < (UPD ? HM : 0); ++s) {
+              // slots beyond the filled ones: valid loads, values discarded
+              const double hts = s < nf ? (double)hti[c][v][UPD ? s : 0] : 0.0;
+              const double hrs = s < nf ? (double)hri[c][v][UPD ? s : 0] : rv;
+              back -= hts;  // age s+1: ts_(s+1) - ts = -(increments of ages 1..s+1)
+              tb += th[s] * back;
+              rb += th[s] * (hrs - rv);
+            }
+            d += tb + mix * rb;
+          }
+          if constexpr (F64) {
+            // refresh: T takes the change, A = T - u is the new reference
+            const double Tn = A[c][v] + d, An = Tn - u[c][v];
+            if (UPD && valid) lane_at(hxw + (long)w * VN + vb, e) = (float)(d - d0[c][v]);
+            if (valid && (UPD || do_run)) {
+              lane_at(Tb + vb, e) = Tn;
+              lane_at(Ab + vb, e) = An;
+              lane_at(dlb + vb, e) = 0.0f;
+            }
+            if (do_run) cs[v] = (R)(closure_g<double>(An, a.closure) - 1.0 - Tn);
+          } else {
+            // the change since the refresh is stored in FP32: round first, then evaluate
+            const float dn = (float)d;
+            if (UPD && valid) {
+              lane_at(hxw + (long)w * VN + vb, e) = (float)((double)dn - d0[c][v]);
+              lane_at(dlb + vb, e) = dn;
+            }
+            if (do_run) {
+              const double dg = closure_g<double>(A[c][v] + (double)dn, a.closure) - closure_g<double>(A[c][v], a.closure);
+              if (valid) lane_at(a.bq + vb, e) = (float)(rf[c][v] - dg);
+              cs[v] = (R)(dg - (double)dn);
+            }
+          }
+        }
+      }
+      if (do_run && valid) {
+        const int i = e / nx, x = e - i * nx;
+        cs[VC] = (R)0;
+        const T2 pw = pre[x];
+#pragma unroll
+        for (int p = 0; p < (VC + 1) / 2; ++p)
+          if (2 * p < V) lds[(p * L + i) * ls + lpad(x)] = cmul(mk<T2>(cs[2 * p], cs[2 * p + 1]), pw);
+      }
+    }
+    // keep the next batch's loads out of this one (register pressure)
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+template <typename R, int EPT, int VC, int HM>
+__global__ __launch_bounds__(kThreads) void k_mx_closure_fx(IterArgs<R, double> a) {
+  using T2 = typename C2<R>::T;
+  const Ctl *ctl = a.ctl;
+  const int iter = ctl->cur;
+  const bool mine = chain_active(ctl, a.gate);
+  const bool do_upd = mine && ctl->upd_iter == iter;
+  const bool do_run = mine && ctl->run_iter == iter;
+  if (!do_upd && !do_run) return;
+  extern __shared__ unsigned char smem_raw[];
+  T2 *lds = reinterpret_cast<T2 *>(smem_raw);
+  const int nx = a.nx, L = a.Lc, ls = line_stride(nx), P = a.P;
+  const long N = a.N, VN = (long)a.V * N;
+  const long base = (long)blockIdx.x * L * nx;
+  const int nel = L * nx;
+  // update coefficients by age; the residual of each slot through the ring
+  // map of this step; slots beyond the filled ones get a zero coefficient
+  // and a valid address
+  int w = 0, nf = 0;
+  double mix = 0.0;
+  double th[HM];
+  long toff[HM], roff[HM];
+  long rcur_off = 0;
+#pragma unroll
+  for (int k = 0; k < HM; ++k) {
+    th[k] = 0.0;
+    toff[k] = roff[k] = 0;
+  }
+  if (do_upd) {
+    nf = ctl->upd_nf;
+    w = ctl->upd_w;
+    mix = ctl->upd_mix;
+    const int nh = ctl->nhist;
+    rcur_off = (long)ctl->upd_rcur * VN;
+#pragma unroll
+    for (int k = 0; k < HM; ++k) {
+      const int slot = (((w - 1 - k) % nh) + nh) % nh;
+      th[k] = (k < nf) ? ctl->upd_theta[slot] : 0.0;
+      toff[k] = (long)slot * VN;
+      roff[k] = (long)ctl->upd_rmap[slot] * VN;
+    }
+  }
+  if (do_upd)
+    mx_points<R, EPT, VC, HM, true>(a, lds, do_run, th, toff, roff, rcur_off, mix, w, nf);
+  else
+    mx_points<R, EPT, VC, HM, false>(a, lds, do_run, th, toff, roff, rcur_off, mix, w, nf);
+  if (!do_run) return;
+  T2 *ltw = lds + P * L * ls;
+  stage_tables(ltw, a.px);
+  __syncthreads();
+  fft_lines<EPT, false, 5>(lds, P * L, ls, a.pxr, ltw);
+  const R dv = (R)a.dv;
+  T2 *__restrict__ Z = a.Z;
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int f = threadIdx.x + k * kThreads;  // (pack, element) flattened
+    if (f < P * nel) {
+      const int p = f / nel, e = f - p * nel;
+      const int i = e / nx, j = e - i * nx;
+      const R sc = dv * (R)jsign(j, a.px.fixsign);
+      Z[p * N + base + e] = cscale(lds[(p * L + i) * ls + lpad(j)], sc);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
+// 5. inverse x + residual into the ring + Anderson inner products
+// ---------------------------------------------------------------------
+template <typename R, int EPT, int HM, int VC>
+__global__ __launch_bounds__(kThreads) void k_mx_resid(IterArgs<R, double> a) {
+  using T2 = typename C2<R>::T;
+  constexpr bool F64 = sizeof(R) == sizeof(double);
+  constexpr int NV = 1 + HM + HM * (HM + 1) / 2;
+  const Ctl *ctl = a.ctl;
+  if (!gate_open(ctl, a.gate)) return;
+  extern __shared__ unsigned char smem_raw[];
+  T2 *lds = reinterpret_cast<T2 *>(smem_raw);
+  const int nx = a.nx, L = a.Lx, ls = line_stride(nx), V = a.V, P = a.P;
+  const long N = a.N, VN = (long)V * N;
+  const int nel = L * nx;
+  const int groups = (int)((long)a.ny * a.nz / L);
+  const int nf = ctl->filled < HM ? ctl->filled : HM;
+  long hoff[HM];
+#pragma unroll
+  for (int s = 0; s < HM; ++s) hoff[s] = (long)ctl->rmap[s < nf ? s : 0] * VN;
+  float *__restrict__ rw = a.rring + (long)ctl->rcur * VN;
+  const float *__restrict__ rr = a.rring;
+  T2 *ltw = lds + P * L * ls;
+  const T2 *pre = ltw + nx;
+  stage_tables(ltw, a.px);  // visible after the first barrier of the group loop
+  const R ivol = (R)a.ivol;
+  T2 *__restrict__ Z = a.Z;
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+  T2 zb[EPT];
+  auto load_z = [&](int g, int lane) {
+    const long gb = (long)g * nel;
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const int f = min(lane + k * kThreads, P * nel - 1);
+      const int p = f / nel, e = f - p * nel;
+      zb[k] = Z[p * N + gb + e];
+    }
+  };
+  load_z(blockIdx.x, threadIdx.x);
+  for (int grp = blockIdx.x; grp < groups; grp += gridDim.x) {
+    const long base = (long)grp * nel;
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    __syncthreads();
+    constexpr int PMIN = VC <= 2 ? 1 : (VC <= 4 ? 2 : 3);
+    constexpr int KPT = (EPT + PMIN - 1) / PMIN;
+    constexpr int CH = KPT < RISM_RESID_CH ? KPT : RISM_RESID_CH;
+    constexpr bool PREH = CH * VC * HM <= 16;
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const int f = tid + k * kThreads;
+      if (f < P * nel) {
+        const int p = f / nel, e = f - p * nel;
+        const int i = e / nx, j = e - i * nx;
+        lds[(p * L + i) * ls + lpad(j)] = cscale(zb[k], (R)jsign(j, a.px.fixsign));
+      }
+    }
+    __syncthreads();
+    int tidf = tid;
+    asm volatile("" : "+v"(tidf));
+    if constexpr (F64)
+      fft_lines<EPT, true, 5>(lds, P * L, ls, a.pxr, ltw, tidf);
+    else
+      fft_lines<EPT, true>(lds, P * L, ls, a.px, ltw, tidf);
+    int tidr = tid;
+    asm volatile("" : "+v"(tidr));
+    if (grp + (int)gridDim.x < groups) load_z(grp + (int)gridDim.x, tidr);
+#pragma unroll
+    for (int k0 = 0; k0 < KPT; k0 += CH) {
+      double hv[CH][VC + 1];
+      double ov[CH][VC];  // FP64: A (g = G(A)); FP32: B
+      float hq[CH][VC][PREH ? HM : 1];
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const unsigned e = (unsigned)min(tidr + (k0 + c) * kThreads, nel - 1);
+        const int i = e / nx, x = e - i * nx;
+        const T2 pw = pre[x];
+#pragma unroll
+        for (int v = 0; v < VC; ++v)
+          if (v < V) {
+            const long vb = (long)v * N + base;
+            if constexpr (F64) ov[c][v] = lane_at(a.A + vb, e);
+            else ov[c][v] = (double)lane_at(a.bq + vb, e);
+#pragma unroll
+            for (int s = 0; s < (PREH ? HM : 0); ++s) hq[c][v][PREH ? s : 0] = lane_at(rr + hoff[s] + vb, e);
+          }
+#pragma unroll
+        for (int p = 0; p < (VC + 1) / 2; ++p) {
+          if (p < P) {
+            const T2 z = cscale(cmulc(lds[(p * L + i) * ls + lpad(x)], pw), ivol);
+            hv[c][2 * p] = (double)z.x;
+            hv[c][2 * p + 1] = (double)z.y;
+          }
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const unsigned e = tidr + (k0 + c) * kThreads;
+        if (k0 + c < KPT && e < (unsigned)nel) {
+#pragma unroll
+          for (int v = 0; v < VC; ++v) {
+            if (v < V) {
+              const long vb = (long)v * N + base;
+              double rd;
+              if constexpr (F64) {
+                rd = hv[c][v] + 1.0 - closure_g<double>(ov[c][v], a.closure);  // tr: R = h + 1 - g
+                lane_at(a.rref + vb, e) = (float)rd;
+              } else {
+                rd = hv[c][v] + ov[c][v];  // R = K dc + B
+              }
+              const float rq = (float)rd;
+              lane_at(rw + vb, e) = rq;
+              // the dot products see the stored residual
+              rd = (double)rq;
+              const double wgt = a.rho_inv[v];
+              acc[0] += rd * rd;
+              double d[HM];
+#pragma unroll
+              for (int s = 0; s < HM; ++s)
+                d[s] = (s < nf) ? rd - (double)(PREH ? hq[c][v][PREH ? s : 0] : lane_at(rr + hoff[s] + vb, e)) : 0.0;
+#pragma unroll
+              for (int s = 0; s < HM; ++s) acc[1 + s] += wgt * rd * d[s];
+              int kk = 1 + HM;
+#pragma unroll
+              for (int s = 0; s < HM; ++s)
+#pragma unroll
+                for (int s2 = s; s2 < HM; ++s2) acc[kk++] += wgt * d[s] * d[s2];
+            }
+          }
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  __shared__ double red[kThreads / 64][NV];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const double sw = wave_sum(acc[k]);
+    if (lane == 0) red[wave][k] = sw;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < NV; k += kThreads) {
+    double sm = 0.0;
+    for (int wv = 0; wv < kThreads / 64; ++wv) sm += red[wv][k];
+    a.partials[(long)blockIdx.x * NV + k] = sm;
+  }
+}
+
+template <typename R>
+hipError_t launch_mx_closure_fx_impl(const IterArgs<R, double> &a, const LaunchGeom &g, int hm, hipStream_t st) {
+  const dim3 grid(g.cgroups), block(kThreads);
+#define RISM_MXC(HMV) \
+  RISM_VC_SWITCH(a.V, RISM_EPTX_SWITCH(g.ept_c, hipLaunchKernelGGL((k_mx_closure_fx<R, E, VC, HMV>), grid, block, g.lds_c, st, a)))
+  switch (hm) {
+    case 2: RISM_MXC(2); break;
+    case 5: RISM_MXC(5); break;
+    case 10: RISM_MXC(10); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef RISM_MXC
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_mx_resid_impl(const IterArgs<R, double> &a, const LaunchGeom &g, int hm, hipStream_t st) {
+  const dim3 grid(g.xblocks), block(kThreads);
+#define RISM_MXR(HMV) \
+  RISM_VC_SWITCH(a.V, RISM_EPTX_SWITCH(g.ept_x, hipLaunchKernelGGL((k_mx_resid<R, E, HMV, VC>), grid, block, g.lds_x, st, a)))
+  switch (hm) {
+    case 2: RISM_MXR(2); break;
+    case 5: RISM_MXR(5); break;
+    case 10: RISM_MXR(10); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef RISM_MXR
+  return hipGetLastError();
+}
+
+}  // namespace rism

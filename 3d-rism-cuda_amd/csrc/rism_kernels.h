@@ -60,7 +60,36 @@ struct Ctl {
   double tol, m, mp;
   double rms;
   double npts;    // V*N
+  // ---- mixed precision (ring = 1) ----
+  // The residuals live in a ring of nhist + 1 slots: the residual pass
+  // writes the current R into slot rcur, and the history slot s (0..nhist-1)
+  // is ring slot rmap[s].  When the solve kernel files R into history slot
+  // w it only relabels (rmap[w] <- rcur, and rcur takes the freed slot); the
+  // pending update reads through the map of its own step (upd_rcur,
+  // upd_rmap).
+  int ring;
+  int rcur, upd_rcur;
+  int rlast;      // ring slot of the last residual pass's R
+  int rmap[kHMax + 1], upd_rmap[kHMax + 1];
+  // Refinement schedule, decided on the device after every step: the step
+  // refresh_iter runs the FP64 chain (a new reference A = x, and R_ref),
+  // every other step the FP32 chain on differences to that reference.  A
+  // refresh follows when the residual has fallen 10x below the one at the
+  // last refresh, or has not improved for 24 steps (the FP32 floor).
+  int refresh_iter;
+  int since_ref, best_at;
+  double ref_rms, best_rms;
 };
+
+// Which chain of kernels runs at the current step.  gate 0: the FP64 path
+// (every step); 1: the FP32 chain of the mixed path (steps that are no
+// refresh); 2: its FP64 refresh chain.
+__device__ __forceinline__ bool chain_active(const Ctl *c, int gate) {
+  return gate == 0 || ((c->refresh_iter == c->cur) == (gate == 2));
+}
+__device__ __forceinline__ bool gate_open(const Ctl *c, int gate) {
+  return c->run_iter == c->cur && chain_active(c, gate);
+}
 
 // Residual / history storage type: FP64 in the FP64 path; FP32 in the mixed
 // path, where the t history is kept as FP32 step increments t_{k+1} - t_k
@@ -116,6 +145,21 @@ struct IterArgs {
   int href_only;        // FP64 residual pass: store h only
   int use_fk;           // z pass subtracts q_v cf fk(k)
   double *rms_hist;
+  // ---- mixed precision on differences to an FP64 reference (gate 1, 2) ----
+  // ts = T + dl: T [V][N] (FP64, in the ts field) is ts at the last refresh,
+  // dl [V][N] (FP32) the change since; A [V][N] (FP64) = T - u, the closure
+  // argument at the refresh; rref [V][N] (FP32) the refresh step's FP64
+  // residual, hx [H][V][N] (FP32) the step increments of ts, rring
+  // [H+1][V][N] (FP32) the residual ring (Ctl::rmap).  The FP32 chain
+  // transforms dc = G(A + dl) - G(A) - dl (the change of c_s, whose
+  // long-range part is constant) and the residual is R = K dc + B with
+  // B = R_ref - (G(A + dl) - G(A)) from the closure pass.
+  int gate;             // chain_active() selector
+  double *A;
+  float *dl;
+  float *rref;
+  float *bq;            // B, closure pass -> residual pass
+  float *hx, *rring;
 };
 
 // launch configuration chosen on the host for a grid
@@ -142,7 +186,17 @@ hipError_t launch_fft_z_oz(const IterArgs<R, S> &a, const LaunchGeom &g, hipStre
 template <typename R, typename S>
 hipError_t launch_fx_resid(const IterArgs<R, S> &a, const LaunchGeom &g, int hm, hipStream_t st);
 hipError_t launch_anderson_solve(Ctl *ctl, const double *partials, int nblk, int nv, int hm,
-                                 double *rms_hist, hipStream_t st);
+                                 double *rms_hist, hipStream_t st, int gate = 0);
+// mixed precision: closure / residual passes of the FP32 chain (R = float)
+// and of the FP64 refresh chain (R = double), k_mixed_f32.hip / k_mixed_f64.hip
+template <typename R>
+hipError_t launch_mx_closure_fx(const IterArgs<R, double> &a, const LaunchGeom &g, int hm, hipStream_t st);
+template <typename R>
+hipError_t launch_mx_resid(const IterArgs<R, double> &a, const LaunchGeom &g, int hm, hipStream_t st);
+// t_s <-> state conversions of the mixed path: dir 0: T = ts, A = ts - u,
+// dl = 0; dir 1: ts = T + dl
+hipError_t launch_mx_convert(double *ts, double *T, double *A, float *dl, const double *u, long n, int dir,
+                             hipStream_t st);
 // ---- adjoint of the converged fixed point (k_adjoint.hip) ----
 struct AdjArgs {
   long N;

@@ -87,7 +87,6 @@ struct PrecSet {
   AxisPlan plan_xr;  // x axis, radices <= 5 (closure pass, FP64 residual pass), same tables as plan[0]
   LaunchGeom geom{};
   void *u = nullptr, *fr = nullptr, *fkh = nullptr, *chi = nullptr, *Z = nullptr, *tables = nullptr;
-  float *gq = nullptr;  // FP32 passes: g handed from the closure pass to the residual pass
 };
 
 template <typename R> constexpr int pidx() { return sizeof(R) == sizeof(double) ? 0 : 1; }
@@ -117,7 +116,11 @@ struct rism_handle {
   double *ulj = nullptr, *de = nullptr, *fr = nullptr, *sr = nullptr;
   double *sel = nullptr, *seq = nullptr;  // [V][N] SE atom sums from the potentials pass
   double2 *fkh = nullptr, *phase = nullptr;
-  float *cref = nullptr;    // mixed: reference c_s of the refinement
+  // mixed precision state (k_mixed_impl.h): reference T of t_s and A of
+  // the closure argument, the change dl, the refresh residual R_ref, the closure
+  // pass's B, the increment history and the residual ring
+  double *mT = nullptr, *mA = nullptr;
+  float *mdl = nullptr, *mrref = nullptr, *mbq = nullptr, *mhx = nullptr, *mring = nullptr;
   double *ts = nullptr;
   void *res = nullptr, *hist_t = nullptr, *hist_r = nullptr;  // FP64, or FP32 in the mixed path
   size_t qsz = 8;           // bytes per residual / history element
@@ -139,9 +142,9 @@ struct rism_handle {
   // state
   bool potentials = false, solved = false, evaluated = false;
   double cf = 1.0;
-  bool delta = false;       // mixed: FP32 passes work on c_s - cref
-  int refreshes = 0;
-  // replayable chunks of iterations: [precision][delta mode]
+  int refreshes = 0;        // mixed: FP64 refresh steps of the last solve
+  bool mx_live = false;     // mixed: A, dl (not ts) hold the current state
+  // replayable chunks of iterations: [precision][0]
   hipGraphExec_t gexec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   int gchunk[2][2] = {{0, 0}, {0, 0}};
   double gcf[2][2] = {{0, 0}, {0, 0}};  // charge-up factor the graph was captured with
@@ -481,7 +484,7 @@ int choose_geometry(rism_handle *h, LaunchGeom &g, size_t rsz, const int *ovr = 
   return RISM_OK;
 }
 
-template <typename R> IterArgs<R, double> make_args(rism_handle *h) {
+template <typename R> IterArgs<R, double> make_args(rism_handle *h, bool force_mx = false) {
   IterArgs<R, double> a;
   std::memset(&a, 0, sizeof a);
   const PrecSet &ps = h->ps[pidx<R>()];
@@ -519,19 +522,55 @@ template <typename R> IterArgs<R, double> make_args(rism_handle *h) {
   a.xblocks = ps.geom.xblocks;
   a.store_h = 0;
   a.use_fk = 1;
-  if (pidx<R>() == 1) {
-    a.gq = ps.gq;
-    a.cref = h->cref;
-    a.cref_out = h->cref;
-    a.href = (const double2 *)h->ps[0].Z;
-    a.delta = h->delta ? 1 : 0;
-    a.use_fk = h->delta ? 0 : 1;
+  if (h->opts.precision == RISM_PREC_MIXED) {
+    if (pidx<R>() == 1 || force_mx) a.ts = h->mT;  // the mixed chains' T
+    a.A = h->mA;
+    a.dl = h->mdl;
+    a.rref = h->mrref;
+    a.bq = h->mbq;
+    a.hx = h->mhx;
+    a.rring = h->mring;
   }
   return a;
 }
 
+// One step of the mixed path: gate 1 the FP32 chain on differences to the
+// reference, gate 2 the FP64 refresh chain (k_mixed_impl.h).  Each launch
+// runs only at a step of its chain (the solve kernel decides the refreshes),
+// so both chains can sit in one stream or graph.
+int launch_mx_iteration(rism_handle *h, int gate, bool solve = true) {
+  if (gate == 2) {
+    IterArgs<double, double> a = make_args<double>(h, true);
+    const LaunchGeom &g = h->ps[0].geom;
+    a.gate = 2;
+    a.use_fk = 1;
+    PROF(h, 10, launch_mx_closure_fx<double>(a, g, h->hm, h->stream));
+    PROF(h, 10, launch_fft_y<double, double>(a, g, false, h->stream));
+    PROF(h, 10, launch_fft_z_oz<double, double>(a, g, h->stream));
+    PROF(h, 10, launch_fft_y<double, double>(a, g, true, h->stream));
+    PROF(h, 10, launch_mx_resid<double>(a, g, h->hm, h->stream));
+    if (solve)
+      PROF(h, 10, launch_anderson_solve(h->ctl, h->partials, g.xblocks, a.nv, h->hm, h->rms_hist, h->stream, 2));
+    return RISM_OK;
+  }
+  IterArgs<float, double> a = make_args<float>(h);
+  const LaunchGeom &g = h->ps[1].geom;
+  a.gate = 1;
+  a.use_fk = 0;  // the change of c_s has no long-range part
+  PROF(h, 0, launch_mx_closure_fx<float>(a, g, h->hm, h->stream));
+  PROF(h, 1, launch_fft_y<float, double>(a, g, false, h->stream));
+  PROF(h, 2, launch_fft_z_oz<float, double>(a, g, h->stream));
+  PROF(h, 3, launch_fft_y<float, double>(a, g, true, h->stream));
+  PROF(h, 4, launch_mx_resid<float>(a, g, h->hm, h->stream));
+  if (solve) PROF(h, 5, launch_anderson_solve(h->ctl, h->partials, g.xblocks, a.nv, h->hm, h->rms_hist, h->stream, 1));
+  return RISM_OK;
+}
+
 template <typename R> int launch_iteration(rism_handle *h, bool store_h, bool solve,
                                           bool h_only = false) {
+  if constexpr (pidx<R>() == 1) {
+    return launch_mx_iteration(h, 1, solve);  // the mixed path's FP32 chain
+  } else {
   IterArgs<R, double> a = make_args<R>(h);
   const LaunchGeom &g = h->ps[pidx<R>()].geom;
   a.store_h = store_h ? 1 : 0;
@@ -545,38 +584,35 @@ template <typename R> int launch_iteration(rism_handle *h, bool store_h, bool so
     PROF(h, 5, launch_anderson_solve(h->ctl, h->partials, g.xblocks, a.nv, h->hm, h->rms_hist,
                                      h->stream));
   return RISM_OK;
+  }
 }
 
-// Mixed precision, refinement step: the FP32 first pass (with the pending
-// Anderson update) stores c_s, rounded to FP32, as the new reference and
-// sends only the rounding remainder through the FP32 passes; the FP64 passes
-// transform the reference into h_ref (long-range term included) and the
-// FP32 residual pass adds it back.  From here on the FP32 passes run in
-// delta mode.
-int launch_refresh_iteration(rism_handle *h) {
-  h->delta = true;
-  ++h->refreshes;
-  IterArgs<float, double> a = make_args<float>(h);
-  const LaunchGeom &g = h->ps[1].geom;
-  a.refresh = 1;
-  PROF(h, 0, launch_closure_fx<float, double>(a, g, true, h->hm, h->stream));
-  IterArgs<double, double> d = make_args<double>(h);
-  const LaunchGeom &gd = h->ps[0].geom;
-  d.src = 1;
-  d.cref = h->cref;
-  d.href_only = 1;
-  d.store_h = 1;
-  PROF(h, 10, launch_closure_fx<double, double>(d, gd, false, h->hm, h->stream));
-  PROF(h, 10, launch_fft_y<double, double>(d, gd, false, h->stream));
-  PROF(h, 10, launch_fft_z_oz<double, double>(d, gd, h->stream));
-  PROF(h, 10, launch_fft_y<double, double>(d, gd, true, h->stream));
-  PROF(h, 10, launch_fx_resid<double, double>(d, gd, h->hm, h->stream));
-  a.refresh = 0;
-  PROF(h, 1, launch_fft_y<float, double>(a, g, false, h->stream));
-  PROF(h, 2, launch_fft_z_oz<float, double>(a, g, h->stream));
-  PROF(h, 3, launch_fft_y<float, double>(a, g, true, h->stream));
-  PROF(h, 4, launch_fx_resid<float, double>(a, g, h->hm, h->stream));
-  PROF(h, 5, launch_anderson_solve(h->ctl, h->partials, g.xblocks, a.nv, h->hm, h->rms_hist, h->stream));
+// the pending Anderson update alone (no closure / FFT): in the mixed path
+// through the closure pass of the chain whose step is current
+int launch_update_only(rism_handle *h) {
+  if (h->opts.precision == RISM_PREC_MIXED) {
+    if (h->ctl_host->refresh_iter == h->ctl_host->cur) {
+      IterArgs<double, double> a = make_args<double>(h, true);
+      a.gate = 2;
+      HIP_OK(launch_mx_closure_fx<double>(a, h->ps[0].geom, h->hm, h->stream));
+    } else {
+      IterArgs<float, double> a = make_args<float>(h);
+      a.gate = 1;
+      HIP_OK(launch_mx_closure_fx<float>(a, h->ps[1].geom, h->hm, h->stream));
+    }
+    return RISM_OK;
+  }
+  IterArgs<double, double> a = make_args<double>(h);
+  HIP_OK(launch_closure_fx<double, double>(a, h->ps[0].geom, true, h->hm, h->stream));
+  return RISM_OK;
+}
+
+// mixed path: the state (A, dl) from t_s (dir 0, a new reference) or t_s
+// from the state (dir 1), with the current stage's FP64 closure potential
+int mx_convert(rism_handle *h, int dir) {
+  if (h->opts.precision != RISM_PREC_MIXED) return RISM_OK;
+  HIP_OK(launch_mx_convert(h->ts, h->mT, h->mA, h->mdl, (const double *)h->ps[0].u, (long)h->V * h->N, dir,
+                           h->stream));
   return RISM_OK;
 }
 
@@ -588,12 +624,8 @@ template <typename R> int stage_potential(rism_handle *h) {
   return RISM_OK;
 }
 
-int stage_potentials(rism_handle *h) {
-  if (int rc = stage_potential<double>(h)) return rc;
-  if (h->opts.precision == RISM_PREC_MIXED)
-    if (int rc = stage_potential<float>(h)) return rc;
-  return RISM_OK;
-}
+// (the mixed path's FP32 chain works on differences and needs no potential)
+int stage_potentials(rism_handle *h) { return stage_potential<double>(h); }
 
 int put_ctl(rism_handle *h) {
   HIP_OK(hipMemcpyAsync(h->ctl, h->ctl_host, sizeof(Ctl), hipMemcpyHostToDevice, h->stream));
@@ -610,6 +642,8 @@ int get_ctl(rism_handle *h) {
 // full t = ts + q_v cf fr written to host [V][N]
 int fetch_t(rism_handle *h, double *out) {
   const size_t VN = (size_t)h->V * h->N;
+  if (h->mx_live)
+    if (int rc = mx_convert(h, 1)) return rc;
   std::vector<double> fr(h->N);
   HIP_OK(hcopy(h, out, h->ts, VN * sizeof(double), hipMemcpyDeviceToHost));
   HIP_OK(hcopy(h, fr.data(), h->fr, h->N * sizeof(double), hipMemcpyDeviceToHost));
@@ -629,6 +663,7 @@ int store_t(rism_handle *h, const double *t) {
     for (long i = 0; i < h->N; ++i) ts[(size_t)v * h->N + i] = t[(size_t)v * h->N + i] - q * fr[i];
   }
   HIP_OK(hcopy(h, h->ts, ts.data(), VN * sizeof(double), hipMemcpyHostToDevice));
+  h->mx_live = false;
   return RISM_OK;
 }
 
@@ -694,14 +729,14 @@ static void drop_graphs(rism_handle *h) {
 // nodes; of the handle's state they carry only q_v cf (captured again when
 // cf differs) and the tile shapes (the autotuner drops the graphs).
 template <typename R> int launch_chunk_graph(rism_handle *h, int n) {
-  hipGraphExec_t &ex = h->gexec[pidx<R>()][h->delta ? 1 : 0];
-  int &gn = h->gchunk[pidx<R>()][h->delta ? 1 : 0];
+  hipGraphExec_t &ex = h->gexec[pidx<R>()][0];
+  int &gn = h->gchunk[pidx<R>()][0];
   if (!h->graph_ok) {
     for (int k = 0; k < n; ++k)
       if (int rc = launch_iteration<R>(h, false, true)) return rc;
     return RISM_OK;
   }
-  double &gc = h->gcf[pidx<R>()][h->delta ? 1 : 0];
+  double &gc = h->gcf[pidx<R>()][0];
   if (!ex || gn != n || gc != h->cf) {
     if (ex) (void)hipGraphExecDestroy(ex);
     ex = nullptr;
@@ -739,15 +774,25 @@ template <typename R> int solve_impl(rism_handle *h, const char *tuv_path, int *
   const int V = h->V;
   const long N = h->N;
   const size_t VN = (size_t)V * N;
+  constexpr bool kMixed = pidx<R>() == 1;
+  const int H = h->opts.nhist;
   int cu = h->opts.chargeup;
   double cuf = 0.0;
   const bool verbose = h->opts.verbose != 0;
+  const bool trace = env_long("RISM_TRACE", 0) > 0;  // refresh steps of the mixed path on stderr
   h->evaluated = false;
   // history and state start from zero (anderson.cu allocates without clearing;
   // unfilled slots are never read here)
-  HIP_OK(hipMemsetAsync(h->hist_t, 0, (size_t)h->opts.nhist * VN * h->qsz, h->stream));
-  HIP_OK(hipMemsetAsync(h->hist_r, 0, (size_t)h->opts.nhist * VN * h->qsz, h->stream));
-  HIP_OK(hipMemsetAsync(h->res, 0, VN * h->qsz, h->stream));
+  float *hx32 = kMixed ? h->mhx : nullptr;
+  if (kMixed) {
+    HIP_OK(hipMemsetAsync(h->mhx, 0, (size_t)H * VN * sizeof(float), h->stream));
+    HIP_OK(hipMemsetAsync(h->mring, 0, (size_t)(H + 1) * VN * sizeof(float), h->stream));
+  } else {
+    HIP_OK(hipMemsetAsync(h->hist_t, 0, (size_t)H * VN * h->qsz, h->stream));
+    HIP_OK(hipMemsetAsync(h->hist_r, 0, (size_t)H * VN * h->qsz, h->stream));
+    HIP_OK(hipMemsetAsync(h->res, 0, VN * h->qsz, h->stream));
+  }
+  h->mx_live = false;
   if (file_exists(tuv_path)) {  // iterate.cu:25-33
     h->cf = 1.0;
     if (int rc = read_tuv_file(h, tuv_path)) return rc;
@@ -764,15 +809,22 @@ template <typename R> int solve_impl(rism_handle *h, const char *tuv_path, int *
   Ctl &c = *h->ctl_host;
   std::memset(&c, 0, sizeof c);
   c.count = h->opts.count;
-  c.nhist = h->opts.nhist;
+  c.nhist = H;
   c.maxstep = h->opts.maxstep;
   c.tol = h->opts.tol;
   c.m = h->opts.m;
   c.mp = h->opts.mp;
   c.npts = (double)VN;
   c.upd_iter = -1;
+  c.refresh_iter = -1;
+  if (kMixed) {  // residual ring: slot H holds the current R, slot s history s
+    c.ring = 1;
+    c.rcur = H;
+    for (int k = 0; k < H; ++k) c.rmap[k] = k;
+  }
   int status = RISM_SOLVE_CONVERGED;
   h->rms_log.clear();
+  h->refreshes = 0;
   // iteration chunks replay as HIP graphs unless timing or tracing every
   // step (RISM_GRAPHS=0 turns them off)
   const char *genv = std::getenv("RISM_GRAPHS");
@@ -783,6 +835,8 @@ template <typename R> int solve_impl(rism_handle *h, const char *tuv_path, int *
   int printed = 0;
   for (int stage = 0; stage <= cu; ++stage) {
     if (stage > 0) {  // add_tuv.cu: t += q_v cuf fr; ts absorbs any clamp remainder
+      if (kMixed)
+        if (int rc = mx_convert(h, 1)) return rc;  // ts of the state, with the old stage's u
       const double cf_old = h->cf;
       h->cf += cuf;
       if (h->cf > 1.0) h->cf = 1.0;
@@ -794,53 +848,44 @@ template <typename R> int solve_impl(rism_handle *h, const char *tuv_path, int *
         hcoef[v] = -h->qv[v] * dcf;  // history keeps the same t in the new frame
       }
       if (rem != 0.0) HIP_OK(launch_axpy_fr<double>(h->ts, h->fr, coef.data(), V, N, 1, h->stream));
-      if (h->qsz == sizeof(double)) {
-        HIP_OK(launch_axpy_fr<double>((double *)h->hist_t, h->fr, hcoef.data(), V, N, h->opts.nhist, h->stream));
+      if (!kMixed) {
+        HIP_OK(launch_axpy_fr<double>((double *)h->hist_t, h->fr, hcoef.data(), V, N, H, h->stream));
       } else if (c.filled > 0) {
-        // increment form: t_(k) - t moves by -q cuf fr for every age k, i.e.
+        // increment form: x_(k) - x moves by -q cuf fr for every age k, i.e.
         // the newest increment grows by q cuf fr
-        const int s1 = (c.write - 1 + h->opts.nhist) % h->opts.nhist;
+        const int s1 = (c.write - 1 + H) % H;
         std::vector<double> inc(V);
         for (int v = 0; v < V; ++v) inc[v] = h->qv[v] * cuf;
-        HIP_OK(launch_axpy_fr<float>((float *)h->hist_t + (size_t)s1 * VN, h->fr, inc.data(), V, N, 1,
-                                     h->stream));
+        HIP_OK(launch_axpy_fr<float>(hx32 + (size_t)s1 * VN, h->fr, inc.data(), V, N, 1, h->stream));
       }
     }
     if (int rc = stage_potentials(h)) return rc;
-    h->delta = false;  // a new stage (new cf) starts without a reference
-    double ref_rms = -1.0, best_rms = HUGE_VAL;
-    int best_at = 0, since_ref = 0;
-    bool want_refresh = false;
+    if (kMixed) {  // this stage's reference: A = ts - u, and a refresh first
+      if (int rc = mx_convert(h, 0)) return rc;
+      h->mx_live = true;
+    }
     if (verbose) std::printf("relaxing 3D UV RISM: Charge Up Factor = %g\n", h->cf);
     c.status = -1;
     c.step = 0;
     c.run_iter = c.cur;
     c.upd_iter = -1;
+    if (kMixed) c.refresh_iter = c.cur;
     if (int rc = put_ctl(h)) return rc;
     while (true) {
       int n_launch = (h->opts.ksave > 0) ? 1 : chunk;
-      if (pidx<R>() == 1 && want_refresh) {
-        if (int rc = launch_refresh_iteration(h)) return rc;
-        want_refresh = false;
-        since_ref = 0;
-        ref_rms = -2.0;  // set from the next rms read
+      if (kMixed && c.refresh_iter == c.cur) {  // the solve kernel asked for a refresh
+        if (trace) std::fprintf(stderr, "rism: refresh at step %d (rms %.3e)\n", c.total, c.rms);
+        if (int rc = launch_mx_iteration(h, 2)) return rc;
+        ++h->refreshes;
         n_launch = 1;
       } else if (graphs && n_launch > 1) {
+        // a refresh asked for inside the chunk turns its remaining launches off
         if (int rc = launch_chunk_graph<R>(h, n_launch)) return rc;
       } else {
         for (int k = 0; k < n_launch; ++k)
           if (int rc = launch_iteration<R>(h, false, true)) return rc;
       }
       if (int rc = get_ctl(h)) return rc;
-      if (pidx<R>() == 1 && c.status < 0) {
-        // refinement schedule: a new reference each time the residual has
-        // fallen 10x below the one at the last reference, or when it has
-        // not improved for 24 steps (the FP32 rounding floor)
-        if (ref_rms < 0.0) ref_rms = c.rms;
-        since_ref += n_launch;
-        if (c.rms < best_rms) { best_rms = c.rms; best_at = c.total; }
-        if (c.rms < 0.1 * ref_rms || (since_ref >= 24 && c.total - best_at >= 24)) want_refresh = true;
-      }
       // progress lines (iterate.cu:63)
       if (c.total > printed) {
         std::vector<double> tail(c.total - printed);
@@ -859,9 +904,12 @@ template <typename R> int solve_impl(rism_handle *h, const char *tuv_path, int *
         // apply the pending update now (one first pass), then clear it so
         // the next iteration does not apply it again
         if (c.upd_iter == c.cur) {
-          IterArgs<R, double> a = make_args<R>(h);
-          HIP_OK(launch_closure_fx<R, double>(a, h->ps[pidx<R>()].geom, true, h->hm, h->stream));
+          const int keep_run = c.run_iter;
+          c.run_iter = -1;  // update only
+          if (int rc = put_ctl(h)) return rc;
+          if (int rc = launch_update_only(h)) return rc;
           c.upd_iter = -1;
+          c.run_iter = keep_run;
           if (int rc = put_ctl(h)) return rc;
         }
         if (verbose) std::printf("saving Tuv to file:  %s  ...\n", tuv_path);
@@ -874,8 +922,7 @@ template <typename R> int solve_impl(rism_handle *h, const char *tuv_path, int *
     if (c.upd_iter >= 0 && c.upd_iter == c.cur) {
       c.run_iter = -1;  // update only, no closure/FFT
       if (int rc = put_ctl(h)) return rc;
-      IterArgs<R, double> a = make_args<R>(h);
-      HIP_OK(launch_closure_fx<R, double>(a, h->ps[pidx<R>()].geom, true, h->hm, h->stream));
+      if (int rc = launch_update_only(h)) return rc;
     }
     status = c.status;
     if (status == RISM_SOLVE_DIVERGED) {
@@ -920,7 +967,11 @@ template <typename R> int evaluate_impl(rism_handle *h) {
 // The final state is always re-evaluated with the FP64 passes (the mixed
 // path keeps an FP64 field set for its refinement), so h, g and every
 // thermodynamic sum come from FP64 fields.
-int evaluate(rism_handle *h) { return evaluate_impl<double>(h); }
+int evaluate(rism_handle *h) {
+  if (h->mx_live)  // t_s of the mixed path's final state
+    if (int rc = mx_convert(h, 1)) return rc;
+  return evaluate_impl<double>(h);
+}
 
 template <typename R> int thermo_impl(rism_handle *h, rism_xmu *x) {
   const int V = h->V;
@@ -985,14 +1036,13 @@ template <typename R> int alloc_precision(rism_handle *h) {
   const size_t VN = (size_t)h->V * h->N;
   if (int rc = build_plans<R>(h)) return rc;
   if (int rc = choose_geometry(h, ps.geom, sizeof(R))) return rc;
-  if (int rc = dalloc(h, &ps.u, VN * sizeof(R))) return rc;
+  // the FP64 set carries the potentials; the mixed path's FP32 chain works
+  // on differences to an FP64 reference and needs neither u nor the
+  // long-range terms
   if (sizeof(R) == sizeof(double)) {
+    if (int rc = dalloc(h, &ps.u, VN * sizeof(R))) return rc;
     ps.fr = h->fr;
     ps.fkh = h->fkh;
-  } else {
-    if (int rc = dalloc(h, &ps.fr, h->N * sizeof(R))) return rc;
-    if (int rc = dalloc(h, &ps.fkh, (size_t)h->nx * (h->ny / 2) * h->nz * sizeof(T2))) return rc;
-    if (int rc = dalloc_t(h, &ps.gq, VN)) return rc;
   }
   if (int rc = dalloc(h, &ps.Z, (size_t)h->P * h->N * sizeof(T2))) return rc;
   // chi spline table [V*V][ntab-1] of (y_i, c0, c1, c2)
@@ -1104,7 +1154,7 @@ int create_impl(const rism_grid *grid, const rism_solvent *sv, const rism_solute
   h->uxyz.assign(su->xyz, su->xyz + 3 * (size_t)su->natom);
   h->opts = *o;
   h->hm = o->nhist <= 2 ? 2 : (o->nhist <= 5 ? 5 : 10);
-  h->qsz = o->precision == RISM_PREC_MIXED ? sizeof(float) : sizeof(double);
+  h->qsz = sizeof(double);  // residual / history width of the FP64 path
   // k range check (solvent_spline.cu:18-21)
   double kmax2 = 0.0;
   for (int k = 0; k < 3; ++k) {
@@ -1122,10 +1172,7 @@ int create_impl(const rism_grid *grid, const rism_solvent *sv, const rism_solute
       (rc = dalloc_t(h, &h->de, h->N)) || (rc = dalloc_t(h, &h->fr, h->N)) ||
       (rc = dalloc_t(h, &h->sr, h->N)) || (rc = dalloc_t(h, &h->fkh, nh)) ||
       (rc = dalloc_t(h, &h->phase, (size_t)std::max(1, h->natu) * (h->nx + h->ny + h->nz))) ||
-      (rc = dalloc_t(h, &h->ts, VN)) || (rc = dalloc(h, &h->res, VN * h->qsz)) ||
-      (rc = dalloc(h, &h->hist_t, (size_t)o->nhist * VN * h->qsz)) ||
-      (rc = dalloc(h, &h->hist_r, (size_t)o->nhist * VN * h->qsz)) ||
-      (rc = dalloc_t(h, &h->gfield, VN)) || (rc = dalloc_t(h, &h->ctl, 1)) ||
+      (rc = dalloc_t(h, &h->ts, VN)) || (rc = dalloc_t(h, &h->gfield, VN)) || (rc = dalloc_t(h, &h->ctl, 1)) ||
       (rc = dalloc_t(h, &h->rms_hist, (size_t)(o->maxstep + 1) * (o->chargeup + 1) + 4)) ||
       (rc = dalloc_t(h, &h->atoms, std::max(1, h->natu))) || (rc = dalloc_t(h, &h->lj, (size_t)h->V * std::max(1, h->natu))) ||
       (rc = dalloc_t(h, &h->sigh, (size_t)h->V * std::max(1, h->natu))) || (rc = dalloc_t(h, &h->d_qv, h->V)) ||
@@ -1135,8 +1182,16 @@ int create_impl(const rism_grid *grid, const rism_solvent *sv, const rism_solute
   std::memset(h->ctl_host, 0, sizeof(Ctl));
   if ((rc = alloc_precision<double>(h))) return rc;
   if (o->precision == RISM_PREC_MIXED) {
-    if ((rc = alloc_precision<float>(h))) return rc;
-    if ((rc = dalloc_t(h, &h->cref, VN))) return rc;
+    // references T, A (FP64), change, R_ref, B (FP32), increment history and
+    // the residual ring of H + 1 slots (FP32)
+    if ((rc = alloc_precision<float>(h)) || (rc = dalloc_t(h, &h->mT, VN)) || (rc = dalloc_t(h, &h->mA, VN)) ||
+        (rc = dalloc_t(h, &h->mdl, VN)) ||
+        (rc = dalloc_t(h, &h->mrref, VN)) || (rc = dalloc_t(h, &h->mbq, VN)) ||
+        (rc = dalloc_t(h, &h->mhx, (size_t)o->nhist * VN)) || (rc = dalloc_t(h, &h->mring, (size_t)(o->nhist + 1) * VN)))
+      return rc;
+  } else if ((rc = dalloc(h, &h->res, VN * h->qsz)) || (rc = dalloc(h, &h->hist_t, (size_t)o->nhist * VN * h->qsz)) ||
+             (rc = dalloc(h, &h->hist_r, (size_t)o->nhist * VN * h->qsz))) {
+    return rc;
   }
   {  // per-workgroup partial sums: residual pass and thermodynamic reductions
     const size_t nv = 1 + h->hm + h->hm * (h->hm + 1) / 2;
@@ -1264,10 +1319,6 @@ int rism_build_potentials(rism_handle *h) {
   a.ulj = h->ulj; a.de = h->de; a.fr = h->fr; a.sr = h->sr;
   PROF(h, 6, launch_potentials(a, h->stream));
   PROF(h, 7, launch_fk_half(h->nx, h->ny, h->nz, h->natu, h->atoms, h->box, a.ubeta, h->phase, h->fkh, h->stream));
-  if (h->opts.precision == RISM_PREC_MIXED) {
-    HIP_OK(launch_convert<float>(h->fr, (float *)h->ps[1].fr, h->N, h->stream));
-    HIP_OK(launch_convert_c<float>(h->fkh, (float2 *)h->ps[1].fkh, (long)h->nx * (h->ny / 2) * h->nz, h->stream));
-  }
   HIP_OK(hipStreamSynchronize(h->stream));
   if (int rc = prof_collect(h)) return rc;
   h->potentials = true;
@@ -1330,6 +1381,12 @@ template <typename R> int time_iteration(rism_handle *h, int reps, double *ms) {
     c.npts = (double)h->V * h->N;
     c.status = -1;
     c.upd_iter = -1;
+    c.refresh_iter = -1;
+    if (h->opts.precision == RISM_PREC_MIXED) {  // the FP32 chain's step
+      c.ring = 1;
+      c.rcur = h->opts.nhist;
+      for (int k = 0; k < h->opts.nhist; ++k) c.rmap[k] = k;
+    }
     if ((rc = put_ctl(h))) break;
     if (hipEventRecord(e0, h->stream) != hipSuccess) { rc = RISM_ERR_HIP; break; }
     if ((rc = launch_iteration<R>(h, false, true))) break;
@@ -1363,10 +1420,15 @@ template <typename R> int autotune_impl(rism_handle *h, int reps) {
     }
   }
   HIP_OK(hipMemsetAsync(h->ts, 0, (size_t)h->V * h->N * sizeof(double), h->stream));
+  if (h->opts.precision == RISM_PREC_MIXED) {  // a finite state for the FP32 chain to time
+    const size_t VN = (size_t)h->V * h->N;
+    HIP_OK(hipMemsetAsync(h->mA, 0, VN * sizeof(double), h->stream));
+    HIP_OK(hipMemsetAsync(h->mdl, 0, VN * sizeof(float), h->stream));
+    HIP_OK(hipMemsetAsync(h->mrref, 0, VN * sizeof(float), h->stream));
+    h->mx_live = false;
+  }
   const double cf_save = h->cf;
-  const bool delta_save = h->delta;
   h->cf = 1.0;
-  h->delta = false;
   auto trial = [&](std::array<int, 5> c, double *ms) -> int {
     LaunchGeom g;
     if (choose_geometry(h, g, sizeof(R), c.data()) != RISM_OK) return 1;         // does not fit: skip
@@ -1421,7 +1483,6 @@ template <typename R> int autotune_impl(rism_handle *h, int reps) {
   };
   const int rc = search();
   h->cf = cf_save;
-  h->delta = delta_save;
   if (rc < 0) return rc;
   LaunchGeom g;
   if (int rc2 = choose_geometry(h, g, sizeof(R), best.data())) return rc2;
@@ -1842,11 +1903,13 @@ int rism_get_field(rism_handle *h, int which, double *out) {
   switch (which) {
     case RISM_FIELD_T: return fetch_t(h, out);
     case RISM_FIELD_RES:
-      if (h->qsz == sizeof(double)) {
+      if (h->opts.precision != RISM_PREC_MIXED) {
         HIP_OK(hcopy(h, out, h->res, VN * sizeof(double), hipMemcpyDeviceToHost));
-      } else {
+      } else {  // the last residual pass wrote ring slot rlast
+        if (int rc = get_ctl(h)) return rc;
         std::vector<float> r(VN);
-        HIP_OK(hcopy(h, r.data(), h->res, VN * sizeof(float), hipMemcpyDeviceToHost));
+        HIP_OK(hcopy(h, r.data(), h->mring + (size_t)h->ctl_host->rlast * VN, VN * sizeof(float),
+                     hipMemcpyDeviceToHost));
         for (size_t i = 0; i < VN; ++i) out[i] = r[i];
       }
       return RISM_OK;
@@ -2057,7 +2120,6 @@ int rism_calculate(rism_handle *h, const double *t_dev, double *res_dev) {
   std::vector<double> coef(h->V);
   for (int v = 0; v < h->V; ++v) coef[v] = -h->qv[v];
   HIP_OK(launch_axpy_fr<double>(h->ts, h->fr, coef.data(), h->V, h->N, 1, h->stream));
-  h->delta = false;
   int rc = stage_potentials(h);
   if (rc) return rc;
   Ctl &c = *h->ctl_host;
@@ -2068,14 +2130,23 @@ int rism_calculate(rism_handle *h, const double *t_dev, double *res_dev) {
   c.status = -1;
   c.run_iter = c.cur;
   c.upd_iter = -1;
+  c.refresh_iter = -1;
+  const bool mixed = h->opts.precision == RISM_PREC_MIXED;
+  if (mixed) {  // the FP64 refresh chain is the map itself (R rounded to FP32)
+    c.ring = 1;
+    c.rcur = h->opts.nhist;
+    for (int k = 0; k < h->opts.nhist; ++k) c.rmap[k] = k;
+    c.refresh_iter = c.cur;
+    if ((rc = mx_convert(h, 0))) return rc;
+  }
   if ((rc = put_ctl(h))) return rc;
-  rc = (h->opts.precision == RISM_PREC_MIXED) ? launch_iteration<float>(h, true, false)
-                                              : launch_iteration<double>(h, true, false);
+  rc = mixed ? launch_mx_iteration(h, 2, false) : launch_iteration<double>(h, true, false);
   if (rc) return rc;
-  if (h->qsz == sizeof(double))
+  if (!mixed)
     HIP_OK(hipMemcpyAsync(res_dev, h->res, VN * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
   else
-    HIP_OK(launch_widen((const float *)h->res, res_dev, (long)VN, h->stream));
+    HIP_OK(launch_widen(h->mring + (size_t)c.rcur * VN, res_dev, (long)VN, h->stream));
+  h->mx_live = mixed;
   HIP_OK(hipStreamSynchronize(h->stream));
   h->solved = true;
   h->evaluated = false;

@@ -12,9 +12,10 @@ The JSON line also carries:
                 bytes per launch / average launch time (HIP events around every
                 launch on the solver's stream, collected during one extra
                 profiled step), against the 8 TB/s HBM peak;
-  cpu_baseline  the FP64 host oracle (OpenMP) on a bounded sample of the same
-                workload: potentials plus a few iterations, scaled to the
-                FP64 step count of a full solve.
+  cpu_baseline  the FP64 host oracle (OpenMP, the host's cores) on the same
+                deck, the whole job measured (--cpu-sample: a bounded sample,
+                scaled); Delta-mu, PMV and g of both GPU paths are reported
+                against this oracle run.
 Solvent: 3-site water, chi_vv(k) from the library's own 1D XRISM-KH solve
 (16384 points, dr 0.01 A, 298.15 K; rism3d.configs.solvent_file), computed
 before any timing and cached.
@@ -113,6 +114,7 @@ def time_path(rism3d, w, solvent, prec, steps, warmup):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
     st, nit, rms, x = res
+    g = s.field(rism3d.FIELD_G).reshape(-1)
     stats = s.stats()
     s.profile(True)
     run_job(s)
@@ -121,29 +123,63 @@ def time_path(rism3d, w, solvent, prec, steps, warmup):
     geo = s.geometry()
     s.close()
     return {"s": dt, "status": st, "iterations": nit, "rms": rms, "xmu": x, "stats": stats,
-            "kernels": kt, "geometry": geo}
+            "kernels": kt, "geometry": geo, "g": g}
 
 
-def cpu_baseline(w, fp64_iterations, iters=8, sdiv=4):
-    """FP64 oracle (OpenMP, all host cores) on a bounded sample of the same job:
-    k-grid + spline in full, the grid potentials and the SE sum on 1/sdiv of
-    the points, `iters` full iterations; each part scaled to the whole job
-    (potentials and SE are linear in the points, iterations in their count)."""
+def cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(w, fp64_iterations, full=True, iters=8, sdiv=4):
+    """The FP64 host oracle (C/OpenMP restatement of the reference path, all
+    host cores) on the same deck.  full: the whole job measured (k-grid and
+    spline, potentials, the solve to RMS 1e-6, thermodynamics), its g written
+    for the GPU paths' error; otherwise a bounded sample (potentials and SE on
+    1/sdiv of the points, `iters` iterations) scaled to the job.  Per-step
+    lines of the oracle go to stderr every 50 steps."""
     sys.path.insert(0, REPO)
     from rism3d import configs
     from oracle import oracle as orc
     if not os.path.exists(orc.EXE):
-        return None
+        return None, None
     d = tempfile.mkdtemp()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    g = None
     try:
         shutil.copy(configs.solvent_file("water"), os.path.join(d, "h2o_vv.xvk"))
         configs.write_inp(os.path.join(d, "c1.inp"), w, "h2o_vv.xvk")
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        r = subprocess.run([orc.EXE, "-q", "-j", "-f", "-n", str(iters), "-x", str(sdiv), "-T", str(threads),
-                            "c1.inp"], cwd=d, capture_output=True, text=True, timeout=900)
-        rec = json.loads(r.stdout.strip().splitlines()[-1])
+        cmd = [orc.EXE, "-j", "-f", "-T", str(threads)]
+        cmd += ["-G", "g.bin"] if full else ["-q", "-n", str(iters), "-x", str(sdiv)]
+        p = subprocess.Popen(cmd + ["c1.inp"], cwd=d, stdout=subprocess.PIPE, text=True)
+        last = ""
+        for ln in p.stdout:
+            if ln.startswith(" Step = ") and int(ln.split()[2]) % 50 == 0:
+                print(f"cpu_baseline oracle:{ln.rstrip()}", file=sys.stderr, flush=True)
+            if ln.strip():
+                last = ln
+        if p.wait(timeout=900) != 0:
+            raise RuntimeError(f"oracle exited with {p.returncode}: {last[:200]}")
+        rec = json.loads(last)
+        if full:
+            g = np.fromfile(os.path.join(d, "g.bin"), dtype=np.float64)
     finally:
         shutil.rmtree(d, ignore_errors=True)
+    host = f"{cpu_model()}, nproc {os.cpu_count()}, {rec['threads']} OpenMP threads"
+    if full:
+        total = rec["t_setup"] + rec["t_iter"] + rec["t_thermo"]
+        return {"value": round(total, 3), "unit": "s", "cores": rec["threads"], "kind": "port",
+                "sample": (f"the whole job, measured: FP64 oracle (C/OpenMP restatement of the reference path) on "
+                           f"the same 256^3 deck, k-grid+spline {rec['t_setup'] - rec['t_pot']:.1f} s, potentials "
+                           f"{rec['t_pot']:.1f} s, {rec['steps']} iterations {rec['t_iter']:.1f} s, thermo with SE "
+                           f"{rec['t_thermo']:.1f} s; {host}"),
+                "steps": rec["steps"], "sfe_sc_J_mol": rec["sfe_sc"], "sfe_gf_J_mol": rec["sfe_gf"],
+                "pmv_L_mol": rec["pmv"], "host": host}, g
     k = rec["sample_div"]
     per_it = rec["t_iter"] / rec["steps"]
     setup = rec["t_setup"] - rec["t_pot"] + rec["t_pot"] * k
@@ -153,8 +189,8 @@ def cpu_baseline(w, fp64_iterations, iters=8, sdiv=4):
                        f"k-grid+spline {rec['t_setup'] - rec['t_pot']:.1f} s, potentials on 1/{k} of the "
                        f"points {rec['t_pot']:.1f} s (x{k}), {rec['steps']} iterations at {per_it:.2f} s "
                        f"(x{fp64_iterations}, the FP64 solve's count), thermo with SE on 1/{k} of the "
-                       f"points {rec['t_thermo']:.1f} s (x{k})"),
-            "measured_s": round(rec["t_setup"] + rec["t_iter"] + rec["t_thermo"], 2)}
+                       f"points {rec['t_thermo']:.1f} s (x{k}); {host}"),
+            "measured_s": round(rec["t_setup"] + rec["t_iter"] + rec["t_thermo"], 2), "host": host}, None
 
 
 def side_workload(which, rism3d, configs):
@@ -215,7 +251,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-sample", action="store_true",
+                    help="CPU baseline on a bounded sample of the job (scaled) instead of the whole job")
     ap.add_argument("--mixed-only", action="store_true", help="time only the mixed path")
     ap.add_argument("--workload", choices=("256", "512", "batch"), default="256",
                     help="256 (default): the headline 256^3 job; 512: the 512^3 / 5000-atom electrolyte "
@@ -261,10 +299,10 @@ def main():
             "bytes_per_launch": bytes_launch, "avg_launch_us": round(avg_s * 1e6, 2),
             "copy_GBps": round(copy_gbps, 1), "frac_of_copy": round(achieved / copy_gbps, 4)}
     per_kernel = per_kernel_table(mixed["kernels"], kb, N)
-    cpu = None
+    cpu, g_orc = None, None
     if not args.no_cpu and fp64 is not None:
         try:
-            cpu = cpu_baseline(w, fp64["iterations"])
+            cpu, g_orc = cpu_baseline(w, fp64["iterations"], full=not args.cpu_sample)
         except Exception as e:  # the baseline is reported, never required
             cpu = {"value": None, "error": str(e)[:200]}
     out = {
@@ -300,6 +338,16 @@ def main():
             "pmv_fp64_L_mol": fp64["xmu"]["PMV"],
             "kernels_fp64": per_kernel_table(fp64["kernels"], kernel_bytes(V, P, w.nhist, False), N),
         })
+    if cpu and cpu.get("sfe_sc_J_mol") is not None:
+        # both GPU paths against the oracle's own solve of the same deck on this host
+        out["dmu_mixed_minus_oracle_J_mol"] = round(mixed["xmu"]["SFE_SC"] - cpu["sfe_sc_J_mol"], 4)
+        if fp64 is not None:
+            out["dmu_fp64_minus_oracle_J_mol"] = round(fp64["xmu"]["SFE_SC"] - cpu["sfe_sc_J_mol"], 6)
+        out["pmv_mixed_minus_oracle_L_mol"] = mixed["xmu"]["PMV"] - cpu["pmv_L_mol"]
+        if g_orc is not None and g_orc.size == mixed["g"].size:
+            out["g_maxabs_mixed_vs_oracle"] = float(np.abs(mixed["g"] - g_orc).max())
+            if fp64 is not None:
+                out["g_maxabs_fp64_vs_oracle"] = float(np.abs(fp64["g"] - g_orc).max())
     print(json.dumps(out))
 
 

@@ -7,6 +7,7 @@
  *   -n K  cap the iteration count (bounded CPU-baseline sample)
  *   -j    print a JSON timing record on stdout after the run
  *   -q    quiet (no per-step lines)
+ *   -G f  write g of the final state to f as raw float64 [V][N]
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -27,10 +28,10 @@ static double now(void) {
 
 int main(int argc, char **argv) {
   int cu = 0, centering = 1, zero = 0, nhist = 2, threads = 0, cap = 0, json = 0, quiet = 0;
-  const char *input = NULL, *structure = NULL;
+  const char *input = NULL, *structure = NULL, *gout = NULL;
   int ch;
   long sdiv = 1;
-  while ((ch = getopt(argc, argv, "c:d:i:s:fzH:T:n:x:jq")) != -1) {
+  while ((ch = getopt(argc, argv, "c:d:i:s:fzH:T:n:x:jqG:")) != -1) {
     switch (ch) {
       case 'c': cu = atoi(optarg); break;
       case 'd': break;  /* device index: meaningless for the host oracle */
@@ -44,6 +45,7 @@ int main(int argc, char **argv) {
       case 'x': sdiv = atol(optarg); break;  /* timing sample: 1/x of the points for potentials and SE */
       case 'j': json = 1; break;
       case 'q': quiet = 1; break;
+      case 'G': gout = optarg; break;
       default: return 1;
     }
   }
@@ -52,6 +54,8 @@ int main(int argc, char **argv) {
     input = argv[optind];
     structure = NULL;
   }
+  /* per-step lines reach a pipe as they happen */
+  setvbuf(stdout, NULL, _IOLBF, 0);
   if (threads > 0) orc_set_threads(threads);
   int nthr = 1;
 #ifdef _OPENMP
@@ -92,11 +96,22 @@ int main(int argc, char **argv) {
   char out[4200];
   snprintf(out, sizeof out, "%s%s", fname, zero ? ".xmu0" : ".xmu");
   if (strchr(s->outlist, 'm') || strchr(s->outlist, 'M')) orc_write_xmu(s, &x, out);
+  if (gout) {
+    const size_t VN = (size_t)s->natv * s->ng;
+    double *g = malloc(VN * sizeof(double));
+    FILE *f = fopen(gout, "wb");
+    if (!g || !f || (orc_copy_field(s, 2, g), fwrite(g, sizeof(double), VN, f)) != VN) {
+      printf("cannot write %s\n", gout);
+      st = -1;
+    }
+    if (f) fclose(f);
+    free(g);
+  }
   if (json)
     printf("{\"status\": %d, \"steps\": %d, \"rms\": %.6e, \"t_setup\": %.6f, \"t_pot\": %.6f, "
            "\"t_iter\": %.6f, \"t_thermo\": %.6f, \"threads\": %d, \"ng\": %ld, \"sample_div\": %ld, "
-           "\"sfe_sc\": %.10f, \"sfe_gf\": %.10f, \"pmv\": %.10f}\n",
-           st, steps, rms, t1 - t0, t1 - t0p, t2 - t1, t3 - t2, nthr, s->ng, sdiv, x.sfe_sc, x.sfe_gf, x.pmv);
+           "\"sfe_sc\": %.10f, \"sfe_gf\": %.10f, \"pmv\": %.10f, \"se\": %.10f}\n",
+           st, steps, rms, t1 - t0, t1 - t0p, t2 - t1, t3 - t2, nthr, s->ng, sdiv, x.sfe_sc, x.sfe_gf, x.pmv, x.se);
   orc_free(s);
   return st < 0 ? 1 : 0;
 }

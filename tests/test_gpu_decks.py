@@ -6,7 +6,10 @@ test3.inp (754 atoms, 64 A / 128^3) and test4.inp (protein G, 846 atoms,
 FP64 and in mixed precision (-p 1) and are compared with
   - the FP64 oracle on the same deck (tests/golden/reference_decks_oracle.json,
     written by tests/golden/make_deck_oracle.py): FP64 follows its Anderson
-    trajectory, so the step count and the free energies agree closely;
+    trajectory (the same step count), and every .xmu quantity agrees within
+    twice the tolerance ball (the FP64 path's own distance between its
+    RMS-1e-6 state and the fixed point, measured in the test) or a floor of
+    1 J/mol (1e-5 L/mol for PMV);
   - the reference's own test3.xmu / test4.xmu.  Those came from an older
     build (they lack the SE lines the current output_xmu.cu:75-97 writes).
     SFE_SC is the small difference of two per-site sums of ~7e9 J/mol
@@ -92,30 +95,34 @@ def test_reference_deck_fp64_and_mixed(gpu, deckdir, deck):
               f"SFE_SC {res[prec][0]['SFE_SC']:.5f}, PMV {res[prec][0]['PMV']:.5f}", flush=True)
     x64, st64, _ = res[0]
     xmx, _, _ = res[1]
-    # FP64: the oracle's trajectory
+    # the fixed point: the FP64 path continued to RMS 1e-9
+    _with_tol(deck, 1e-9, "fp.inp")
+    _run_cli(["fp.inp"])
+    xf = _xmu("fp.xmu")
+    keys = ("SFE_SC", "SFE_GF", "SE_LJ", "SE_ES", "PMV", "Correction_Term")
+    ball = {k: abs(x64[k] - xf[k]) for k in keys}
     ox = orc_rec["xmu"]
+    print(f"{deck} FP64 - oracle: " + ", ".join(f"{k} {x64[k] - ox[k]:.4g}" for k in keys), flush=True)
+    print(f"{deck} ball (RMS 1e-6 vs fixed point): " + ", ".join(f"{k} {ball[k]:.4g}" for k in keys), flush=True)
+    print(f"{deck} mixed - fixed point: " + ", ".join(f"{k} {xmx[k] - xf[k]:.4g}" for k in keys), flush=True)
+    # FP64: the oracle's trajectory (the step count), and its solution within
+    # the tolerance ball: rounding differences grow over some 300 Anderson
+    # steps, so the two RMS-1e-6 states differ by a fraction of the ball
     assert abs(st64 - orc_rec["steps"]) <= 1, (st64, orc_rec["steps"])
-    tol_sfe = 1.0 if st64 == orc_rec["steps"] else 2e-5 * per_site
-    assert x64["SFE_SC"] == pytest.approx(ox["SFE_SC"], abs=tol_sfe)
-    assert x64["SFE_GF"] == pytest.approx(ox["SFE_GF"], abs=tol_sfe)
-    assert x64["PMV"] == pytest.approx(ox["PMV"], abs=1e-5)
-    for k in ("SE_LJ", "SE_ES", "Pressure", "Correction_Term"):
-        assert x64[k] == pytest.approx(ox[k], rel=1e-6, abs=1e-3), k
+    floor = {"SFE_SC": 1.0, "SFE_GF": 1.0, "SE_LJ": 1.0, "SE_ES": 1.0, "PMV": 1e-5, "Correction_Term": 1.0}
+    for k in keys:
+        assert abs(x64[k] - ox[k]) <= max(floor[k], 2 * ball[k]), (k, x64[k], ox[k], ball[k])
+    assert x64["Pressure"] == pytest.approx(ox["Pressure"], rel=1e-12)
     # the reference's older-build .xmu
     for x in (x64, xmx):
         assert x["SFE_SC"] == pytest.approx(ref["SFE_SC"], abs=2e-5 * per_site)
         assert x["SFE_GF"] == pytest.approx(ref["SFE_GF"], abs=2e-5 * per_site)
         assert x["PMV"] == pytest.approx(ref["PMV"], abs=2e-4)
     # mixed: within the tolerance ball of the FP64 fixed point
-    _with_tol(deck, 1e-9, "fp.inp")
-    _run_cli(["fp.inp"])
-    xf = _xmu("fp.xmu")
-    ball = abs(x64["SFE_SC"] - xf["SFE_SC"])
-    bar = max(0.01 * KCAL, 2 * ball)
-    print(f"{deck}: fixed point SFE_SC {xf['SFE_SC']:.5f}, ball {ball:.3f} J/mol, "
-          f"mixed - fixed point {xmx['SFE_SC'] - xf['SFE_SC']:.3f} J/mol (bar {bar:.3f})", flush=True)
+    bar = max(0.01 * KCAL, 2 * ball["SFE_SC"])
     assert abs(xmx["SFE_SC"] - xf["SFE_SC"]) <= bar
-    assert abs(xmx["PMV"] - xf["PMV"]) <= max(1e-4, 2 * abs(x64["PMV"] - xf["PMV"]))
+    assert abs(xmx["SFE_GF"] - xf["SFE_GF"]) <= max(0.01 * KCAL, 2 * ball["SFE_GF"])
+    assert abs(xmx["PMV"] - xf["PMV"]) <= max(1e-4, 2 * ball["PMV"])
 
 
 def test_gra_file_matches_oracle_grad(gpu, deckdir):
