@@ -54,12 +54,11 @@ __device__ __forceinline__ double wave_sum(double v) {
 #ifndef RISM_CLOS_CH
 #define RISM_CLOS_CH 2
 #endif
-template <typename R, typename S, int EPT, int VC, int HM, bool UPD, bool CREF>
+template <typename R, typename S, int EPT, int VC, int HM, bool UPD>
 __device__ __forceinline__ void closure_points(const IterArgs<R, S> &a, typename C2<R>::T *lds, bool do_run,
                                                const double *th, const long *off, double mix, int w, int nf) {
   using T2 = typename C2<R>::T;
   using Q = typename HistT<R>::T;
-  constexpr bool kIncr = sizeof(Q) < sizeof(S);
   constexpr int CH = EPT < RISM_CLOS_CH ? EPT : RISM_CLOS_CH;
   const int nx = a.nx, L = a.Lc, ls = line_stride(nx), V = a.V;
   const long N = a.N, VN = (long)V * N;
@@ -71,12 +70,10 @@ __device__ __forceinline__ void closure_points(const IterArgs<R, S> &a, typename
   Q *__restrict__ ht = a.hist_t;
   Q *__restrict__ hr = a.hist_r;
   const R *__restrict__ uu = a.u;
-  const float *__restrict__ crf = a.cref;
 #pragma unroll
   for (int k0 = 0; k0 < EPT; k0 += CH) {
     double t[CH][VC], u[CH][VC];
     Q r[CH][VC], hti[CH][VC][UPD ? HM : 1], hri[CH][VC][UPD ? HM : 1];
-    float cr[CH][VC];
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
       const unsigned e = (unsigned)min((int)threadIdx.x + (k0 + c) * kThreads, nel - 1);
@@ -94,7 +91,6 @@ __device__ __forceinline__ void closure_points(const IterArgs<R, S> &a, typename
               hri[c][v][s] = lane_at(hr + off[s] + vb, e);
             }
           }
-          if (CREF) cr[c][v] = lane_at(crf + vb, e);
         }
       }
     }
@@ -108,23 +104,18 @@ __device__ __forceinline__ void closure_points(const IterArgs<R, S> &a, typename
           const long vb = (long)v * N + base;
           if (UPD) {  // newdt / newdt0 (anderson_cuda.cu:1-23)
             const double rv = (double)r[c][v];
-            double tb = t[c][v], rb = rv, back = 0.0;
+            double tb = t[c][v], rb = rv;
 #pragma unroll
             for (int s = 0; s < (UPD ? HM : 0); ++s) {
               // slots beyond the filled ones: valid loads, values discarded
               const double hts = s < nf ? (double)hti[c][v][UPD ? s : 0] : 0.0;
               const double hrs = s < nf ? (double)hri[c][v][UPD ? s : 0] : rv;
-              if (kIncr) {  // age s+1: t_(s+1) - t = -(sum of the increments of ages 1..s+1)
-                back -= hts;
-                tb += th[s] * back;
-              } else {
-                tb += th[s] * (hts - t[c][v]);
-              }
+              tb += th[s] * (hts - t[c][v]);
               rb += th[s] * (hrs - rv);
             }
             const double tn = tb + mix * rb;
             if (valid) {
-              lane_at(ht + (long)w * VN + vb, e) = kIncr ? (Q)(tn - t[c][v]) : (Q)t[c][v];
+              lane_at(ht + (long)w * VN + vb, e) = (Q)t[c][v];
               lane_at(hr + (long)w * VN + vb, e) = r[c][v];
               lane_at(ts + vb, e) = (S)tn;
             }
@@ -138,28 +129,7 @@ __device__ __forceinline__ void closure_points(const IterArgs<R, S> &a, typename
 #pragma unroll
         for (int v = 0; v < VC; ++v) {
           cs[v] = (R)0;
-          if (v < V) {
-            const long vb = (long)v * N + base;
-            if (a.src) {
-              cs[v] = (R)cr[c][v];
-            } else {
-              // closure in FP64 in both precisions: near a charged solute its
-              // argument ts - u is the small difference of large terms
-              const double g = closure_g<double>(t[c][v] - u[c][v], a.closure);
-              // FP32 passes: g goes to the residual pass, which then reads 4 bytes
-              // per site instead of t_s and u (12) to evaluate the closure again
-              if constexpr (sizeof(R) == sizeof(float)) lane_at(a.gq + vb, e) = (float)g;
-              double cv = g - 1.0 - t[c][v];
-              if (a.refresh) {  // new reference; the rounding remainder goes to the FP32 passes
-                const float cf = (float)cv;
-                lane_at(a.cref_out + vb, e) = cf;
-                cv -= (double)cf;
-              } else if (a.delta) {
-                cv -= (double)cr[c][v];
-              }
-              cs[v] = (R)cv;
-            }
-          }
+          if (v < V) cs[v] = (R)(closure_g<double>(t[c][v] - u[c][v], a.closure) - 1.0 - t[c][v]);
         }
         cs[VC] = (R)0;
         const T2 pw = pre[x];
@@ -173,11 +143,10 @@ __device__ __forceinline__ void closure_points(const IterArgs<R, S> &a, typename
   }
 }
 
+// the FP64 path's first pass (the mixed path's is k_mx_closure_fx)
 template <typename R, typename S, int EPT, bool UPDATE, int VC, int HM>
 __global__ __launch_bounds__(kThreads) void k_closure_fx(IterArgs<R, S> a) {
   using T2 = typename C2<R>::T;
-  using Q = typename HistT<R>::T;
-  constexpr bool kIncr = sizeof(Q) < sizeof(S);
   const Ctl *ctl = a.ctl;
   const int iter = ctl->cur;
   const bool do_upd = UPDATE && (ctl->upd_iter == iter);
@@ -189,8 +158,8 @@ __global__ __launch_bounds__(kThreads) void k_closure_fx(IterArgs<R, S> a) {
   const long N = a.N, VN = (long)a.V * N;
   const long base = (long)blockIdx.x * L * nx;
   const int nel = L * nx;
-  // update coefficients by age (incremental form) or by slot (absolute form);
-  // slots beyond the filled ones get a zero coefficient and a valid address
+  // update coefficients by slot; slots beyond the filled ones get a zero
+  // coefficient and a valid address
   int w = 0, nf = 0;
   double mix = 0.0;
   double th[HM];
@@ -207,27 +176,15 @@ __global__ __launch_bounds__(kThreads) void k_closure_fx(IterArgs<R, S> a) {
     const int nh = ctl->nhist;
 #pragma unroll
     for (int k = 0; k < HM; ++k) {
-      const int slot = kIncr ? (((w - 1 - k) % nh) + nh) % nh : (k < nh ? k : 0);
+      const int slot = k < nh ? k : 0;
       th[k] = (k < nf) ? ctl->upd_theta[slot] : 0.0;
       off[k] = (long)slot * VN;
     }
   }
-  // c_ref is an input of the delta-mode FP32 pass and of the FP64 source pass
-  // (the FP64 update pass never reads it: delta mode is FP32-only)
-  constexpr bool kCrefUpd = sizeof(R) < sizeof(double);
-  const bool cref_in = do_run && a.cref && (a.src || (a.delta && !a.refresh));
-  if (UPDATE && do_upd) {
-    if (kCrefUpd && cref_in)
-      closure_points<R, S, EPT, VC, HM, true, kCrefUpd>(a, lds, do_run, th, off, mix, w, nf);
-    else
-      closure_points<R, S, EPT, VC, HM, true, false>(a, lds, do_run, th, off, mix, w, nf);
-  } else {
-    constexpr bool kCrefRun = kCrefUpd || !UPDATE;  // FP64: only the source pass reads c_ref
-    if (kCrefRun && cref_in)
-      closure_points<R, S, EPT, VC, HM, false, kCrefRun>(a, lds, do_run, th, off, mix, w, nf);
-    else
-      closure_points<R, S, EPT, VC, HM, false, false>(a, lds, do_run, th, off, mix, w, nf);
-  }
+  if (UPDATE && do_upd)
+    closure_points<R, S, EPT, VC, HM, true>(a, lds, do_run, th, off, mix, w, nf);
+  else
+    closure_points<R, S, EPT, VC, HM, false>(a, lds, do_run, th, off, mix, w, nf);
   if (!do_run) return;
   T2 *ltw = lds + P * L * ls;
   stage_tables(ltw, a.px);
@@ -687,7 +644,6 @@ __global__ __launch_bounds__(kThreads) void k_fx_resid(IterArgs<R, S> a) {
   const R *__restrict__ uu = a.u;
   Q *__restrict__ res = a.res;
   const Q *__restrict__ hr = a.hist_r;
-  const double2 *__restrict__ href = a.href;
   double acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) acc[k] = 0.0;
@@ -721,38 +677,21 @@ __global__ __launch_bounds__(kThreads) void k_fx_resid(IterArgs<R, S> a) {
 #ifndef RISM_RESID_CH
 #define RISM_RESID_CH 1
 #endif
-    // Operands of a point (h_ref, t_s, u, history residuals) are loaded after
+    // Operands of a point (t_s, u, history residuals) are loaded after
     // the line transform in batches of CH points.  Loading every point's
     // operands before the transform instead (to overlap their latency with
     // the LDS work) needs so many registers that occupancy halves; measured
     // slower at 256^3.
     constexpr int CH = KPT < RISM_RESID_CH ? KPT : RISM_RESID_CH;
     constexpr bool PREH = CH * VC * HM <= 16;  // history residuals join the batch
-    // FP32 passes: g comes from the closure pass of the same iteration, which
-    // ran under the same gate and wrote it for every point
-    constexpr bool kGq = sizeof(R) == sizeof(float);
-    auto load_ops = [&](unsigned e, double *hbo, double *tvo, double *uvo, Q *hqo) {
-#pragma unroll
-      for (int p = 0; p < (VC + 1) / 2; ++p) {
-        hbo[2 * p] = hbo[2 * p + 1] = 0.0;
-        if (p < P && a.delta && !a.href_only) {
-          const double2 hr2 = lane_at(href + p * N + base, e);
-          hbo[2 * p] = hr2.x;
-          hbo[2 * p + 1] = hr2.y;
-        }
-      }
+    auto load_ops = [&](unsigned e, double *tvo, double *uvo, Q *hqo) {
       if (!a.href_only) {
 #pragma unroll
         for (int v = 0; v < VC; ++v)
           if (v < V) {
             const long vb = (long)v * N + base;
-            if constexpr (kGq) {  // g of this iteration's closure pass (tvo), uvo unused
-              tvo[v] = (double)lane_at(a.gq + vb, e);
-              uvo[v] = 0.0;
-            } else {
-              tvo[v] = (double)lane_at(ts + vb, e);
-              uvo[v] = (double)lane_at(uu + vb, e);
-            }
+            tvo[v] = (double)lane_at(ts + vb, e);
+            uvo[v] = (double)lane_at(uu + vb, e);
 #pragma unroll
             for (int s = 0; s < (PREH ? HM : 0); ++s) hqo[v * (PREH ? HM : 1) + s] = lane_at(hr + (s < nf ? s : 0) * VN + vb, e);
           }
@@ -797,14 +736,13 @@ __global__ __launch_bounds__(kThreads) void k_fx_resid(IterArgs<R, S> a) {
         const unsigned e = (unsigned)min(tidr + (k0 + c) * kThreads, nel - 1);
         const int i = e / nx, x = e - i * nx;
         const T2 pw = pre[x];
-        double hbr[VC + 1];
-        load_ops(e, hbr, tv[c], uv[c], &hq[c][0][0]);
+        load_ops(e, tv[c], uv[c], &hq[c][0][0]);
 #pragma unroll
         for (int p = 0; p < (VC + 1) / 2; ++p) {
           if (p < P) {
             const T2 z = cscale(cmulc(lds[(p * L + i) * ls + lpad(x)], pw), ivol);
-            hv[c][2 * p] = (double)z.x + hbr[2 * p];
-            hv[c][2 * p + 1] = (double)z.y + hbr[2 * p + 1];
+            hv[c][2 * p] = (double)z.x;
+            hv[c][2 * p + 1] = (double)z.y;
           }
         }
       }
@@ -823,7 +761,7 @@ __global__ __launch_bounds__(kThreads) void k_fx_resid(IterArgs<R, S> a) {
           for (int v = 0; v < VC; ++v) {
             if (v < V) {
               const long vb = (long)v * N + base;
-              const double g = kGq ? tv[c][v] : closure_g<double>(tv[c][v] - uv[c][v], a.closure);
+              const double g = closure_g<double>(tv[c][v] - uv[c][v], a.closure);
               const double rd = hv[c][v] + 1.0 - g;  // tr: R = h + 1 - g
               lane_at(res + vb, e) = (Q)rd;
               const double wgt = a.rho_inv[v];
@@ -954,14 +892,7 @@ template <typename R, typename S>
 hipError_t launch_closure_fx(const IterArgs<R, S> &a, const LaunchGeom &g, bool update, int hm,
                              hipStream_t st) {
   const dim3 grid(g.cgroups), block(kThreads);
-  if (!update) {  // closure + transform only (the FP64 source pass): no history class
-    if constexpr (sizeof(R) == sizeof(double)) {
-      RISM_VC_SWITCH(a.V, RISM_EPTX_SWITCH(g.ept_c, hipLaunchKernelGGL((k_closure_fx<R, S, E, false, VC, 2>), grid, block, g.lds_c, st, a)))
-      return hipGetLastError();
-    } else {
-      return hipErrorInvalidValue;
-    }
-  }
+  if (!update) return hipErrorInvalidValue;  // every launch carries the history class
 #define RISM_CLOS(HMV) \
   RISM_VC_SWITCH(a.V, RISM_EPTX_SWITCH(g.ept_c, hipLaunchKernelGGL((k_closure_fx<R, S, E, true, VC, HMV>), grid, block, g.lds_c, st, a)))
   switch (hm) {

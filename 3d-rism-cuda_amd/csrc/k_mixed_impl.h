@@ -29,6 +29,16 @@
 #pragma once
 #include "k_iterate_impl.h"
 
+#ifndef RISM_MXR_PLAN4
+#define RISM_MXR_PLAN4 0  // FP32 residual pass on the radix-4 x plan
+#endif
+#ifndef RISM_MXR_SB
+#define RISM_MXR_SB 1     // residual pass: each batch's loads stay in the batch
+#endif
+#ifndef RISM_MXR_ZPF
+#define RISM_MXR_ZPF 1    // residual pass loads the next group's spectrum during the point loop
+#endif
+
 namespace rism {
 
 // ---------------------------------------------------------------------
@@ -251,12 +261,13 @@ __global__ __launch_bounds__(kThreads) void k_mx_resid(IterArgs<R, double> a) {
       zb[k] = Z[p * N + gb + e];
     }
   };
-  load_z(blockIdx.x, threadIdx.x);
+  if (RISM_MXR_ZPF) load_z(blockIdx.x, threadIdx.x);
   for (int grp = blockIdx.x; grp < groups; grp += gridDim.x) {
     const long base = (long)grp * nel;
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
     __syncthreads();
+    if (!RISM_MXR_ZPF) load_z(grp, tid);
     constexpr int PMIN = VC <= 2 ? 1 : (VC <= 4 ? 2 : 3);
     constexpr int KPT = (EPT + PMIN - 1) / PMIN;
     constexpr int CH = KPT < RISM_RESID_CH ? KPT : RISM_RESID_CH;
@@ -273,13 +284,13 @@ __global__ __launch_bounds__(kThreads) void k_mx_resid(IterArgs<R, double> a) {
     __syncthreads();
     int tidf = tid;
     asm volatile("" : "+v"(tidf));
-    if constexpr (F64)
+    if constexpr (F64 || RISM_MXR_PLAN4)
       fft_lines<EPT, true, 5>(lds, P * L, ls, a.pxr, ltw, tidf);
     else
       fft_lines<EPT, true>(lds, P * L, ls, a.px, ltw, tidf);
     int tidr = tid;
     asm volatile("" : "+v"(tidr));
-    if (grp + (int)gridDim.x < groups) load_z(grp + (int)gridDim.x, tidr);
+    if (RISM_MXR_ZPF && grp + (int)gridDim.x < groups) load_z(grp + (int)gridDim.x, tidr);
 #pragma unroll
     for (int k0 = 0; k0 < KPT; k0 += CH) {
       double hv[CH][VC + 1];
@@ -344,7 +355,7 @@ __global__ __launch_bounds__(kThreads) void k_mx_resid(IterArgs<R, double> a) {
           }
         }
       }
-      __builtin_amdgcn_sched_barrier(0);
+      if (RISM_MXR_SB) __builtin_amdgcn_sched_barrier(0);
     }
   }
   __shared__ double red[kThreads / 64][NV];

@@ -299,10 +299,7 @@ hipError_t launch_grad(int nx, int ny, int nz, const double *dr, int V, int natu
 }
 
 template hipError_t launch_gfield<double, double>(const IterArgs<double, double> &, double *, hipStream_t);
-template hipError_t launch_gfield<float, double>(const IterArgs<float, double> &, double *, hipStream_t);
 template hipError_t launch_exchem<double, double>(const IterArgs<double, double> &, const double *,
                                                   double *, int, hipStream_t);
-template hipError_t launch_exchem<float, double>(const IterArgs<float, double> &, const double *,
-                                                 double *, int, hipStream_t);
 
 }  // namespace rism

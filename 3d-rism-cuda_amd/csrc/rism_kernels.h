@@ -13,7 +13,8 @@
 //   Z    C[P][N]      work spectrum: two real sites per complex value
 //   chi  R4[V*V][ntab-1]  natural-spline coefficients of chi_vv'(k)
 // S is the state precision (double), R the field/FFT precision (double for
-// RISM_PREC_FP64, float for RISM_PREC_MIXED).
+// RISM_PREC_FP64, float for the mixed path's FP32 chain; its state is the
+// FP64 reference plus FP32 change of IterArgs::A, dl).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -91,9 +92,8 @@ __device__ __forceinline__ bool gate_open(const Ctl *c, int gate) {
   return c->run_iter == c->cur && chain_active(c, gate);
 }
 
-// Residual / history storage type: FP64 in the FP64 path; FP32 in the mixed
-// path, where the t history is kept as FP32 step increments t_{k+1} - t_k
-// (rounding then scales with the step, not with |t|) and the state ts stays FP64.
+// Residual / history storage type of the FP64 path's passes (the mixed
+// path's state is described with IterArgs::A below).
 template <typename R> struct HistT;
 template <> struct HistT<double> { using T = double; };
 template <> struct HistT<float> { using T = float; };
@@ -130,19 +130,7 @@ struct IterArgs {
   int Ty, Tz;       // tile widths (consecutive x) of the y / z passes
   int xblocks;      // workgroups of the x passes (grid-stride over line groups)
   int store_h;      // residual pass also writes real-space h into Z
-  // ---- refinement of the FP32 convolution (mixed precision) ----
-  // The OZ step is linear in c_s.  With a reference c_ref (FP32 values)
-  // whose real-space response h_ref was computed once in FP64, the FP32
-  // passes transform only c_s - c_ref and add h_ref back, so their rounding
-  // scales with |c_s - c_ref|, which shrinks as the solve converges.
-  const float *cref;    // [V][N] reference c_s
-  float *cref_out;      // refresh: first pass stores c_s here
-  float *gq;            // FP32 passes: g of the closure pass [V][N], read by the residual pass
-  const double2 *href;  // [P][N] FP64 real-space response to cref (incl. long range)
-  int delta;            // FP32 passes: subtract cref / add href, no fk term
-  int refresh;          // FP32 first pass: c_s -> cref_out, emit a zero spectrum
-  int src;              // FP64 first pass: transform cref instead of the closure
-  int href_only;        // FP64 residual pass: store h only
+  int href_only;        // residual pass: store h only (the final evaluation)
   int use_fk;           // z pass subtracts q_v cf fk(k)
   double *rms_hist;
   // ---- mixed precision on differences to an FP64 reference (gate 1, 2) ----

@@ -420,7 +420,7 @@ template <typename R> int build_plans(rism_handle *h) {
 
 // per-workgroup partial sums of the residual pass are sized for this many
 // workgroups (the autotuner's largest candidate)
-constexpr int kMaxXBlocks = 2048;
+constexpr int kMaxXBlocks = 8192;
 
 long env_long(const char *name, long dflt) {
   const char *s = std::getenv(name);

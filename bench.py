@@ -44,29 +44,35 @@ TRAFFIC = os.path.join(REPO, "profiles", "iteration_kernels_256cubed_mixed.json"
 def kernel_bytes(V, P, H, mixed):
     """Algorithmic HBM bytes per grid point of each iteration kernel: every
     operand read once and every result written once at its stored width, for
-    a steady-state Anderson step with H filled history slots.
+    a steady-state Anderson step with H filled history slots (V sites, P
+    complex packs of two sites).
 
-    Stored widths: the state t_s is FP64 in both paths; u, the spectrum and,
-    in the mixed path, R and the history are FP32 (FP64 in the FP64 path).
-    The mixed path transforms the difference to an FP64 reference, so its
-    update step also reads c_ref (FP32) and its residual step reads h_ref
-    (complex FP64 per site pair) instead of the half-grid long-range term.
-    Its update step also writes the closure's g (FP32), which the residual
-    step reads in place of t_s and u.
+    FP64 path: the state t_s, the closure potential u, the residual R and
+    the H-slot histories of t_s and R in FP64; the spectrum in complex FP64.
+    Mixed path, FP32 chain (every step but the few FP64 refreshes): the
+    closure pass reads the reference A (FP64), the change dl (FP32, also
+    written), the current R and H history residuals from the ring, H
+    increments and R_ref (FP32), writes one increment and B (FP32) and the
+    spectrum (complex FP32); the residual pass reads the spectrum, B and the
+    H history residuals and writes R into the ring; no long-range term.
     """
-    s_t = 8
-    s_f = 4 if mixed else 8
-    s_h = 4 if mixed else 8
-    c = 2 * s_f
+    if mixed:
+        return {
+            "closure_fx": V * (8 + 4 + 4 + 4 + 8 * H + 4 + 4 + 4) + 8 * P,
+            "fft_y_fwd": 16 * P,
+            "fft_z_oz": 16 * P,
+            "fft_y_inv": 16 * P,
+            "resid": 8 * P + V * (4 + 4 * H + 4),
+        }
     return {
-        # t_s, u, R, H x (t_i, R_i) in; new slot (t, R) and t_s out; [c_ref in, g out]; spectrum out
-        "closure_fx": V * (s_t + s_f + s_h + 2 * H * s_h + 2 * s_h + s_t + (8 if mixed else 0)) + P * c,
-        "fft_y_fwd": 2 * P * c,
-        # spectrum in/out + the half-grid long-range term (FP64 path only)
-        "fft_z_oz": 2 * P * c + (0 if mixed else c // 2),
-        "fft_y_inv": 2 * P * c,
-        # spectrum, [h_ref], t_s and u (mixed: g), H history R_i in; R out
-        "resid": P * c + (16 * P if mixed else 0) + V * ((4 if mixed else s_t + s_f) + H * s_h + s_h),
+        # t_s, u, R, H x (t_i, R_i) in; new slot (t, R) and t_s out; spectrum out
+        "closure_fx": V * (8 + 8 + 8 + 16 * H + 16 + 8) + 16 * P,
+        "fft_y_fwd": 32 * P,
+        # spectrum in/out + the half-grid long-range term
+        "fft_z_oz": 32 * P + 8,
+        "fft_y_inv": 32 * P,
+        # spectrum, t_s and u, H history R_i in; R out
+        "resid": 16 * P + V * (8 + 8 + 8 * H + 8),
     }
 
 
@@ -316,7 +322,7 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 fields / f64 state",
+        "dtype": "f32 fields and changes / f64 reference",
         "data": "synthetic",
         "config": {"workload": w.name, "grid": list(w.n), "box_A": list(w.box), "atoms": len(w.solute.q),
                    "solvent": "3-site water, chi from the library's 1D XRISM-KH solve (16384 pts, dr 0.01 A)", "closure": "KH",
