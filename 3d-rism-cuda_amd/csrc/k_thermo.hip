@@ -108,22 +108,23 @@ __global__ __launch_bounds__(kThreads) void k_se_sum(long N, int V, const double
 }
 
 // cal_grad: per atom, sum_v rho_v sum_r g dU/dR terms; a workgroup owns a
-// chunk of grid points and walks all atoms, writing per-(chunk, atom)
-// partials that a second kernel sums in chunk order.
+// chunk of grid points and walks the atoms u0 .. u0+nb-1, writing
+// per-(chunk, atom) partials [nblk][nb][6] that a second kernel sums in chunk
+// order (each atom's sums do not depend on the batch it is in).
 constexpr int kGradPPT = 16;
 
 __global__ __launch_bounds__(kThreads) void k_grad_partial(int nx, int ny, int nz, double drx,
                                                            double dry, double drz, int V, int natu,
                                                            const double4 *atoms, const double2 *lj,
                                                            const double *sigh, const double *qv,
-                                                           const double *rho, const double *g,
-                                                           double *partials) {
+                                                           const double *rho, const double *g, int u0,
+                                                           int nb, double *partials) {
   const long N = (long)nx * ny * nz;
   const double cc = 4.359744722207185e-18 * 0.5291771500001 * 6.02214076e23;
   const long cbase = (long)blockIdx.x * kThreads * kGradPPT;
   __shared__ double red[kThreads / 64][6];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int u = 0; u < natu; ++u) {
+  for (int u = u0; u < u0 + nb; ++u) {
     const double4 at = atoms[u];
     double a6[6] = {0, 0, 0, 0, 0, 0};
     for (int k = 0; k < kGradPPT; ++k) {
@@ -156,18 +157,18 @@ __global__ __launch_bounds__(kThreads) void k_grad_partial(int nx, int ny, int n
     if (threadIdx.x < 6) {
       double s = 0.0;
       for (int w = 0; w < kThreads / 64; ++w) s += red[w][threadIdx.x];
-      partials[((long)blockIdx.x * natu + u) * 6 + threadIdx.x] = s;
+      partials[((long)blockIdx.x * nb + (u - u0)) * 6 + threadIdx.x] = s;
     }
     __syncthreads();
   }
 }
 
-__global__ __launch_bounds__(kThreads) void k_grad_sum(const double *partials, int nblk, int natu,
+__global__ __launch_bounds__(kThreads) void k_grad_sum(const double *partials, int nblk, int nb,
                                                        double scale, double *out) {
   const long id = (long)blockIdx.x * kThreads + threadIdx.x;
-  if (id >= (long)natu * 6) return;
+  if (id >= (long)nb * 6) return;
   double s = 0.0;
-  for (int b = 0; b < nblk; ++b) s += partials[(long)b * natu * 6 + id];
+  for (int b = 0; b < nblk; ++b) s += partials[(long)b * nb * 6 + id];
   out[id] = s * scale;
 }
 
@@ -288,13 +289,13 @@ hipError_t launch_se_sum(long N, int V, const double *rho, const double *qv, con
 
 hipError_t launch_grad(int nx, int ny, int nz, const double *dr, int V, int natu,
                        const double4 *atoms, const double2 *lj, const double *sigh,
-                       const double *qv, const double *rho, const double *g, double *partials,
-                       int nblk, double *out, hipStream_t st) {
+                       const double *qv, const double *rho, const double *g, int u0, int nb,
+                       double *partials, int nblk, double *out, hipStream_t st) {
   hipLaunchKernelGGL(k_grad_partial, dim3(nblk), dim3(kThreads), 0, st, nx, ny, nz, dr[0], dr[1], dr[2],
-                     V, natu, atoms, lj, sigh, qv, rho, g, partials);
+                     V, natu, atoms, lj, sigh, qv, rho, g, u0, nb, partials);
   const double dv = dr[0] * dr[1] * dr[2];
-  hipLaunchKernelGGL(k_grad_sum, dim3((unsigned)((natu * 6 + kThreads - 1) / kThreads)), dim3(kThreads),
-                     0, st, partials, nblk, natu, dv, out);
+  hipLaunchKernelGGL(k_grad_sum, dim3((unsigned)((nb * 6 + kThreads - 1) / kThreads)), dim3(kThreads),
+                     0, st, partials, nblk, nb, dv, out + (long)u0 * 6);
   return hipGetLastError();
 }
 

@@ -291,9 +291,11 @@ hipError_t launch_euv_atoms(int nx, int ny, int nz, const double *dr, int V, int
                             const double *g, double *partials, int nblk, double *out, hipStream_t st);
 hipError_t launch_qv(long N, int V, const double *rho, const double *qv, const double *g, double *out,
                      hipStream_t st);
+// cal_grad sums for the atoms u0 .. u0+nb-1: partials [nblk][nb][6], results
+// into out[u0 .. u0+nb-1][6]
 hipError_t launch_grad(int nx, int ny, int nz, const double *dr, int V, int natu,
                        const double4 *atoms, const double2 *lj, const double *sigh,
-                       const double *qv, const double *rho, const double *g, double *partials,
-                       int nblk, double *out, hipStream_t st);
+                       const double *qv, const double *rho, const double *g, int u0, int nb,
+                       double *partials, int nblk, double *out, hipStream_t st);
 
 }  // namespace rism

@@ -340,7 +340,7 @@ template <typename T> int dalloc_t(rism_handle *h, T **p, size_t count) {
 
 // per-handle scratch for the one-off reductions (grad, euv, qv): grown on
 // demand through the buffer cache, never hipMalloc/hipFree per call
-int scratch(rism_handle *h, size_t bytes, void **p) {
+int handle_scratch(rism_handle *h, size_t bytes, void **p) {
   bytes = (bytes + 255) & ~(size_t)255;
   if (h->scratch_bytes < bytes) {
     dfree(h, h->scratch);
@@ -1267,6 +1267,39 @@ int rism::set_solute(rism_handle *h, const rism_solute *su) {
   return RISM_OK;
 }
 
+// ---------------------------------------------------------------------
+// Static device scratch of rism_grad_noalloc: one copy per GPU, held until
+// the process ends.  In host code `scratch` is a stand-in for the symbol,
+// not a device address; each device's own address comes from
+// hipGetSymbolAddress, called once per device with that device current,
+// and only that address is handed to kernels.
+static __device__ __align__(256) unsigned char scratch[RISM_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {nullptr};
+
+int grad_scratch_ptr(int device, void **p) {
+  if (device < 0 || device >= kMaxDevices) {
+    set_error("device index out of range for the static scratch");
+    return RISM_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {  // the caller has made `device` current
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      set_error(std::string("hipGetSymbolAddress of the static scratch failed: ") + hipGetErrorString(e));
+      return RISM_ERR_HIP;
+    }
+    g_scratch_ptr[device] = ptr;
+  }
+  *p = g_scratch_ptr[device];
+  return RISM_OK;
+}
+}  // namespace
+
 // =====================================================================
 // C ABI
 // =====================================================================
@@ -1540,10 +1573,10 @@ int rism_grad(rism_handle *h, double *out) {
   const int nblk = (int)((h->N + (long)kThreads * 16 - 1) / ((long)kThreads * 16));
   const size_t bpart = align256((size_t)nblk * nu * 6 * sizeof(double));
   void *sp = nullptr;
-  if (int rc = scratch(h, bpart + (size_t)nu * 6 * sizeof(double), &sp)) return rc;
+  if (int rc = handle_scratch(h, bpart + (size_t)nu * 6 * sizeof(double), &sp)) return rc;
   double *part = (double *)sp, *dout = (double *)((char *)sp + bpart);
   hipError_t e = launch_grad(h->nx, h->ny, h->nz, h->dr, h->V, nu, h->atoms, h->lj, h->sigh, h->d_qv, h->d_rho,
-                             h->gfield, part, nblk, dout, h->stream);
+                             h->gfield, 0, nu, part, nblk, dout, h->stream);
   std::vector<double> g((size_t)nu * 6);
   if (e == hipSuccess) e = hipMemcpyAsync(g.data(), dout, g.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
@@ -1553,6 +1586,51 @@ int rism_grad(rism_handle *h, double *out) {
   }
   // kernel layout per atom: LJ xyz then ES xyz, already x dv (output_grad.cu:15-24)
   std::memcpy(out, g.data(), g.size() * sizeof(double));
+  return RISM_OK;
+}
+
+// rism_grad without any device allocation: the partial sums live in the
+// library's static device array (RISM_NOALLOC_SCRATCH_BYTES per GPU, see
+// grad_scratch_ptr below), the atoms taken in batches that fit it.  The
+// kernels and their summation order are rism_grad's, so the results are
+// identical.  Every partial and result slot is written before it is read,
+// so nothing is cleared; calls that overlap on one GPU share the array and
+// race.
+int rism_grad_noalloc(rism_handle *h, double *out) {
+  if (!h || !out) { set_error("null argument"); return RISM_ERR_ARG; }
+  if (!h->solved) { set_error("rism_solve must run before rism_grad_noalloc"); return RISM_ERR_STATE; }
+  HIP_OK(hipSetDevice(h->device));
+  const int nu = h->natu;
+  if (nu == 0) return RISM_OK;
+  const int nblk = (int)((h->N + (long)kThreads * 16 - 1) / ((long)kThreads * 16));
+  const size_t bout = align256((size_t)nu * 6 * sizeof(double));
+  const size_t per_atom = (size_t)nblk * 6 * sizeof(double);
+  if (bout + per_atom > (size_t)RISM_NOALLOC_SCRATCH_BYTES) {
+    set_error("rism_grad_noalloc: " + std::to_string(nu) + " atoms on " + std::to_string(h->N) +
+              " points need " + std::to_string(bout + per_atom) + " bytes of scratch for one atom batch; the limit is " +
+              std::to_string((long)RISM_NOALLOC_SCRATCH_BYTES));
+    return RISM_ERR_ARG;
+  }
+  void *sp = nullptr;
+  if (int rc = grad_scratch_ptr(h->device, &sp)) return rc;
+  // evaluate() allocates nothing; per-kernel timing (which may create events) is off meanwhile
+  const bool prof = h->prof;
+  h->prof = false;
+  int rc = h->evaluated ? RISM_OK : evaluate(h);
+  h->prof = prof;
+  if (rc) return rc;
+  double *dout = (double *)sp, *part = (double *)((char *)sp + bout);
+  const int nb = (int)std::min<size_t>((size_t)nu, ((size_t)RISM_NOALLOC_SCRATCH_BYTES - bout) / per_atom);
+  hipError_t e = hipSuccess;
+  for (int u0 = 0; u0 < nu && e == hipSuccess; u0 += nb)
+    e = launch_grad(h->nx, h->ny, h->nz, h->dr, h->V, nu, h->atoms, h->lj, h->sigh, h->d_qv, h->d_rho, h->gfield,
+                    u0, std::min(nb, nu - u0), part, nblk, dout, h->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dout, (size_t)nu * 6 * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) {
+    set_error(std::string("HIP error in rism_grad_noalloc: ") + hipGetErrorString(e));
+    return RISM_ERR_HIP;
+  }
   return RISM_OK;
 }
 
@@ -1567,7 +1645,7 @@ int rism_euv(rism_handle *h, double *out) {
   const int nblk = (int)((h->N + (long)kThreads * 16 - 1) / ((long)kThreads * 16));
   const size_t bpart = align256((size_t)nblk * nu * 2 * V * sizeof(double));
   void *sp = nullptr;
-  if (int rc = scratch(h, bpart + (size_t)nu * 2 * V * sizeof(double), &sp)) return rc;
+  if (int rc = handle_scratch(h, bpart + (size_t)nu * 2 * V * sizeof(double), &sp)) return rc;
   double *part = (double *)sp, *dout = (double *)((char *)sp + bpart);
   hipError_t e = launch_euv_atoms(h->nx, h->ny, h->nz, h->dr, V, nu, h->atoms, h->lj, h->sigh, h->d_qv, h->d_rho,
                                   h->gfield, part, nblk, dout, h->stream);
@@ -1588,7 +1666,7 @@ int rism_qv(rism_handle *h, double *out) {
   if (!h->evaluated)
     if (int rc = evaluate(h)) return rc;
   void *sp = nullptr;
-  if (int rc = scratch(h, (size_t)h->N * sizeof(double), &sp)) return rc;
+  if (int rc = handle_scratch(h, (size_t)h->N * sizeof(double), &sp)) return rc;
   double *d = (double *)sp;
   hipError_t e = launch_qv(h->N, h->V, h->d_rho, h->d_qv, h->gfield, d, h->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(out, d, (size_t)h->N * sizeof(double), hipMemcpyDeviceToHost, h->stream);
@@ -1869,7 +1947,7 @@ int rism_adjoint_grad(rism_handle *h, double rtol, double *dxyz, double *dq, dou
     const size_t b6 = align256((size_t)gblk * nu * 6 * sizeof(double)),
                  b4 = align256((size_t)gblk * nu * 4 * sizeof(double));
     void *sp = nullptr;
-    if (int rc2 = scratch(h, b6 + b4 + (size_t)nu * 10 * sizeof(double), &sp)) return rc2;
+    if (int rc2 = handle_scratch(h, b6 + b4 + (size_t)nu * 10 * sizeof(double), &sp)) return rc2;
     double *p6 = (double *)sp, *p4 = (double *)((char *)sp + b6), *o = (double *)((char *)sp + b6 + b4);
     hipError_t e = launch_adj_grad(a, h->adj_Y, p6, p4, o, o + (size_t)nu * 6, st);
     std::vector<double> r((size_t)nu * 10);

@@ -33,7 +33,7 @@ FIELD_T, FIELD_G, FIELD_H, FIELD_C, FIELD_RES, FIELD_ULJ, FIELD_DE, FIELD_FR, FI
 EXPORTED = [
     "rism_last_error", "rism_version", "rism_device_count", "rism_deck_read", "rism_deck_free",
     "rism_deck_views", "rism_xvk_read", "rism_create", "rism_build_potentials", "rism_solve",
-    "rism_thermo", "rism_grad", "rism_get_field", "rism_set_t", "rism_save_tuv", "rism_load_tuv",
+    "rism_thermo", "rism_grad", "rism_grad_noalloc", "rism_get_field", "rism_set_t", "rism_save_tuv", "rism_load_tuv",
     "rism_write_xmu", "rism_write_gra", "rism_stats", "rism_rms_history", "rism_destroy", "rism_release_cache",
     "rism_fft3d", "rism_calculate", "rism_profile", "rism_kernel_times", "rism_geometry",
     "rism_adjoint_grad", "rism_batch_solve", "rism_chi_default_opts", "rism_chi_nreduced",
@@ -149,6 +149,7 @@ def lib():
         L.rism_solve.argtypes = [C.c_void_p, C.c_char_p, P(C.c_int), dp]
         L.rism_thermo.argtypes = [C.c_void_p, P(Xmu)]
         L.rism_grad.argtypes = [C.c_void_p, dp]
+        L.rism_grad_noalloc.argtypes = [C.c_void_p, dp]
         L.rism_get_field.argtypes = [C.c_void_p, C.c_int, dp]
         L.rism_set_t.argtypes = [C.c_void_p, dp]
         L.rism_save_tuv.argtypes = [C.c_void_p, C.c_char_p]
@@ -479,6 +480,13 @@ class Solver:
     def grad(self) -> np.ndarray:
         out = np.zeros(self.natom * 6)
         _check(lib().rism_grad(self.h, _dptr(out)), "rism_grad")
+        return out.reshape(self.natom, 6)
+
+    def grad_noalloc(self) -> np.ndarray:
+        """rism_grad through the entry point that allocates no device memory
+        (partial sums in the library's static per-GPU scratch)."""
+        out = np.zeros(self.natom * 6)
+        _check(lib().rism_grad_noalloc(self.h, _dptr(out)), "rism_grad_noalloc")
         return out.reshape(self.natom, 6)
 
     def adjoint_grad(self, rtol: float = 1e-10) -> dict:

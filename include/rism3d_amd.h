@@ -206,6 +206,18 @@ int rism_thermo(rism_handle *h, rism_xmu *out);
 /* cal_grad (cal_grad.cu:8-48) scaled as written by output_grad.cu:15-24:
    out[n][6] = (dLJ_x, dLJ_y, dLJ_z, dES_x, dES_y, dES_z), J/mol/Angstrom */
 int rism_grad(rism_handle *h, double *out);
+/* rism_grad that allocates no device memory, for callers that cannot
+   allocate (code under HIP graph capture, tight serving or MD loops,
+   harnesses that forbid hipMalloc).  No counterpart in the reference.  Its
+   partial sums use a static per-GPU device array of
+   RISM_NOALLOC_SCRATCH_BYTES; the atoms are taken in batches that fit it.
+   Returns RISM_ERR_ARG when one atom's partials plus the output do not fit
+   (48 * (natom + ceil(N / 4096)) bytes above the limit, i.e. natom plus
+   N / 4096 above about 700 000).  The results equal rism_grad's.  Calls that overlap on
+   one GPU share the array and race; the handle's stream orders calls on one
+   handle. */
+#define RISM_NOALLOC_SCRATCH_BYTES (32L << 20)
+int rism_grad_noalloc(rism_handle *h, double *out);
 /* Adjoint gradient of Delta-mu (SFE_SC, J/mol) with respect to the solute
    parameters, at the converged state of the last rism_solve.  New: the
    reference has no counterpart (its cal_grad, rism_grad above, is the
