@@ -1,4 +1,7 @@
-// k_adjoint.hip -- adjoint of the converged 3D-RISM fixed point (FP64).
+// k_adjoint.hip -- adjoint of the converged 3D-RISM fixed point.  All
+// arithmetic is FP64; the spectrum of the iterations' K^T applications is
+// complex FP64, or complex FP32 in the mixed path (T2 = float2: FP32
+// transforms, everything else FP64).
 //
 // The solve finds the short-range state s with
 //   R(s) = h(s) + 1 - g(s) = 0,  g = closure(s - u),  c_s = g - 1 - s,
@@ -53,9 +56,16 @@ __global__ __launch_bounds__(kThreads) void k_adj_fields(AdjArgs a) {
   }
 }
 
+template <typename T2> __device__ __forceinline__ T2 mk2(double a, double b) {
+  T2 r;
+  r.x = (decltype(r.x))a;
+  r.y = (decltype(r.y))b;
+  return r;
+}
+
 // Z_p = x_{2p} + i x_{2p+1} (+ y): two real site fields per complex field
-__global__ __launch_bounds__(kThreads) void k_adj_pack(long N, int V, const double *x, const double *y,
-                                                       double2 *Z) {
+template <typename T2>
+__global__ __launch_bounds__(kThreads) void k_adj_pack(long N, int V, const double *x, const double *y, T2 *Z) {
   const long ip = (long)blockIdx.x * kThreads + threadIdx.x;
   if (ip >= N) return;
   for (int p = 0; p < (V + 1) / 2; ++p) {
@@ -65,14 +75,15 @@ __global__ __launch_bounds__(kThreads) void k_adj_pack(long N, int V, const doub
       re += y[i0];
       if (2 * p + 1 < V) im += y[i1];
     }
-    Z[(long)p * N + ip] = make_double2(re, im);
+    Z[(long)p * N + ip] = mk2<T2>(re, im);
   }
 }
 
 // k-space: out_v(k) = sum_w chi_wv(|k|) X_w(k) on a full spectrum (pairs
 // ip and N-1-ip are k and -k on the half-shifted grid); optionally
 // Y(k) = sum_w conj(X_w(k)) sum_v chi_wv(k) q_v.
-__global__ __launch_bounds__(kThreads) void k_adj_kspace(AdjArgs a, double2 *Z, double2 *Y) {
+template <typename T2>
+__global__ __launch_bounds__(kThreads) void k_adj_kspace(AdjArgs a, T2 *Z, double2 *Y) {
   const long N = a.N;
   const long ip = (long)blockIdx.x * kThreads + threadIdx.x;
   if (ip >= N / 2) return;
@@ -98,7 +109,7 @@ __global__ __launch_bounds__(kThreads) void k_adj_kspace(AdjArgs a, double2 *Z, 
     }
   double2 X[kMaxV];  // X_w(k)
   for (int p = 0; p < P; ++p) {
-    const double2 za = Z[(long)p * N + ip], zb = Z[(long)p * N + im];
+    const T2 za = Z[(long)p * N + ip], zb = Z[(long)p * N + im];
     X[2 * p] = make_double2(0.5 * (za.x + zb.x), 0.5 * (za.y - zb.y));
     if (2 * p + 1 < V) X[2 * p + 1] = make_double2(0.5 * (za.y + zb.y), -0.5 * (za.x - zb.x));
   }
@@ -126,8 +137,8 @@ __global__ __launch_bounds__(kThreads) void k_adj_kspace(AdjArgs a, double2 *Z, 
   for (int p = 0; p < P; ++p) {
     const double2 o0 = O[2 * p], o1 = O[2 * p + 1];
     // Z(k) = O0 + i O1 ; Z(-k) = conj(O0) + i conj(O1)
-    Z[(long)p * N + ip] = make_double2(o0.x - o1.y, o0.y + o1.x);
-    Z[(long)p * N + im] = make_double2(o0.x + o1.y, o1.x - o0.y);
+    Z[(long)p * N + ip] = mk2<T2>(o0.x - o1.y, o0.y + o1.x);
+    Z[(long)p * N + im] = mk2<T2>(o0.x + o1.y, o1.x - o0.y);
   }
 }
 
@@ -170,8 +181,8 @@ __global__ __launch_bounds__(kThreads) void k_adj_update(AdjArgs a) {
 
 // residual r = (g' - 1)(K^T lambda - lambda) + b - lambda and the Anderson
 // inner products (same partial layout as the forward residual pass)
-template <int HM>
-__global__ __launch_bounds__(kThreads) void k_adj_resid(AdjArgs a, const double2 *Z) {
+template <int HM, typename T2>
+__global__ __launch_bounds__(kThreads) void k_adj_resid(AdjArgs a, const T2 *Z) {
   constexpr int NV = 1 + HM + HM * (HM + 1) / 2;
   const Ctl *ctl = a.ctl;
   if (ctl->run_iter != ctl->cur) return;
@@ -183,8 +194,8 @@ __global__ __launch_bounds__(kThreads) void k_adj_resid(AdjArgs a, const double2
   for (long ip = (long)blockIdx.x * kThreads + threadIdx.x; ip < N; ip += (long)gridDim.x * kThreads) {
     for (int v = 0; v < a.V; ++v) {
       const long iv = (long)v * N + ip;
-      const double2 z = Z[(long)(v >> 1) * N + ip];
-      const double kl = (v & 1) ? z.y : z.x;
+      const T2 z = Z[(long)(v >> 1) * N + ip];
+      const double kl = (double)((v & 1) ? z.y : z.x);
       const double l = a.lam[iv];
       const double r = (a.gp[iv] - 1.0) * (kl - l) + a.b[iv] - l;
       a.res[iv] = r;
@@ -367,12 +378,14 @@ hipError_t launch_adj_fields(const AdjArgs &a, hipStream_t st) {
   hipLaunchKernelGGL(k_adj_fields, agrid(a.N), dim3(kThreads), 0, st, a);
   return hipGetLastError();
 }
-hipError_t launch_adj_pack(const AdjArgs &a, const double *x, const double *y, double2 *Z, hipStream_t st) {
-  hipLaunchKernelGGL(k_adj_pack, agrid(a.N), dim3(kThreads), 0, st, a.N, a.V, x, y, Z);
+template <typename T2>
+hipError_t launch_adj_pack(const AdjArgs &a, const double *x, const double *y, T2 *Z, hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_pack<T2>, agrid(a.N), dim3(kThreads), 0, st, a.N, a.V, x, y, Z);
   return hipGetLastError();
 }
-hipError_t launch_adj_kspace(const AdjArgs &a, double2 *Z, double2 *Y, hipStream_t st) {
-  hipLaunchKernelGGL(k_adj_kspace, agrid(a.N / 2), dim3(kThreads), 0, st, a, Z, Y);
+template <typename T2>
+hipError_t launch_adj_kspace(const AdjArgs &a, T2 *Z, double2 *Y, hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_kspace<T2>, agrid(a.N / 2), dim3(kThreads), 0, st, a, Z, Y);
   return hipGetLastError();
 }
 hipError_t launch_adj_rhs(const AdjArgs &a, const double2 *Z, hipStream_t st) {
@@ -389,15 +402,22 @@ hipError_t launch_adj_update(const AdjArgs &a, int hm, hipStream_t st) {
   }
   return hipGetLastError();
 }
-hipError_t launch_adj_resid(const AdjArgs &a, const double2 *Z, int hm, int nblk, hipStream_t st) {
+template <typename T2>
+hipError_t launch_adj_resid(const AdjArgs &a, const T2 *Z, int hm, int nblk, hipStream_t st) {
   switch (hm) {
-    case 2: hipLaunchKernelGGL(k_adj_resid<2>, dim3(nblk), dim3(kThreads), 0, st, a, Z); break;
-    case 5: hipLaunchKernelGGL(k_adj_resid<5>, dim3(nblk), dim3(kThreads), 0, st, a, Z); break;
-    case 10: hipLaunchKernelGGL(k_adj_resid<10>, dim3(nblk), dim3(kThreads), 0, st, a, Z); break;
+    case 2: hipLaunchKernelGGL((k_adj_resid<2, T2>), dim3(nblk), dim3(kThreads), 0, st, a, Z); break;
+    case 5: hipLaunchKernelGGL((k_adj_resid<5, T2>), dim3(nblk), dim3(kThreads), 0, st, a, Z); break;
+    case 10: hipLaunchKernelGGL((k_adj_resid<10, T2>), dim3(nblk), dim3(kThreads), 0, st, a, Z); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
+template hipError_t launch_adj_pack<double2>(const AdjArgs &, const double *, const double *, double2 *, hipStream_t);
+template hipError_t launch_adj_pack<float2>(const AdjArgs &, const double *, const double *, float2 *, hipStream_t);
+template hipError_t launch_adj_kspace<double2>(const AdjArgs &, double2 *, double2 *, hipStream_t);
+template hipError_t launch_adj_kspace<float2>(const AdjArgs &, float2 *, double2 *, hipStream_t);
+template hipError_t launch_adj_resid<double2>(const AdjArgs &, const double2 *, int, int, hipStream_t);
+template hipError_t launch_adj_resid<float2>(const AdjArgs &, const float2 *, int, int, hipStream_t);
 hipError_t launch_adj_bar(const AdjArgs &a, const double2 *Z, hipStream_t st) {
   hipLaunchKernelGGL(k_adj_bar, agrid(a.N), dim3(kThreads), 0, st, a, Z);
   return hipGetLastError();

@@ -205,11 +205,15 @@ struct AdjArgs {
   const double2 *phase;        // [natu][nx+ny+nz] exp(-i k R) per axis
 };
 hipError_t launch_adj_fields(const AdjArgs &a, hipStream_t st);
-hipError_t launch_adj_pack(const AdjArgs &a, const double *x, const double *y, double2 *Z, hipStream_t st);
-hipError_t launch_adj_kspace(const AdjArgs &a, double2 *Z, double2 *Y, hipStream_t st);
+// T2: the spectrum of a K^T application, double2, or float2 in the mixed path
+template <typename T2>
+hipError_t launch_adj_pack(const AdjArgs &a, const double *x, const double *y, T2 *Z, hipStream_t st);
+template <typename T2>
+hipError_t launch_adj_kspace(const AdjArgs &a, T2 *Z, double2 *Y, hipStream_t st);
 hipError_t launch_adj_rhs(const AdjArgs &a, const double2 *Z, hipStream_t st);
 hipError_t launch_adj_update(const AdjArgs &a, int hm, hipStream_t st);
-hipError_t launch_adj_resid(const AdjArgs &a, const double2 *Z, int hm, int nblk, hipStream_t st);
+template <typename T2>
+hipError_t launch_adj_resid(const AdjArgs &a, const T2 *Z, int hm, int nblk, hipStream_t st);
 hipError_t launch_adj_bar(const AdjArgs &a, const double2 *Z, hipStream_t st);
 int adj_grad_blocks(long N);
 hipError_t launch_adj_grad(const AdjArgs &a, const double2 *Y, double *part6, double *part4, double *out6,

@@ -1903,6 +1903,23 @@ int rism_adjoint_grad(rism_handle *h, double rtol, double *dxyz, double *dq, dou
     if (e == hipSuccess) e = fft(+1);
     return e;
   };
+  // The mixed path runs the iterations' K^T applications with FP32
+  // transforms on its FP32 spectrum buffer (idle after the solve); the right
+  // side b and the final application behind the field adjoints stay FP64.
+  const bool f32 = h->opts.precision == RISM_PREC_MIXED;
+  float2 *Z32 = f32 ? (float2 *)h->ps[1].Z : nullptr;
+  auto apply_kt32 = [&](const double *x) -> hipError_t {
+    const PrecSet &p1 = h->ps[1];
+    hipError_t e = launch_adj_pack(a, x, (const double *)nullptr, Z32, st);
+    for (int p = 0; p < P && e == hipSuccess; ++p)
+      e = launch_fft3d<float>(Z32 + (size_t)p * N, p1.plan[0], p1.plan[1], p1.plan[2], p1.geom, h->nx, h->ny, h->nz,
+                              h->dv, ivol, -1, st);
+    if (e == hipSuccess) e = launch_adj_kspace(a, Z32, (double2 *)nullptr, st);
+    for (int p = 0; p < P && e == hipSuccess; ++p)
+      e = launch_fft3d<float>(Z32 + (size_t)p * N, p1.plan[0], p1.plan[1], p1.plan[2], p1.geom, h->nx, h->ny, h->nz,
+                              h->dv, ivol, +1, st);
+    return e;
+  };
   HIP_OK(launch_adj_fields(a, st));
   HIP_OK(apply_kt(a.alpha, nullptr, nullptr));
   HIP_OK(launch_adj_rhs(a, h->adj_Z, st));
@@ -1915,13 +1932,21 @@ int rism_adjoint_grad(rism_handle *h, double rtol, double *dxyz, double *dq, dou
   c.tol = -1.0;  // set after the first step from rms(b)
   c.m = h->opts.m; c.mp = h->opts.mp; c.npts = (double)VN;
   HIP_OK(hipMemcpyAsync(h->adj_ctl, &c, sizeof c, hipMemcpyHostToDevice, st));
+  // FP32 transforms carry about 1e-7 of relative rounding: the mixed path's
+  // adjoint solve stops at rms <= max(rtol, 1e-6) rms(b)
+  if (f32) rtol = std::max(rtol, 1e-6);
   const int nblk = ps.geom.xblocks;
   const int nv = 1 + h->hm + h->hm * (h->hm + 1) / 2;
   int it = 0;
   for (;;) {
     HIP_OK(launch_adj_update(a, h->hm, st));
-    HIP_OK(apply_kt(a.lam, nullptr, nullptr));
-    HIP_OK(launch_adj_resid(a, h->adj_Z, h->hm, nblk, st));
+    if (f32) {
+      HIP_OK(apply_kt32(a.lam));
+      HIP_OK(launch_adj_resid(a, (const float2 *)Z32, h->hm, nblk, st));
+    } else {
+      HIP_OK(apply_kt(a.lam, nullptr, nullptr));
+      HIP_OK(launch_adj_resid(a, (const double2 *)h->adj_Z, h->hm, nblk, st));
+    }
     HIP_OK(launch_anderson_solve(h->adj_ctl, h->partials, nblk, nv, h->hm, h->adj_rms, st));
     ++it;
     if (it == 1 || it % 8 == 0 || it >= h->opts.maxstep + 1) {

@@ -228,8 +228,10 @@ int rism_grad_noalloc(rism_handle *h, double *out);
    adjoints against the derivatives of the potentials (cal_LJ.cu,
    cal_Coulomb.cu).  Outputs (any may be NULL): dxyz[n][3] J/mol/Angstrom,
    dq[n] J/mol/e, dsig[n] J/mol/Angstrom, deps[n] (J/mol per J/mol).
-   Returns RISM_OK, RISM_SOLVE_MAXSTEP (outputs from the last iterate) or an
-   error; steps receives the adjoint iteration count. */
+   Precision follows the handle: RISM_PREC_MIXED runs the iterations' K^T
+   applications with FP32 transforms (rtol floored at 1e-6), everything else
+   in FP64.  Returns RISM_OK, RISM_SOLVE_MAXSTEP (outputs from the last
+   iterate) or an error; steps receives the adjoint iteration count. */
 int rism_adjoint_grad(rism_handle *h, double rtol, double *dxyz, double *dq, double *dsig,
                       double *deps, int *steps);
 /* copy a field (RISM_FIELD_*) to host memory */
