@@ -57,6 +57,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 template <typename R, typename S, int EPT, int VC, int HM, bool UPD>
 __device__ __forceinline__ void closure_points(const IterArgs<R, S> &a, typename C2<R>::T *lds, bool do_run,
                                                const double *th, const long *off, double mix, int w, int nf) {
+  const long so = (long)blockIdx.y * a.astride;  // this launch's solute (batched launches)
   using T2 = typename C2<R>::T;
   using Q = typename HistT<R>::T;
   constexpr int CH = EPT < RISM_CLOS_CH ? EPT : RISM_CLOS_CH;
@@ -65,11 +66,11 @@ __device__ __forceinline__ void closure_points(const IterArgs<R, S> &a, typename
   const long base = (long)blockIdx.x * L * nx;
   const int nel = L * nx;
   const T2 *__restrict__ pre = (const T2 *)a.px.pre;
-  S *__restrict__ ts = a.ts;
-  const Q *__restrict__ res = a.res;
-  Q *__restrict__ ht = a.hist_t;
-  Q *__restrict__ hr = a.hist_r;
-  const R *__restrict__ uu = a.u;
+  S *__restrict__ ts = soff(a.ts, so);
+  const Q *__restrict__ res = soff(a.res, so);
+  Q *__restrict__ ht = soff(a.hist_t, so);
+  Q *__restrict__ hr = soff(a.hist_r, so);
+  const R *__restrict__ uu = soff(a.u, so);
 #pragma unroll
   for (int k0 = 0; k0 < EPT; k0 += CH) {
     double t[CH][VC], u[CH][VC];
@@ -146,8 +147,9 @@ __device__ __forceinline__ void closure_points(const IterArgs<R, S> &a, typename
 // the FP64 path's first pass (the mixed path's is k_mx_closure_fx)
 template <typename R, typename S, int EPT, bool UPDATE, int VC, int HM>
 __global__ __launch_bounds__(kThreads) void k_closure_fx(IterArgs<R, S> a) {
+  const long so = (long)blockIdx.y * a.astride;  // this launch's solute (batched launches)
   using T2 = typename C2<R>::T;
-  const Ctl *ctl = a.ctl;
+  const Ctl *ctl = soff(a.ctl, so);
   const int iter = ctl->cur;
   const bool do_upd = UPDATE && (ctl->upd_iter == iter);
   const bool do_run = (ctl->run_iter == iter);
@@ -194,7 +196,7 @@ __global__ __launch_bounds__(kThreads) void k_closure_fx(IterArgs<R, S> a) {
   // (measured 0-1% faster, and no radix-8/16 stage is compiled in)
   fft_lines<EPT, false, 5>(lds, P * L, ls, a.pxr, ltw);
   const R dv = (R)a.dv;
-  T2 *__restrict__ Z = a.Z;
+  T2 *__restrict__ Z = soff(a.Z, so);
 #pragma unroll
   for (int k = 0; k < EPT; ++k) {
     const int f = threadIdx.x + k * kThreads;  // (pack, element) flattened
@@ -215,9 +217,12 @@ __global__ __launch_bounds__(kThreads) void k_closure_fx(IterArgs<R, S> a) {
 template <typename R, int EPT, bool INV, int NL = 0>
 __global__ __launch_bounds__(kThreads) void k_fft_y(typename C2<R>::T *Z, const Ctl *ctl, int nx,
                                                      int ny_rt, int nz, long N, int T_rt, int P, AxisPlan py,
-                                                     int gate) {
+                                                     int gate, long astride) {
   // NL > 0: compile-time line length (power of two) with T = EPT * kThreads / NL
   using T2 = typename C2<R>::T;
+  const long so = (long)blockIdx.y * astride;  // this launch's solute (batched launches)
+  Z = soff(Z, so);
+  if (ctl) ctl = soff(ctl, so);
   if (ctl && !gate_open(ctl, gate)) return;
   extern __shared__ unsigned char smem_raw[];
   T2 *lds = reinterpret_cast<T2 *>(smem_raw);
@@ -278,10 +283,11 @@ __global__ __launch_bounds__(kThreads) void k_fft_y(typename C2<R>::T *Z, const 
 constexpr int kXcds = 8;
 template <typename R, typename S, int EPT, int VC, int NL = 0>
 __global__ __launch_bounds__(kThreads) void k_fft_z_oz(IterArgs<R, S> a) {
+  const long so = (long)blockIdx.y * a.astride;  // this launch's solute (batched launches)
   // NL > 0: compile-time z length (power of two), one pack (VC == 2), T = EPT * kThreads / (2 NL)
   using T2 = typename C2<R>::T;
   using T4 = typename R4T<R>::T;
-  if (!gate_open(a.ctl, a.gate)) return;
+  if (!gate_open(soff(a.ctl, so), a.gate)) return;
   extern __shared__ unsigned char smem_raw[];
   T2 *lds = reinterpret_cast<T2 *>(smem_raw);
   const int nx = a.nx, ny = a.ny, V = a.V;
@@ -295,7 +301,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_oz(IterArgs<R, S> a) {
   const int t = (ntile % kXcds) ? b : (b % kXcds) * (ntile / kXcds) + b / kXcds;
   const int xa = (t % nxt) * T, ya = t / nxt;
   const int xb = nx - xa - T, yb = ny - 1 - ya;
-  T2 *__restrict__ Z = a.Z;
+  T2 *__restrict__ Z = soff(a.Z, so);
   const int nel = 2 * P * T * nz;
   // element e of the tile pair: line (p*2 + tile)*T + i, position z
   auto gofs = [&](int e) -> long {
@@ -354,7 +360,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_oz(IterArgs<R, S> a) {
     const double hx = 0.5 - nx / 2, hz = 0.5 - nz / 2;
     const int nseg = a.ntab - 1;
     const T4 *__restrict__ chi = a.chi;
-    const T2 *__restrict__ fkh = a.fkh;
+    const T2 *__restrict__ fkh = soff(a.fkh, so);
     const int npair = T * nz;
     int tido = tid;
     asm volatile("" : "+v"(tido));
@@ -612,9 +618,11 @@ static __device__ void anderson_solve_block(Ctl *ctl, const double *partials, in
 
 template <int HM>
 __global__ __launch_bounds__(kThreads) void k_anderson_solve(Ctl *ctl, const double *partials, int nblk,
-                                                             double *rms_hist, int gate) {
+                                                             double *rms_hist, int gate, long astride) {
+  const long so = (long)blockIdx.y * astride;  // this launch's solute (batched launches)
+  ctl = soff(ctl, so);
   if (!gate_open(ctl, gate)) return;
-  anderson_solve_block<HM>(ctl, partials, nblk, rms_hist);
+  anderson_solve_block<HM>(ctl, soff(partials, so), nblk, soff(rms_hist, so));
 }
 
 // ---------------------------------------------------------------------
@@ -623,10 +631,11 @@ __global__ __launch_bounds__(kThreads) void k_anderson_solve(Ctl *ctl, const dou
 // ---------------------------------------------------------------------
 template <typename R, typename S, int EPT, int HM, int VC>
 __global__ __launch_bounds__(kThreads) void k_fx_resid(IterArgs<R, S> a) {
+  const long so = (long)blockIdx.y * a.astride;  // this launch's solute (batched launches)
   using T2 = typename C2<R>::T;
   using Q = typename HistT<R>::T;
   constexpr int NV = 1 + HM + HM * (HM + 1) / 2;
-  const Ctl *ctl = a.ctl;
+  const Ctl *ctl = soff(a.ctl, so);
   if (ctl->run_iter != ctl->cur) return;
   extern __shared__ unsigned char smem_raw[];
   T2 *lds = reinterpret_cast<T2 *>(smem_raw);
@@ -639,11 +648,11 @@ __global__ __launch_bounds__(kThreads) void k_fx_resid(IterArgs<R, S> a) {
   const T2 *pre = ltw + nx;
   stage_tables(ltw, a.px);  // visible after the first barrier of the group loop
   const R ivol = (R)a.ivol;
-  T2 *__restrict__ Z = a.Z;
-  const S *__restrict__ ts = a.ts;
-  const R *__restrict__ uu = a.u;
-  Q *__restrict__ res = a.res;
-  const Q *__restrict__ hr = a.hist_r;
+  T2 *__restrict__ Z = soff(a.Z, so);
+  const S *__restrict__ ts = soff(a.ts, so);
+  const R *__restrict__ uu = soff(a.u, so);
+  Q *__restrict__ res = soff(a.res, so);
+  const Q *__restrict__ hr = soff(a.hist_r, so);
   double acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) acc[k] = 0.0;
@@ -797,7 +806,7 @@ __global__ __launch_bounds__(kThreads) void k_fx_resid(IterArgs<R, S> a) {
   for (int k = threadIdx.x; k < NV; k += kThreads) {
     double sm = 0.0;
     for (int wv = 0; wv < kThreads / 64; ++wv) sm += red[wv][k];
-    a.partials[(long)blockIdx.x * NV + k] = sm;
+    soff(a.partials, so)[(long)blockIdx.x * NV + k] = sm;
   }
 }
 
@@ -891,7 +900,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_pure(typename C2<R>::T *Z, i
 template <typename R, typename S>
 hipError_t launch_closure_fx(const IterArgs<R, S> &a, const LaunchGeom &g, bool update, int hm,
                              hipStream_t st) {
-  const dim3 grid(g.cgroups), block(kThreads);
+  const dim3 grid(g.cgroups, a.nsol > 1 ? a.nsol : 1), block(kThreads);
   if (!update) return hipErrorInvalidValue;  // every launch carries the history class
 #define RISM_CLOS(HMV) \
   RISM_VC_SWITCH(a.V, RISM_EPTX_SWITCH(g.ept_c, hipLaunchKernelGGL((k_closure_fx<R, S, E, true, VC, HMV>), grid, block, g.lds_c, st, a)))
@@ -926,27 +935,27 @@ inline int ct_length(int n, long tile_elems, int ept) {
 
 template <typename R, typename S>
 hipError_t launch_fft_y(const IterArgs<R, S> &a, const LaunchGeom &g, bool inverse, hipStream_t st) {
-  const dim3 grid(a.nx / g.Ty * a.nz * a.P), block(kThreads);
+  const dim3 grid(a.nx / g.Ty * a.nz * a.P, a.nsol > 1 ? a.nsol : 1), block(kThreads);
   const int nl = ct_length(a.ny, (long)g.Ty * a.ny, g.ept_y);
   if (inverse) {
     RISM_CT_SWITCH(nl, g.ept_y,
                    RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, true>), grid, block, g.lds_y, st, a.Z,
-                                                               a.ctl, a.nx, a.ny, a.nz, a.N, g.Ty, a.P, a.py, a.gate)),
+                                                               a.ctl, a.nx, a.ny, a.nz, a.N, g.Ty, a.P, a.py, a.gate, a.astride)),
                    hipLaunchKernelGGL((k_fft_y<R, E, true, NLV>), grid, block, g.lds_y, st, a.Z, a.ctl, a.nx,
-                                      a.ny, a.nz, a.N, g.Ty, a.P, a.py, a.gate))
+                                      a.ny, a.nz, a.N, g.Ty, a.P, a.py, a.gate, a.astride))
   } else {
     RISM_CT_SWITCH(nl, g.ept_y,
                    RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, false>), grid, block, g.lds_y, st, a.Z,
-                                                               a.ctl, a.nx, a.ny, a.nz, a.N, g.Ty, a.P, a.py, a.gate)),
+                                                               a.ctl, a.nx, a.ny, a.nz, a.N, g.Ty, a.P, a.py, a.gate, a.astride)),
                    hipLaunchKernelGGL((k_fft_y<R, E, false, NLV>), grid, block, g.lds_y, st, a.Z, a.ctl, a.nx,
-                                      a.ny, a.nz, a.N, g.Ty, a.P, a.py, a.gate))
+                                      a.ny, a.nz, a.N, g.Ty, a.P, a.py, a.gate, a.astride))
   }
   return hipGetLastError();
 }
 
 template <typename R, typename S>
 hipError_t launch_fft_z_oz(const IterArgs<R, S> &a, const LaunchGeom &g, hipStream_t st) {
-  const dim3 grid(a.nx / g.Tz * (a.ny / 2)), block(kThreads);
+  const dim3 grid(a.nx / g.Tz * (a.ny / 2), a.nsol > 1 ? a.nsol : 1), block(kThreads);
   const int nl = a.V <= 2 ? ct_length(a.nz, 2L * g.Tz * a.nz, g.ept_z) : 0;
   RISM_CT_SWITCH(nl, g.ept_z,
                  RISM_VC_SWITCH(a.V, RISM_EPT_SWITCH(g.ept_z, hipLaunchKernelGGL((k_fft_z_oz<R, S, E, VC>), grid, block,
@@ -957,7 +966,7 @@ hipError_t launch_fft_z_oz(const IterArgs<R, S> &a, const LaunchGeom &g, hipStre
 
 template <typename R, typename S>
 hipError_t launch_fx_resid(const IterArgs<R, S> &a, const LaunchGeom &g, int hm, hipStream_t st) {
-  const dim3 grid(g.xblocks), block(kThreads);
+  const dim3 grid(g.xblocks, a.nsol > 1 ? a.nsol : 1), block(kThreads);
 #define RISM_RESID(HMV)                                                                       \
   RISM_VC_SWITCH(a.V, RISM_EPTX_SWITCH(g.ept_x, hipLaunchKernelGGL((k_fx_resid<R, S, E, HMV, VC>), grid, block, g.lds_x, st, a)))
   switch (hm) {
@@ -971,11 +980,12 @@ hipError_t launch_fx_resid(const IterArgs<R, S> &a, const LaunchGeom &g, int hm,
 }
 
 inline hipError_t launch_anderson_solve_impl(Ctl *ctl, const double *partials, int nblk, int nv, int hm,
-                                 double *rms_hist, hipStream_t st, int gate) {
+                                 double *rms_hist, hipStream_t st, int gate, long astride, int nsol) {
+  const dim3 grid(1, nsol > 1 ? nsol : 1);
   switch (hm) {
-    case 2: hipLaunchKernelGGL(k_anderson_solve<2>, dim3(1), dim3(kThreads), 0, st, ctl, partials, nblk, rms_hist, gate); break;
-    case 5: hipLaunchKernelGGL(k_anderson_solve<5>, dim3(1), dim3(kThreads), 0, st, ctl, partials, nblk, rms_hist, gate); break;
-    case 10: hipLaunchKernelGGL(k_anderson_solve<10>, dim3(1), dim3(kThreads), 0, st, ctl, partials, nblk, rms_hist, gate); break;
+    case 2: hipLaunchKernelGGL(k_anderson_solve<2>, grid, dim3(kThreads), 0, st, ctl, partials, nblk, rms_hist, gate, astride); break;
+    case 5: hipLaunchKernelGGL(k_anderson_solve<5>, grid, dim3(kThreads), 0, st, ctl, partials, nblk, rms_hist, gate, astride); break;
+    case 10: hipLaunchKernelGGL(k_anderson_solve<10>, grid, dim3(kThreads), 0, st, ctl, partials, nblk, rms_hist, gate, astride); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -996,11 +1006,11 @@ hipError_t launch_fft3d(typename C2<R>::T *data, const AxisPlan &px, const AxisP
   const int ez = (g.Tz * nz <= 4 * kThreads) ? 4 : (g.Tz * nz <= 8 * kThreads) ? 8 : (g.Tz * nz <= 16 * kThreads) ? 16 : 32;
   if (dir < 0) {
     RISM_EPTX_SWITCH(g.ept_x, hipLaunchKernelGGL((k_fft_x_pure<R, E, false>), gx, block, ldsx, st, data, nx, L, px, (R)dv))
-    RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, false>), gy, block, ldsy, st, data, (const Ctl *)nullptr, nx, ny, nz, N, g.Ty, 1, py, 0))
+    RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, false>), gy, block, ldsy, st, data, (const Ctl *)nullptr, nx, ny, nz, N, g.Ty, 1, py, 0, 0L))
     RISM_EPT_SWITCH(ez, hipLaunchKernelGGL((k_fft_z_pure<R, E, false>), gz, block, ldsz, st, data, nx, ny, nz, g.Tz, pz))
   } else {
     RISM_EPT_SWITCH(ez, hipLaunchKernelGGL((k_fft_z_pure<R, E, true>), gz, block, ldsz, st, data, nx, ny, nz, g.Tz, pz))
-    RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, true>), gy, block, ldsy, st, data, (const Ctl *)nullptr, nx, ny, nz, N, g.Ty, 1, py, 0))
+    RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, true>), gy, block, ldsy, st, data, (const Ctl *)nullptr, nx, ny, nz, N, g.Ty, 1, py, 0, 0L))
     RISM_EPTX_SWITCH(g.ept_x, hipLaunchKernelGGL((k_fft_x_pure<R, E, true>), gx, block, ldsx, st, data, nx, L, px, (R)ivol))
   }
   return hipGetLastError();

@@ -49,6 +49,7 @@ template <typename R, int EPT, int VC, int HM, bool UPD>
 __device__ __forceinline__ void mx_points(const IterArgs<R, double> &a, typename C2<R>::T *lds, bool do_run,
                                           const double *th, const long *toff, const long *roff, long rcur_off,
                                           double mix, int w, int nf) {
+  const long so = (long)blockIdx.y * a.astride;  // this launch's solute (batched launches)
   using T2 = typename C2<R>::T;
   constexpr bool F64 = sizeof(R) == sizeof(double);  // the refresh chain
   constexpr int CH = EPT < RISM_CLOS_CH ? EPT : RISM_CLOS_CH;
@@ -57,14 +58,14 @@ __device__ __forceinline__ void mx_points(const IterArgs<R, double> &a, typename
   const long base = (long)blockIdx.x * L * nx;
   const int nel = L * nx;
   const T2 *__restrict__ pre = (const T2 *)a.px.pre;
-  double *__restrict__ Ab = a.A;
-  double *__restrict__ Tb = a.ts;  // the refresh chain's T
-  float *__restrict__ dlb = a.dl;
-  const float *__restrict__ hx = a.hx;
-  float *__restrict__ hxw = a.hx;
-  const float *__restrict__ rr = a.rring;
-  const float *__restrict__ rref = a.rref;
-  const double *__restrict__ uu = (const double *)a.u;
+  double *__restrict__ Ab = soff(a.A, so);
+  double *__restrict__ Tb = soff(a.ts, so);  // the refresh chain's T
+  float *__restrict__ dlb = soff(a.dl, so);
+  const float *__restrict__ hx = soff(a.hx, so);
+  float *__restrict__ hxw = soff(a.hx, so);
+  const float *__restrict__ rr = soff(a.rring, so);
+  const float *__restrict__ rref = soff(a.rref, so);
+  const double *__restrict__ uu = (const double *)soff(a.u, so);
 #pragma unroll
   for (int k0 = 0; k0 < EPT; k0 += CH) {
     double A[CH][VC], d0[CH][VC], u[CH][VC], rf[CH][VC];  // A: T in the refresh chain
@@ -136,7 +137,7 @@ __device__ __forceinline__ void mx_points(const IterArgs<R, double> &a, typename
             }
             if (do_run) {
               const double dg = closure_g<double>(A[c][v] + (double)dn, a.closure) - closure_g<double>(A[c][v], a.closure);
-              if (valid) lane_at(a.bq + vb, e) = (float)(rf[c][v] - dg);
+              if (valid) lane_at(soff(a.bq, so) + vb, e) = (float)(rf[c][v] - dg);
               cs[v] = (R)(dg - (double)dn);
             }
           }
@@ -158,8 +159,9 @@ __device__ __forceinline__ void mx_points(const IterArgs<R, double> &a, typename
 
 template <typename R, int EPT, int VC, int HM>
 __global__ __launch_bounds__(kThreads) void k_mx_closure_fx(IterArgs<R, double> a) {
+  const long so = (long)blockIdx.y * a.astride;  // this launch's solute (batched launches)
   using T2 = typename C2<R>::T;
-  const Ctl *ctl = a.ctl;
+  const Ctl *ctl = soff(a.ctl, so);
   const int iter = ctl->cur;
   const bool mine = chain_active(ctl, a.gate);
   const bool do_upd = mine && ctl->upd_iter == iter;
@@ -208,7 +210,7 @@ __global__ __launch_bounds__(kThreads) void k_mx_closure_fx(IterArgs<R, double> 
   __syncthreads();
   fft_lines<EPT, false, 5>(lds, P * L, ls, a.pxr, ltw);
   const R dv = (R)a.dv;
-  T2 *__restrict__ Z = a.Z;
+  T2 *__restrict__ Z = soff(a.Z, so);
 #pragma unroll
   for (int k = 0; k < EPT; ++k) {
     const int f = threadIdx.x + k * kThreads;  // (pack, element) flattened
@@ -226,10 +228,11 @@ __global__ __launch_bounds__(kThreads) void k_mx_closure_fx(IterArgs<R, double> 
 // ---------------------------------------------------------------------
 template <typename R, int EPT, int HM, int VC>
 __global__ __launch_bounds__(kThreads) void k_mx_resid(IterArgs<R, double> a) {
+  const long so = (long)blockIdx.y * a.astride;  // this launch's solute (batched launches)
   using T2 = typename C2<R>::T;
   constexpr bool F64 = sizeof(R) == sizeof(double);
   constexpr int NV = 1 + HM + HM * (HM + 1) / 2;
-  const Ctl *ctl = a.ctl;
+  const Ctl *ctl = soff(a.ctl, so);
   if (!gate_open(ctl, a.gate)) return;
   extern __shared__ unsigned char smem_raw[];
   T2 *lds = reinterpret_cast<T2 *>(smem_raw);
@@ -241,13 +244,13 @@ __global__ __launch_bounds__(kThreads) void k_mx_resid(IterArgs<R, double> a) {
   long hoff[HM];
 #pragma unroll
   for (int s = 0; s < HM; ++s) hoff[s] = (long)ctl->rmap[s < nf ? s : 0] * VN;
-  float *__restrict__ rw = a.rring + (long)ctl->rcur * VN;
-  const float *__restrict__ rr = a.rring;
+  float *__restrict__ rw = soff(a.rring, so) + (long)ctl->rcur * VN;
+  const float *__restrict__ rr = soff(a.rring, so);
   T2 *ltw = lds + P * L * ls;
   const T2 *pre = ltw + nx;
   stage_tables(ltw, a.px);  // visible after the first barrier of the group loop
   const R ivol = (R)a.ivol;
-  T2 *__restrict__ Z = a.Z;
+  T2 *__restrict__ Z = soff(a.Z, so);
   double acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) acc[k] = 0.0;
@@ -305,8 +308,8 @@ __global__ __launch_bounds__(kThreads) void k_mx_resid(IterArgs<R, double> a) {
         for (int v = 0; v < VC; ++v)
           if (v < V) {
             const long vb = (long)v * N + base;
-            if constexpr (F64) ov[c][v] = lane_at(a.A + vb, e);
-            else ov[c][v] = (double)lane_at(a.bq + vb, e);
+            if constexpr (F64) ov[c][v] = lane_at(soff(a.A, so) + vb, e);
+            else ov[c][v] = (double)lane_at(soff(a.bq, so) + vb, e);
 #pragma unroll
             for (int s = 0; s < (PREH ? HM : 0); ++s) hq[c][v][PREH ? s : 0] = lane_at(rr + hoff[s] + vb, e);
           }
@@ -330,7 +333,7 @@ __global__ __launch_bounds__(kThreads) void k_mx_resid(IterArgs<R, double> a) {
               double rd;
               if constexpr (F64) {
                 rd = hv[c][v] + 1.0 - closure_g<double>(ov[c][v], a.closure);  // tr: R = h + 1 - g
-                lane_at(a.rref + vb, e) = (float)rd;
+                lane_at(soff(a.rref, so) + vb, e) = (float)rd;
               } else {
                 rd = hv[c][v] + ov[c][v];  // R = K dc + B
               }
@@ -369,13 +372,13 @@ __global__ __launch_bounds__(kThreads) void k_mx_resid(IterArgs<R, double> a) {
   for (int k = threadIdx.x; k < NV; k += kThreads) {
     double sm = 0.0;
     for (int wv = 0; wv < kThreads / 64; ++wv) sm += red[wv][k];
-    a.partials[(long)blockIdx.x * NV + k] = sm;
+    soff(a.partials, so)[(long)blockIdx.x * NV + k] = sm;
   }
 }
 
 template <typename R>
 hipError_t launch_mx_closure_fx_impl(const IterArgs<R, double> &a, const LaunchGeom &g, int hm, hipStream_t st) {
-  const dim3 grid(g.cgroups), block(kThreads);
+  const dim3 grid(g.cgroups, a.nsol > 1 ? a.nsol : 1), block(kThreads);
 #define RISM_MXC(HMV) \
   RISM_VC_SWITCH(a.V, RISM_EPTX_SWITCH(g.ept_c, hipLaunchKernelGGL((k_mx_closure_fx<R, E, VC, HMV>), grid, block, g.lds_c, st, a)))
   switch (hm) {
@@ -390,7 +393,7 @@ hipError_t launch_mx_closure_fx_impl(const IterArgs<R, double> &a, const LaunchG
 
 template <typename R>
 hipError_t launch_mx_resid_impl(const IterArgs<R, double> &a, const LaunchGeom &g, int hm, hipStream_t st) {
-  const dim3 grid(g.xblocks), block(kThreads);
+  const dim3 grid(g.xblocks, a.nsol > 1 ? a.nsol : 1), block(kThreads);
 #define RISM_MXR(HMV) \
   RISM_VC_SWITCH(a.V, RISM_EPTX_SWITCH(g.ept_x, hipLaunchKernelGGL((k_mx_resid<R, E, HMV, VC>), grid, block, g.lds_x, st, a)))
   switch (hm) {

@@ -33,4 +33,11 @@ bool is_supported_length(int n);
 // (the batched solves reuse a worker's handle per grid shape).
 int set_solute(rism_handle *h, const rism_solute *su);
 
+// S solutes on one grid (same grid, box, solvent and options) in batched
+// launches: one launch per kernel covers all of them, each gated by its own
+// control block; then each solute's thermodynamics.  Per solute status,
+// steps, rms, xmu as rism_batch_solve reports them.
+int batch_solve_class(const rism_grid *grid, const rism_solvent *sv, const rism_solute *const *sus, int S,
+                      const rism_opts *o, int device, int *status, int *steps, double *rms, rism_xmu *xmu);
+
 }  // namespace rism

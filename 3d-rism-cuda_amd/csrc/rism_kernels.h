@@ -92,6 +92,11 @@ __device__ __forceinline__ bool gate_open(const Ctl *c, int gate) {
   return c->run_iter == c->cur && chain_active(c, gate);
 }
 
+// a per-solute pointer of a batched launch (o = blockIdx.y * astride bytes)
+template <typename P> __device__ __forceinline__ P soff(P p, long o) {
+  return (P)((const char *)p + o);
+}
+
 // Residual / history storage type of the FP64 path's passes (the mixed
 // path's state is described with IterArgs::A below).
 template <typename R> struct HistT;
@@ -143,6 +148,13 @@ struct IterArgs {
   // long-range part is constant) and the residual is R = K dc + B with
   // B = R_ref - (G(A + dl) - G(A)) from the closure pass.
   int gate;             // chain_active() selector
+  // ---- batched launches: nsol solutes of one grid class in one launch ----
+  // blockIdx.y selects the solute; every per-solute field (state, history,
+  // potentials, spectrum, control block, partials) of solute s sits
+  // s * astride bytes after solute 0's; the plans and the chi table are
+  // the same for all and are read from solute 0.
+  long astride;
+  int nsol;
   double *A;
   float *dl;
   float *rref;
@@ -174,7 +186,7 @@ hipError_t launch_fft_z_oz(const IterArgs<R, S> &a, const LaunchGeom &g, hipStre
 template <typename R, typename S>
 hipError_t launch_fx_resid(const IterArgs<R, S> &a, const LaunchGeom &g, int hm, hipStream_t st);
 hipError_t launch_anderson_solve(Ctl *ctl, const double *partials, int nblk, int nv, int hm,
-                                 double *rms_hist, hipStream_t st, int gate = 0);
+                                 double *rms_hist, hipStream_t st, int gate = 0, long astride = 0, int nsol = 1);
 // mixed precision: closure / residual passes of the FP32 chain (R = float)
 // and of the FP64 refresh chain (R = double), k_mixed_f32.hip / k_mixed_f64.hip
 template <typename R>

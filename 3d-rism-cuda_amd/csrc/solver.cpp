@@ -111,7 +111,19 @@ struct rism_handle {
   bool tuned = false;       // rism_autotune has chosen the tile shapes
   // device memory
   std::vector<DevBuf> bufs;
-  void *scratch = nullptr;  // one-off reductions (see scratch())
+  // batched solves: the handle's grid-sized buffers are carved, in the same
+  // order for every solute of a batch, from its slice of one arena
+  // (arena_cap bytes at arena), so each field of solute s sits s * stride
+  // bytes after solute 0's; arena_measure records the slice size instead.
+  // Solute-sized buffers (atom_alloc) never come from the arena.
+  char *arena = nullptr;
+  size_t arena_cap = 0, arena_used = 0, arena_need = 0;
+  bool arena_measure = false, atom_alloc = false;
+  // a batch member after the first: the plans, their tables and the chi
+  // spline tables are the first member's (same grid and solvent), not copies
+  const rism_handle *tmpl = nullptr;
+  bool borrowed_ctx = false;  // stream and pinned control block belong to the batch
+  void *scratch = nullptr;  // one-off reductions (see handle_scratch())
   size_t scratch_bytes = 0;
   double *ulj = nullptr, *de = nullptr, *fr = nullptr, *sr = nullptr;
   double *sel = nullptr, *seq = nullptr;  // [V][N] SE atom sums from the potentials pass
@@ -277,6 +289,19 @@ void cache_flush_locked(int device) {
 
 int dalloc(rism_handle *h, void **p, size_t bytes) {
   if (bytes == 0) bytes = 16;
+  if (!h->atom_alloc) {
+    const size_t b = (bytes + 255) & ~(size_t)255;
+    if (h->arena) {  // a batch member: the next piece of its arena slice (not in bufs)
+      if (h->arena_used + b > h->arena_cap) {
+        set_error("batch arena slice exhausted");
+        return RISM_ERR_NOMEM;
+      }
+      *p = h->arena + h->arena_used;
+      h->arena_used += b;
+      return RISM_OK;
+    }
+    if (h->arena_measure) h->arena_need += b;
+  }
   hipError_t e = hipSuccess;
   {
     std::lock_guard<std::mutex> lk(g_device_mu);
@@ -1034,8 +1059,20 @@ template <typename R> int alloc_precision(rism_handle *h) {
   using T4 = typename R4T<R>::T;
   PrecSet &ps = h->ps[pidx<R>()];
   const size_t VN = (size_t)h->V * h->N;
-  if (int rc = build_plans<R>(h)) return rc;
-  if (int rc = choose_geometry(h, ps.geom, sizeof(R))) return rc;
+  if (h->tmpl) {  // the first member's plans and tables (not owned)
+    const PrecSet &t = h->tmpl->ps[pidx<R>()];
+    for (int k = 0; k < 3; ++k) ps.plan[k] = t.plan[k];
+    ps.plan_xr = t.plan_xr;
+    ps.geom = t.geom;
+    ps.tables = t.tables;
+    ps.chi = t.chi;
+  } else {
+    h->atom_alloc = true;  // tables: outside a batch arena, shared by its members
+    int rc = build_plans<R>(h);
+    h->atom_alloc = false;
+    if (rc) return rc;
+    if ((rc = choose_geometry(h, ps.geom, sizeof(R)))) return rc;
+  }
   // the FP64 set carries the potentials; the mixed path's FP32 chain works
   // on differences to an FP64 reference and needs neither u nor the
   // long-range terms
@@ -1045,6 +1082,10 @@ template <typename R> int alloc_precision(rism_handle *h) {
     ps.fkh = h->fkh;
   }
   if (int rc = dalloc(h, &ps.Z, (size_t)h->P * h->N * sizeof(T2))) return rc;
+  if (h->tmpl) {
+    ps.ready = true;
+    return RISM_OK;
+  }
   // chi spline table [V*V][ntab-1] of (y_i, c0, c1, c2)
   const int nt = h->ntab, V = h->V;
   std::vector<double> kt(nt), y(nt), c0(nt), c1(nt), c2(nt);
@@ -1062,7 +1103,10 @@ template <typename R> int alloc_precision(rism_handle *h) {
         e.w = (R)c2[i];
       }
     }
-  if (int rc = dalloc(h, &ps.chi, tab.size() * sizeof(T4))) return rc;
+  h->atom_alloc = true;
+  const int rcc = dalloc(h, &ps.chi, tab.size() * sizeof(T4));
+  h->atom_alloc = false;
+  if (rcc) return rcc;
   HIP_OK(hcopy(h, ps.chi, tab.data(), tab.size() * sizeof(T4), hipMemcpyHostToDevice));
   ps.ready = true;
   return RISM_OK;
@@ -1117,7 +1161,7 @@ int create_impl(const rism_grid *grid, const rism_solvent *sv, const rism_solute
   HIP_OK(hipSetDevice(device));
   {
     std::lock_guard<std::mutex> lk(g_device_mu);
-    for (size_t i = 0; i < g_spare.size(); ++i)
+    for (size_t i = 0; i < g_spare.size() && !h->borrowed_ctx; ++i)
       if (g_spare[i].device == device) {
         h->stream = g_spare[i].stream;
         h->ctl_host = g_spare[i].ctl_host;
@@ -1171,13 +1215,17 @@ int create_impl(const rism_grid *grid, const rism_solvent *sv, const rism_solute
   if ((rc = dalloc_t(h, &h->ulj, VN)) || (rc = dalloc_t(h, &h->sel, 2 * VN)) ||
       (rc = dalloc_t(h, &h->de, h->N)) || (rc = dalloc_t(h, &h->fr, h->N)) ||
       (rc = dalloc_t(h, &h->sr, h->N)) || (rc = dalloc_t(h, &h->fkh, nh)) ||
-      (rc = dalloc_t(h, &h->phase, (size_t)std::max(1, h->natu) * (h->nx + h->ny + h->nz))) ||
       (rc = dalloc_t(h, &h->ts, VN)) || (rc = dalloc_t(h, &h->gfield, VN)) || (rc = dalloc_t(h, &h->ctl, 1)) ||
       (rc = dalloc_t(h, &h->rms_hist, (size_t)(o->maxstep + 1) * (o->chargeup + 1) + 4)) ||
-      (rc = dalloc_t(h, &h->atoms, std::max(1, h->natu))) || (rc = dalloc_t(h, &h->lj, (size_t)h->V * std::max(1, h->natu))) ||
-      (rc = dalloc_t(h, &h->sigh, (size_t)h->V * std::max(1, h->natu))) || (rc = dalloc_t(h, &h->d_qv, h->V)) ||
-      (rc = dalloc_t(h, &h->d_rho, h->V)))
+      (rc = dalloc_t(h, &h->d_qv, h->V)) || (rc = dalloc_t(h, &h->d_rho, h->V)))
     return rc;
+  h->atom_alloc = true;  // solute-sized: never from a batch arena
+  rc = dalloc_t(h, &h->phase, (size_t)std::max(1, h->natu) * (h->nx + h->ny + h->nz));
+  if (!rc) rc = dalloc_t(h, &h->atoms, std::max(1, h->natu));
+  if (!rc) rc = dalloc_t(h, &h->lj, (size_t)h->V * std::max(1, h->natu));
+  if (!rc) rc = dalloc_t(h, &h->sigh, (size_t)h->V * std::max(1, h->natu));
+  h->atom_alloc = false;
+  if (rc) return rc;
   h->seq = h->sel + VN;  // the two SE atom sums share one allocation
   std::memset(h->ctl_host, 0, sizeof(Ctl));
   if ((rc = alloc_precision<double>(h))) return rc;
@@ -1251,10 +1299,14 @@ int rism::set_solute(rism_handle *h, const rism_solute *su) {
     h->phase = nullptr;
     h->natu_cap = 0;
     int rc = 0;
+    h->atom_alloc = true;
     if ((rc = dalloc_t(h, &h->atoms, cap)) || (rc = dalloc_t(h, &h->lj, (size_t)h->V * cap)) ||
         (rc = dalloc_t(h, &h->sigh, (size_t)h->V * cap)) ||
-        (rc = dalloc_t(h, &h->phase, (size_t)cap * (h->nx + h->ny + h->nz))))
+        (rc = dalloc_t(h, &h->phase, (size_t)cap * (h->nx + h->ny + h->nz)))) {
+      h->atom_alloc = false;
       return rc;
+    }
+    h->atom_alloc = false;
     h->natu_cap = cap;
   }
   h->natu = nu;
@@ -1265,6 +1317,288 @@ int rism::set_solute(rism_handle *h, const rism_solute *su) {
   if (int rc = upload_solute(h)) return rc;
   h->potentials = h->solved = h->evaluated = false;
   return RISM_OK;
+}
+
+// ---------------------------------------------------------------------
+// Batched solves: many solutes of one grid class (same grid, box, solvent
+// and options) in one launch per kernel.  Every member handle carves its
+// grid-sized buffers from its slice of one arena, in the same order, so
+// each field of solute s lies s * stride bytes after solute 0's and one
+// launch with blockIdx.y = s covers them all (IterArgs::astride).  Each
+// solute keeps its own control block, which gates its workgroups: that is
+// its convergence mask.  A chunk of iterations (in the mixed path both
+// the FP32 chain and the FP64 refresh chain, each running only for the
+// solutes whose step it is) is captured once as a HIP graph and replayed
+// until every solute has stopped.
+namespace {
+
+int batch_iteration(rism_handle *h0, int S, long stride) {
+  const bool mixed = h0->opts.precision == RISM_PREC_MIXED;
+  if (mixed) {
+    {  // FP64 refresh chain: the solutes whose refresh step this is
+      IterArgs<double, double> a = make_args<double>(h0, true);
+      const LaunchGeom &g = h0->ps[0].geom;
+      a.gate = 2;
+      a.use_fk = 1;
+      a.astride = stride;
+      a.nsol = S;
+      HIP_OK(launch_mx_closure_fx<double>(a, g, h0->hm, h0->stream));
+      HIP_OK(launch_fft_y<double, double>(a, g, false, h0->stream));
+      HIP_OK(launch_fft_z_oz<double, double>(a, g, h0->stream));
+      HIP_OK(launch_fft_y<double, double>(a, g, true, h0->stream));
+      HIP_OK(launch_mx_resid<double>(a, g, h0->hm, h0->stream));
+      HIP_OK(launch_anderson_solve(h0->ctl, h0->partials, g.xblocks, a.nv, h0->hm, h0->rms_hist, h0->stream, 2,
+                                   stride, S));
+    }
+    IterArgs<float, double> a = make_args<float>(h0);
+    const LaunchGeom &g = h0->ps[1].geom;
+    a.gate = 1;
+    a.use_fk = 0;
+    a.astride = stride;
+    a.nsol = S;
+    HIP_OK(launch_mx_closure_fx<float>(a, g, h0->hm, h0->stream));
+    HIP_OK(launch_fft_y<float, double>(a, g, false, h0->stream));
+    HIP_OK(launch_fft_z_oz<float, double>(a, g, h0->stream));
+    HIP_OK(launch_fft_y<float, double>(a, g, true, h0->stream));
+    HIP_OK(launch_mx_resid<float>(a, g, h0->hm, h0->stream));
+    HIP_OK(launch_anderson_solve(h0->ctl, h0->partials, g.xblocks, a.nv, h0->hm, h0->rms_hist, h0->stream, 1,
+                                 stride, S));
+    return RISM_OK;
+  }
+  IterArgs<double, double> a = make_args<double>(h0);
+  const LaunchGeom &g = h0->ps[0].geom;
+  a.astride = stride;
+  a.nsol = S;
+  HIP_OK(launch_closure_fx<double, double>(a, g, true, h0->hm, h0->stream));
+  HIP_OK(launch_fft_y<double, double>(a, g, false, h0->stream));
+  HIP_OK(launch_fft_z_oz<double, double>(a, g, h0->stream));
+  HIP_OK(launch_fft_y<double, double>(a, g, true, h0->stream));
+  HIP_OK(launch_fx_resid<double, double>(a, g, h0->hm, h0->stream));
+  HIP_OK(launch_anderson_solve(h0->ctl, h0->partials, g.xblocks, a.nv, h0->hm, h0->rms_hist, h0->stream, 0, stride,
+                               S));
+  return RISM_OK;
+}
+
+// the per-solute start of a solve (solve_impl's set-up, cu = 0, no restart
+// file) on the batch's stream
+int batch_member_start(rism_handle *h, hipStream_t st) {
+  const int V = h->V, H = h->opts.nhist;
+  const size_t VN = (size_t)V * h->N;
+  const bool mixed = h->opts.precision == RISM_PREC_MIXED;
+  hipStream_t own = h->stream;
+  h->stream = st;  // every launch of the batch is ordered on one stream
+  h->evaluated = false;
+  h->mx_live = false;
+  h->cf = 1.0;
+  if (mixed) {
+    HIP_OK(hipMemsetAsync(h->mhx, 0, (size_t)H * VN * sizeof(float), st));
+    HIP_OK(hipMemsetAsync(h->mring, 0, (size_t)(H + 1) * VN * sizeof(float), st));
+  } else {
+    HIP_OK(hipMemsetAsync(h->hist_t, 0, (size_t)H * VN * h->qsz, st));
+    HIP_OK(hipMemsetAsync(h->hist_r, 0, (size_t)H * VN * h->qsz, st));
+    HIP_OK(hipMemsetAsync(h->res, 0, VN * h->qsz, st));
+  }
+  HIP_OK(hipMemsetAsync(h->ts, 0, VN * sizeof(double), st));  // initialize_tuv.cu: t = q_v fr
+  int rc = stage_potentials(h);
+  if (!rc && mixed) {
+    rc = mx_convert(h, 0);
+    h->mx_live = true;
+  }
+  Ctl &c = *h->ctl_host;
+  std::memset(&c, 0, sizeof c);
+  c.count = h->opts.count;
+  c.nhist = H;
+  c.maxstep = h->opts.maxstep;
+  c.tol = h->opts.tol;
+  c.m = h->opts.m;
+  c.mp = h->opts.mp;
+  c.npts = (double)VN;
+  c.status = -1;
+  c.run_iter = 0;
+  c.upd_iter = -1;
+  c.refresh_iter = mixed ? 0 : -1;
+  if (mixed) {
+    c.ring = 1;
+    c.rcur = H;
+    for (int k = 0; k < H; ++k) c.rmap[k] = k;
+  }
+  if (!rc && hipMemcpyAsync(h->ctl, h->ctl_host, sizeof(Ctl), hipMemcpyHostToDevice, st) != hipSuccess) {
+    set_error("batch: control block upload failed");
+    rc = RISM_ERR_HIP;
+  }
+  h->stream = own;
+  return rc;
+}
+
+}  // namespace
+
+// S solutes on one grid: create, potentials, one batched solve, thermo.
+int rism::batch_solve_class(const rism_grid *grid, const rism_solvent *sv, const rism_solute *const *sus, int S,
+                            const rism_opts *o, int device, int *status, int *steps, double *rms, rism_xmu *xmu) {
+  for (int s = 0; s < S; ++s) status[s] = RISM_ERR_STATE;
+  if (S <= 0) return RISM_OK;
+  if (o->chargeup != 0 || o->ksave != 0) {
+    set_error("batched solves need chargeup = 0 and ksave = 0");
+    for (int s = 0; s < S; ++s) status[s] = RISM_ERR_ARG;
+    return RISM_ERR_ARG;
+  }
+  const bool trace = env_long("RISM_TRACE", 0) > 0;
+  const double tb0 = now_s();
+  // 1. the slice size: one member created normally, its grid-sized buffers counted
+  size_t stride = 0;
+  {
+    rism_handle *m = new rism_handle();
+    m->arena_measure = true;
+    int rc = create_impl(grid, sv, sus[0], o, device, m);
+    stride = m->arena_need;
+    const std::string err = rc ? std::string(rism::last_error()) : std::string();
+    rism_destroy(m);
+    if (rc) {
+      set_error(err);
+      for (int s = 0; s < S; ++s) status[s] = rc;
+      return rc;
+    }
+  }
+  HIP_OK(hipSetDevice(device));
+  char *arena = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_device_mu);
+    hipError_t e = hipMalloc((void **)&arena, stride * (size_t)S);
+    if (e != hipSuccess && !g_cache.empty()) {
+      (void)hipGetLastError();
+      cache_flush_locked(device);
+      e = hipMalloc((void **)&arena, stride * (size_t)S);
+    }
+    if (e != hipSuccess) {
+      set_error("batch arena of " + std::to_string(stride * (size_t)S) + " bytes: " + hipGetErrorString(e));
+      for (int s = 0; s < S; ++s) status[s] = RISM_ERR_NOMEM;
+      return RISM_ERR_NOMEM;
+    }
+  }
+  std::vector<rism_handle *> hs(S, nullptr);
+  int rc = RISM_OK;
+  // one pinned block holds the members' control blocks (the first member's is its own)
+  Ctl *pinned = nullptr;
+  if (S > 1 && hipHostMalloc((void **)&pinned, sizeof(Ctl) * (size_t)S, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    pinned = nullptr;
+  }
+  for (int s = 0; s < S && rc == RISM_OK; ++s) {
+    rism_handle *h = new rism_handle();
+    h->arena = arena + (size_t)s * stride;
+    h->arena_cap = stride;
+    h->tmpl = s > 0 ? hs[0] : nullptr;
+    if (s > 0 && pinned) {  // the batch's stream and a slot of the pinned block
+      h->stream = hs[0]->stream;
+      h->ctl_host = pinned + s;
+      h->borrowed_ctx = true;
+    }
+    rc = create_impl(grid, sv, sus[s], o, device, h);
+    if (rc == RISM_OK && h->arena_used != stride) {
+      set_error("batch member's buffers differ from the measured slice");
+      rc = RISM_ERR_STATE;
+    }
+    h->arena = nullptr;  // later buffers (adjoint, scratch) are the handle's own
+    hs[s] = h;
+  }
+  rism_handle *h0 = hs[0];
+  hipStream_t st = h0 ? h0->stream : nullptr;
+  const bool mixed = o->precision == RISM_PREC_MIXED;
+  const double tb1 = now_s();
+  // 2. potentials, then every member's start of the solve on the batch's stream
+  for (int s = 0; s < S && rc == RISM_OK; ++s) rc = rism_build_potentials(hs[s]);
+  const double tb2 = now_s();
+  if (rc == RISM_OK && env_long("RISM_AUTOTUNE", 0) > 0 && !h0->tuned) rc = rism_autotune(h0, (int)env_long("RISM_AUTOTUNE", 0), nullptr);
+  if (rc == RISM_OK)  // every member launches with solute 0's tile shapes
+    for (int s = 1; s < S; ++s) {
+      hs[s]->ps[0].geom = h0->ps[0].geom;
+      if (mixed) hs[s]->ps[1].geom = h0->ps[1].geom;
+    }
+  for (int s = 0; s < S && rc == RISM_OK; ++s) rc = batch_member_start(hs[s], st);
+  if (rc == RISM_OK && hipStreamSynchronize(st) != hipSuccess) {
+    set_error("batch: start of the solves failed");
+    rc = RISM_ERR_HIP;
+  }
+  // 3. the batched solve: chunks of iterations as one graph, replayed until every solute stopped
+  const double t0 = now_s();
+  const int chunk = 16;
+  hipGraphExec_t ex = nullptr;
+  if (rc == RISM_OK) {
+    const char *genv = std::getenv("RISM_GRAPHS");
+    bool graphs = !(genv && genv[0] == '0');
+    if (graphs) {
+      std::unique_lock<std::mutex> lk(g_device_mu);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
+      int r2 = RISM_OK;
+      if (e == hipSuccess) {
+        for (int k = 0; k < chunk && r2 == RISM_OK; ++k) r2 = batch_iteration(h0, S, (long)stride);
+        e = hipStreamEndCapture(st, &g);
+      }
+      if (e == hipSuccess && r2 == RISM_OK) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+      if (g) (void)hipGraphDestroy(g);
+      if (e != hipSuccess || r2 != RISM_OK) {
+        (void)hipGetLastError();
+        if (ex) (void)hipGraphExecDestroy(ex);
+        ex = nullptr;
+      }
+    }
+    int live = S;
+    long guard = 0;
+    while (live > 0 && rc == RISM_OK) {
+      if (ex) {
+        if (hipGraphLaunch(ex, st) != hipSuccess) { set_error("batch: graph launch failed"); rc = RISM_ERR_HIP; break; }
+      } else {
+        for (int k = 0; k < chunk && rc == RISM_OK; ++k) rc = batch_iteration(h0, S, (long)stride);
+      }
+      for (int s = 0; s < S && rc == RISM_OK; ++s)
+        if (hipMemcpyAsync(hs[s]->ctl_host, hs[s]->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st) != hipSuccess) {
+          set_error("batch: control block read failed");
+          rc = RISM_ERR_HIP;
+        }
+      if (rc == RISM_OK && hipStreamSynchronize(st) != hipSuccess) { set_error("batch: solve failed"); rc = RISM_ERR_HIP; }
+      live = 0;
+      for (int s = 0; s < S; ++s) live += hs[s]->ctl_host->status < 0;
+      if (++guard > (long)o->maxstep / chunk + 4) break;  // every solute stops within maxstep steps
+    }
+  }
+  if (ex) (void)hipGraphExecDestroy(ex);
+  const double t_solve = now_s() - t0;
+  const double tb3 = now_s();
+  // 4. per solute: a step-limit stop applies its pending update, then thermo
+  for (int s = 0; s < S; ++s) {
+    rism_handle *h = hs[s];
+    if (!h || rc != RISM_OK) { status[s] = rc ? rc : RISM_ERR_STATE; continue; }
+    Ctl &c = *h->ctl_host;
+    int r = RISM_OK;
+    if (c.status == RISM_SOLVE_MAXSTEP && c.upd_iter >= 0 && c.upd_iter == c.cur) {
+      c.run_iter = -1;
+      r = put_ctl(h);
+      if (!r) r = launch_update_only(h);
+    }
+    h->steps = c.total;
+    h->rms = c.rms;
+    h->solved = true;
+    h->t_solve = t_solve;
+    if (steps) steps[s] = c.total;
+    if (rms) rms[s] = c.rms;
+    const int st_s = c.status < 0 ? RISM_ERR_STATE : c.status;
+    if (!r && xmu && st_s >= 0) r = rism_thermo(h, &xmu[s]);
+    status[s] = r < 0 ? r : st_s;
+  }
+  const double tb4 = now_s();
+  for (int s = S - 1; s >= 0; --s)  // the first member, whose tables and stream the others share, last
+    if (hs[s]) rism_destroy(hs[s]);
+  if (pinned) (void)hipHostFree(pinned);
+  {
+    std::lock_guard<std::mutex> lk(g_device_mu);
+    (void)hipFree(arena);
+  }
+  if (trace)
+    std::fprintf(stderr, "rism batch %d x %dx%dx%d: create %.3f s, potentials %.3f, start %.3f, solve %.3f, thermo %.3f, "
+                 "release %.3f\n", S, grid->n[0], grid->n[1], grid->n[2], tb1 - tb0, tb2 - tb1, t0 - tb2, tb3 - t0,
+                 tb4 - tb3, now_s() - tb4);
+  return rc;
 }
 
 // ---------------------------------------------------------------------
@@ -2140,7 +2474,9 @@ void rism_destroy(rism_handle *h) {
   drop_graphs(h);
   for (auto &b : h->bufs) release_locked(h->device, b);
   for (auto &e : h->ev_pool) (void)hipEventDestroy(e);
-  if (h->stream && h->ctl_host && g_spare.size() < kMaxSpare) {
+  if (h->borrowed_ctx) {
+    // the batch's stream and pinned block: its owner releases them
+  } else if (h->stream && h->ctl_host && g_spare.size() < kMaxSpare) {
     g_spare.push_back({h->device, h->stream, h->ctl_host});
   } else {
     if (h->ctl_host) (void)hipHostFree(h->ctl_host);

@@ -284,12 +284,17 @@ long rism_release_cache(void);
 
 /* ---- many solutes (no counterpart in the reference: one solve per run) ---- */
 /* n independent solves sharing one solvent, each solute on its own grid
-   (grids[i], solutes[i]), run concurrently on one device: nworkers host
-   threads each own a handle and its HIP stream and take solutes from a
-   shared queue (rism_create, rism_build_potentials, rism_solve, rism_thermo,
-   rism_destroy).  Per solute: status[i] (RISM_SOLVE_* or RISM_ERR_*),
-   steps[i], rms[i], xmu[i] (any of steps/rms/xmu may be NULL); seconds
-   receives the wall time.  Returns RISM_OK once every solve has run. */
+   (grids[i], solutes[i]), on one device.  The solutes are grouped by grid
+   class (grid and box) and each class runs in batched launches: one launch
+   per kernel covers every solute of the class (up to RISM_BATCH_MAX = 128
+   per launch), each solute's control block gating its own workgroups (its
+   convergence mask), the iteration chunks replayed as one HIP graph;
+   nworkers host threads take classes from a shared queue.  With
+   RISM_BATCH_LAUNCH=0 (or chargeup / ksave set) nworkers threads instead
+   each own a handle and its HIP stream and take solutes one at a time.
+   Per solute: status[i] (RISM_SOLVE_* or RISM_ERR_*), steps[i], rms[i],
+   xmu[i] (any of steps/rms/xmu may be NULL); seconds receives the wall
+   time.  Returns RISM_OK once every solve has run. */
 int rism_batch_solve(int n, const rism_grid *grids, const rism_solvent *solvent,
                      const rism_solute *solutes, const rism_opts *opts, int device,
                      int nworkers, int *status, int *steps, double *rms, rism_xmu *xmu,
