@@ -261,6 +261,8 @@ def main():
     ap.add_argument("--cpu-sample", action="store_true",
                     help="CPU baseline on a bounded sample of the job (scaled) instead of the whole job")
     ap.add_argument("--mixed-only", action="store_true", help="time only the mixed path")
+    ap.add_argument("--fp64-only", action="store_true",
+                    help="time only the FP64 path (profiling its kernels on their own; no headline line)")
     ap.add_argument("--workload", choices=("256", "512", "batch"), default="256",
                     help="256 (default): the headline 256^3 job; 512: the 512^3 / 5000-atom electrolyte "
                          "job (memory, times, adjoint forces); batch: 256 small solutes (solutes/s)")
@@ -290,6 +292,13 @@ def main():
     V = solvent.nsite
     P = (V + 1) // 2
     N = int(np.prod(w.n))
+    if args.fp64_only:
+        fp64 = time_path(rism3d, w, solvent, rism3d.FP64, args.steps, args.warmup)
+        kb = kernel_bytes(V, P, w.nhist, False)
+        print(json.dumps({"workload": w.name, "path": "fp64", "s": round(fp64["s"], 4),
+                          "iterations": fp64["iterations"], "copy_GBps": round(copy_gbps, 1),
+                          "kernels_fp64": per_kernel_table(fp64["kernels"], kb, N)}))
+        return
     mixed = time_path(rism3d, w, solvent, rism3d.MIXED, args.steps, args.warmup)
     fp64 = None if args.mixed_only else time_path(rism3d, w, solvent, rism3d.FP64, args.steps, args.warmup)
     # roofline: dominant iteration kernel of the mixed path
