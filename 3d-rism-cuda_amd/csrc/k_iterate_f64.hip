@@ -5,8 +5,8 @@
 namespace rism {
 
 hipError_t launch_anderson_solve(Ctl *ctl, const double *partials, int nblk, int nv, int hm,
-                                 double *rms_hist, hipStream_t st, int gate, long astride, int nsol) {
-  return launch_anderson_solve_impl(ctl, partials, nblk, nv, hm, rms_hist, st, gate, astride, nsol);
+                                 double *rms_hist, hipStream_t st, int gate, long astride, int nsol, int *rlist) {
+  return launch_anderson_solve_impl(ctl, partials, nblk, nv, hm, rms_hist, st, gate, astride, nsol, rlist);
 }
 
 template hipError_t launch_closure_fx<double, double>(const IterArgs<double, double> &, const LaunchGeom &, bool, int, hipStream_t);

@@ -217,10 +217,12 @@ __global__ __launch_bounds__(kThreads) void k_closure_fx(IterArgs<R, S> a) {
 template <typename R, int EPT, bool INV, int NL = 0>
 __global__ __launch_bounds__(kThreads) void k_fft_y(typename C2<R>::T *Z, const Ctl *ctl, int nx,
                                                      int ny_rt, int nz, long N, int T_rt, int P, AxisPlan py,
-                                                     int gate, long astride) {
+                                                     int gate, long astride, const int *rlist) {
   // NL > 0: compile-time line length (power of two) with T = EPT * kThreads / NL
   using T2 = typename C2<R>::T;
-  const long so = (long)blockIdx.y * astride;  // this launch's solute (batched launches)
+  const int sid = batch_solute(gate, rlist);  // this launch's solute (batched launches)
+  if (sid < 0) return;
+  const long so = (long)sid * astride;
   Z = soff(Z, so);
   if (ctl) ctl = soff(ctl, so);
   if (ctl && !gate_open(ctl, gate)) return;
@@ -283,7 +285,9 @@ __global__ __launch_bounds__(kThreads) void k_fft_y(typename C2<R>::T *Z, const 
 constexpr int kXcds = 8;
 template <typename R, typename S, int EPT, int VC, int NL = 0>
 __global__ __launch_bounds__(kThreads) void k_fft_z_oz(IterArgs<R, S> a) {
-  const long so = (long)blockIdx.y * a.astride;  // this launch's solute (batched launches)
+  const int sid = batch_solute(a.gate, a.rlist);  // this launch's solute (batched launches)
+  if (sid < 0) return;
+  const long so = (long)sid * a.astride;
   // NL > 0: compile-time z length (power of two), one pack (VC == 2), T = EPT * kThreads / (2 NL)
   using T2 = typename C2<R>::T;
   using T4 = typename R4T<R>::T;
@@ -497,7 +501,8 @@ static __device__ void leg_solve_dev(double *A, double *x, double *b, int n) {
 // One workgroup: reduce the per-workgroup partials (fixed order), rms,
 // convergence test, Picard / Anderson coefficients (leg), control block.
 template <int HM>
-static __device__ void anderson_solve_block(Ctl *ctl, const double *partials, int nblk, double *rms_hist) {
+static __device__ void anderson_solve_block(Ctl *ctl, const double *partials, int nblk, double *rms_hist,
+                                            int *rlist, int sid) {
   constexpr int NV = 1 + HM + HM * (HM + 1) / 2;
   const int hm = HM;
   __shared__ double tot[NV];
@@ -569,10 +574,19 @@ static __device__ void anderson_solve_block(Ctl *ctl, const double *partials, in
         ctl->best_rms = rms;
         ctl->best_at = ctl->total;
       }
+      // (a refresh step itself never asks for the next one: its rms is the
+      // new reference and since_ref is 0)
+      if (ctl->step < ctl->maxstep &&
+          (rms < 0.1 * ctl->ref_rms || (ctl->since_ref >= 24 && ctl->total - ctl->best_at >= 24))) {
+        bool filed = true;
+        if (rlist) {  // batched launches: a place on the refresh list, else again next step
+          const int slot = atomicAdd(rlist, 1);
+          filed = slot < rlist[1];
+          if (filed) rlist[2 + slot] = sid;
+        }
+        if (filed) ctl->refresh_iter = iter + 1;
+      }
     }
-    if (ctl->step < ctl->maxstep &&
-        (rms < 0.1 * ctl->ref_rms || (ctl->since_ref >= 24 && ctl->total - ctl->best_at >= 24)))
-      ctl->refresh_iter = iter + 1;
   }
   if (ctl->count > 0) {  // Picard warm-up, anderson.cu:43-49
     ctl->count -= 1;
@@ -618,11 +632,15 @@ static __device__ void anderson_solve_block(Ctl *ctl, const double *partials, in
 
 template <int HM>
 __global__ __launch_bounds__(kThreads) void k_anderson_solve(Ctl *ctl, const double *partials, int nblk,
-                                                             double *rms_hist, int gate, long astride) {
-  const long so = (long)blockIdx.y * astride;  // this launch's solute (batched launches)
+                                                             double *rms_hist, int gate, long astride,
+                                                             int *rlist) {
+  const int sid = batch_solute(gate, rlist);  // this launch's solute (batched launches)
+  if (sid < 0) return;
+  const long so = (long)sid * astride;
   ctl = soff(ctl, so);
   if (!gate_open(ctl, gate)) return;
-  anderson_solve_block<HM>(ctl, soff(partials, so), nblk, soff(rms_hist, so));
+  // the FP32 chain's solve files refreshes on the list; the refresh chain's reads it
+  anderson_solve_block<HM>(ctl, soff(partials, so), nblk, soff(rms_hist, so), gate == 1 ? rlist : nullptr, sid);
 }
 
 // ---------------------------------------------------------------------
@@ -940,15 +958,15 @@ hipError_t launch_fft_y(const IterArgs<R, S> &a, const LaunchGeom &g, bool inver
   if (inverse) {
     RISM_CT_SWITCH(nl, g.ept_y,
                    RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, true>), grid, block, g.lds_y, st, a.Z,
-                                                               a.ctl, a.nx, a.ny, a.nz, a.N, g.Ty, a.P, a.py, a.gate, a.astride)),
+                                                               a.ctl, a.nx, a.ny, a.nz, a.N, g.Ty, a.P, a.py, a.gate, a.astride, a.rlist)),
                    hipLaunchKernelGGL((k_fft_y<R, E, true, NLV>), grid, block, g.lds_y, st, a.Z, a.ctl, a.nx,
-                                      a.ny, a.nz, a.N, g.Ty, a.P, a.py, a.gate, a.astride))
+                                      a.ny, a.nz, a.N, g.Ty, a.P, a.py, a.gate, a.astride, a.rlist))
   } else {
     RISM_CT_SWITCH(nl, g.ept_y,
                    RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, false>), grid, block, g.lds_y, st, a.Z,
-                                                               a.ctl, a.nx, a.ny, a.nz, a.N, g.Ty, a.P, a.py, a.gate, a.astride)),
+                                                               a.ctl, a.nx, a.ny, a.nz, a.N, g.Ty, a.P, a.py, a.gate, a.astride, a.rlist)),
                    hipLaunchKernelGGL((k_fft_y<R, E, false, NLV>), grid, block, g.lds_y, st, a.Z, a.ctl, a.nx,
-                                      a.ny, a.nz, a.N, g.Ty, a.P, a.py, a.gate, a.astride))
+                                      a.ny, a.nz, a.N, g.Ty, a.P, a.py, a.gate, a.astride, a.rlist))
   }
   return hipGetLastError();
 }
@@ -980,12 +998,13 @@ hipError_t launch_fx_resid(const IterArgs<R, S> &a, const LaunchGeom &g, int hm,
 }
 
 inline hipError_t launch_anderson_solve_impl(Ctl *ctl, const double *partials, int nblk, int nv, int hm,
-                                 double *rms_hist, hipStream_t st, int gate, long astride, int nsol) {
+                                 double *rms_hist, hipStream_t st, int gate, long astride, int nsol,
+                                 int *rlist) {
   const dim3 grid(1, nsol > 1 ? nsol : 1);
   switch (hm) {
-    case 2: hipLaunchKernelGGL(k_anderson_solve<2>, grid, dim3(kThreads), 0, st, ctl, partials, nblk, rms_hist, gate, astride); break;
-    case 5: hipLaunchKernelGGL(k_anderson_solve<5>, grid, dim3(kThreads), 0, st, ctl, partials, nblk, rms_hist, gate, astride); break;
-    case 10: hipLaunchKernelGGL(k_anderson_solve<10>, grid, dim3(kThreads), 0, st, ctl, partials, nblk, rms_hist, gate, astride); break;
+    case 2: hipLaunchKernelGGL(k_anderson_solve<2>, grid, dim3(kThreads), 0, st, ctl, partials, nblk, rms_hist, gate, astride, rlist); break;
+    case 5: hipLaunchKernelGGL(k_anderson_solve<5>, grid, dim3(kThreads), 0, st, ctl, partials, nblk, rms_hist, gate, astride, rlist); break;
+    case 10: hipLaunchKernelGGL(k_anderson_solve<10>, grid, dim3(kThreads), 0, st, ctl, partials, nblk, rms_hist, gate, astride, rlist); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -1006,11 +1025,11 @@ hipError_t launch_fft3d(typename C2<R>::T *data, const AxisPlan &px, const AxisP
   const int ez = (g.Tz * nz <= 4 * kThreads) ? 4 : (g.Tz * nz <= 8 * kThreads) ? 8 : (g.Tz * nz <= 16 * kThreads) ? 16 : 32;
   if (dir < 0) {
     RISM_EPTX_SWITCH(g.ept_x, hipLaunchKernelGGL((k_fft_x_pure<R, E, false>), gx, block, ldsx, st, data, nx, L, px, (R)dv))
-    RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, false>), gy, block, ldsy, st, data, (const Ctl *)nullptr, nx, ny, nz, N, g.Ty, 1, py, 0, 0L))
+    RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, false>), gy, block, ldsy, st, data, (const Ctl *)nullptr, nx, ny, nz, N, g.Ty, 1, py, 0, 0L, (const int *)nullptr))
     RISM_EPT_SWITCH(ez, hipLaunchKernelGGL((k_fft_z_pure<R, E, false>), gz, block, ldsz, st, data, nx, ny, nz, g.Tz, pz))
   } else {
     RISM_EPT_SWITCH(ez, hipLaunchKernelGGL((k_fft_z_pure<R, E, true>), gz, block, ldsz, st, data, nx, ny, nz, g.Tz, pz))
-    RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, true>), gy, block, ldsy, st, data, (const Ctl *)nullptr, nx, ny, nz, N, g.Ty, 1, py, 0, 0L))
+    RISM_EPT_SWITCH(g.ept_y, hipLaunchKernelGGL((k_fft_y<R, E, true>), gy, block, ldsy, st, data, (const Ctl *)nullptr, nx, ny, nz, N, g.Ty, 1, py, 0, 0L, (const int *)nullptr))
     RISM_EPTX_SWITCH(g.ept_x, hipLaunchKernelGGL((k_fft_x_pure<R, E, true>), gx, block, ldsx, st, data, nx, L, px, (R)ivol))
   }
   return hipGetLastError();

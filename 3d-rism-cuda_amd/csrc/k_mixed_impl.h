@@ -49,7 +49,7 @@ template <typename R, int EPT, int VC, int HM, bool UPD>
 __device__ __forceinline__ void mx_points(const IterArgs<R, double> &a, typename C2<R>::T *lds, bool do_run,
                                           const double *th, const long *toff, const long *roff, long rcur_off,
                                           double mix, int w, int nf) {
-  const long so = (long)blockIdx.y * a.astride;  // this launch's solute (batched launches)
+  const long so = (long)batch_solute(a.gate, a.rlist) * a.astride;  // this launch's solute (batched launches)
   using T2 = typename C2<R>::T;
   constexpr bool F64 = sizeof(R) == sizeof(double);  // the refresh chain
   constexpr int CH = EPT < RISM_CLOS_CH ? EPT : RISM_CLOS_CH;
@@ -159,7 +159,12 @@ __device__ __forceinline__ void mx_points(const IterArgs<R, double> &a, typename
 
 template <typename R, int EPT, int VC, int HM>
 __global__ __launch_bounds__(kThreads) void k_mx_closure_fx(IterArgs<R, double> a) {
-  const long so = (long)blockIdx.y * a.astride;  // this launch's solute (batched launches)
+  // the refresh list of the next step starts empty (the refresh chain of
+  // this step, launched before this pass, has read it)
+  if (a.gate == 1 && a.rlist && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) a.rlist[0] = 0;
+  const int sid = batch_solute(a.gate, a.rlist);  // this launch's solute (batched launches)
+  if (sid < 0) return;
+  const long so = (long)sid * a.astride;
   using T2 = typename C2<R>::T;
   const Ctl *ctl = soff(a.ctl, so);
   const int iter = ctl->cur;
@@ -228,7 +233,9 @@ __global__ __launch_bounds__(kThreads) void k_mx_closure_fx(IterArgs<R, double> 
 // ---------------------------------------------------------------------
 template <typename R, int EPT, int HM, int VC>
 __global__ __launch_bounds__(kThreads) void k_mx_resid(IterArgs<R, double> a) {
-  const long so = (long)blockIdx.y * a.astride;  // this launch's solute (batched launches)
+  const int sid = batch_solute(a.gate, a.rlist);  // this launch's solute (batched launches)
+  if (sid < 0) return;
+  const long so = (long)sid * a.astride;
   using T2 = typename C2<R>::T;
   constexpr bool F64 = sizeof(R) == sizeof(double);
   constexpr int NV = 1 + HM + HM * (HM + 1) / 2;

@@ -92,7 +92,17 @@ __device__ __forceinline__ bool gate_open(const Ctl *c, int gate) {
   return c->run_iter == c->cur && chain_active(c, gate);
 }
 
-// a per-solute pointer of a batched launch (o = blockIdx.y * astride bytes)
+// The solute of a workgroup of a batched launch: blockIdx.y; in the refresh
+// chain's compacted launches (gate 2 with a refresh list
+// rl = {count, capacity, solute ...}) entry blockIdx.y of the list, and -1
+// (no solute: the workgroup exits) past its end.
+__device__ __forceinline__ int batch_solute(int gate, const int *rl) {
+  if (gate != 2 || !rl) return (int)blockIdx.y;
+  const int n = min(rl[0], rl[1]);
+  return (int)blockIdx.y < n ? rl[2 + blockIdx.y] : -1;
+}
+
+// a per-solute pointer of a batched launch (o = solute * astride bytes)
 template <typename P> __device__ __forceinline__ P soff(P p, long o) {
   return (P)((const char *)p + o);
 }
@@ -155,6 +165,13 @@ struct IterArgs {
   // the same for all and are read from solute 0.
   long astride;
   int nsol;
+  // Refresh list of a batched mixed solve, {count, capacity, solute ...}:
+  // the FP32 chain's solve kernel files the solutes whose next step is a
+  // refresh (up to capacity; the others try again at their next step), the
+  // refresh chain launches capacity rows of workgroups mapped through it
+  // (batch_solute), and the FP32 chain's closure pass empties it again.
+  // nullptr: blockIdx.y is the solute.
+  int *rlist;
   double *A;
   float *dl;
   float *rref;
@@ -186,7 +203,8 @@ hipError_t launch_fft_z_oz(const IterArgs<R, S> &a, const LaunchGeom &g, hipStre
 template <typename R, typename S>
 hipError_t launch_fx_resid(const IterArgs<R, S> &a, const LaunchGeom &g, int hm, hipStream_t st);
 hipError_t launch_anderson_solve(Ctl *ctl, const double *partials, int nblk, int nv, int hm,
-                                 double *rms_hist, hipStream_t st, int gate = 0, long astride = 0, int nsol = 1);
+                                 double *rms_hist, hipStream_t st, int gate = 0, long astride = 0, int nsol = 1,
+                                 int *rlist = nullptr);
 // mixed precision: closure / residual passes of the FP32 chain (R = float)
 // and of the FP64 refresh chain (R = double), k_mixed_f32.hip / k_mixed_f64.hip
 template <typename R>

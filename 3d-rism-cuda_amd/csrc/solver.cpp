@@ -1332,23 +1332,33 @@ int rism::set_solute(rism_handle *h, const rism_solute *su) {
 // until every solute has stopped.
 namespace {
 
-int batch_iteration(rism_handle *h0, int S, long stride) {
+// One step of every solute of a batch.  Mixed precision: the FP64 refresh
+// chain, then the FP32 chain.  With a refresh list (rlist, capacity cap)
+// the refresh chain launches cap rows of workgroups for the solutes on the
+// list, and the FP32 chain's solve files the next step's refreshes there;
+// at the first step (all_refresh: every solute refreshes) or without a list
+// it launches all S rows, each gated by its solute's control block.
+int batch_iteration(rism_handle *h0, int S, long stride, int *rlist = nullptr, int cap = 0,
+                    bool all_refresh = false) {
   const bool mixed = h0->opts.precision == RISM_PREC_MIXED;
   if (mixed) {
     {  // FP64 refresh chain: the solutes whose refresh step this is
       IterArgs<double, double> a = make_args<double>(h0, true);
       const LaunchGeom &g = h0->ps[0].geom;
+      const bool listed = rlist && !all_refresh;
+      const int rows = listed ? cap : S;
       a.gate = 2;
       a.use_fk = 1;
       a.astride = stride;
-      a.nsol = S;
+      a.nsol = rows;
+      a.rlist = listed ? rlist : nullptr;
       HIP_OK(launch_mx_closure_fx<double>(a, g, h0->hm, h0->stream));
       HIP_OK(launch_fft_y<double, double>(a, g, false, h0->stream));
       HIP_OK(launch_fft_z_oz<double, double>(a, g, h0->stream));
       HIP_OK(launch_fft_y<double, double>(a, g, true, h0->stream));
       HIP_OK(launch_mx_resid<double>(a, g, h0->hm, h0->stream));
       HIP_OK(launch_anderson_solve(h0->ctl, h0->partials, g.xblocks, a.nv, h0->hm, h0->rms_hist, h0->stream, 2,
-                                   stride, S));
+                                   stride, rows, a.rlist));
     }
     IterArgs<float, double> a = make_args<float>(h0);
     const LaunchGeom &g = h0->ps[1].geom;
@@ -1356,13 +1366,14 @@ int batch_iteration(rism_handle *h0, int S, long stride) {
     a.use_fk = 0;
     a.astride = stride;
     a.nsol = S;
+    a.rlist = rlist;  // emptied by the closure pass, filled by the solve
     HIP_OK(launch_mx_closure_fx<float>(a, g, h0->hm, h0->stream));
     HIP_OK(launch_fft_y<float, double>(a, g, false, h0->stream));
     HIP_OK(launch_fft_z_oz<float, double>(a, g, h0->stream));
     HIP_OK(launch_fft_y<float, double>(a, g, true, h0->stream));
     HIP_OK(launch_mx_resid<float>(a, g, h0->hm, h0->stream));
     HIP_OK(launch_anderson_solve(h0->ctl, h0->partials, g.xblocks, a.nv, h0->hm, h0->rms_hist, h0->stream, 1,
-                                 stride, S));
+                                 stride, S, rlist));
     return RISM_OK;
   }
   IterArgs<double, double> a = make_args<double>(h0);
@@ -1523,6 +1534,31 @@ int rism::batch_solve_class(const rism_grid *grid, const rism_solvent *sv, const
   const double t0 = now_s();
   const int chunk = 16;
   hipGraphExec_t ex = nullptr;
+  // mixed: the refresh list, {count, capacity, solute ...}; a solute
+  // refreshes about once in 30 steps, so capacity S/8 (RISM_BATCH_REFRESH_DIV)
+  // leaves room for bursts, and a solute that finds the list full refreshes
+  // a step later
+  int *rlist = nullptr;
+  int rcap = 0;
+  if (rc == RISM_OK && mixed && S > 1) {
+    const long div = std::max(1L, env_long("RISM_BATCH_REFRESH_DIV", 8));
+    rcap = (int)std::max(1L, ((long)S + div - 1) / div);
+    std::vector<int> init(S + 2, 0);
+    init[1] = rcap;
+    {
+      std::lock_guard<std::mutex> lk(g_device_mu);
+      if (hipMalloc((void **)&rlist, sizeof(int) * (S + 2)) != hipSuccess) {
+        (void)hipGetLastError();
+        rlist = nullptr;  // no list: every refresh launch covers all S solutes
+      }
+    }
+    if (rlist && hipMemcpy(rlist, init.data(), sizeof(int) * (S + 2), hipMemcpyHostToDevice) != hipSuccess) {
+      set_error("batch: refresh list upload failed");
+      rc = RISM_ERR_HIP;
+    }
+    // the first step is a refresh for every solute: all S rows
+    if (rc == RISM_OK) rc = batch_iteration(h0, S, (long)stride, rlist, rcap, true);
+  }
   if (rc == RISM_OK) {
     const char *genv = std::getenv("RISM_GRAPHS");
     bool graphs = !(genv && genv[0] == '0');
@@ -1532,7 +1568,7 @@ int rism::batch_solve_class(const rism_grid *grid, const rism_solvent *sv, const
       hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
       int r2 = RISM_OK;
       if (e == hipSuccess) {
-        for (int k = 0; k < chunk && r2 == RISM_OK; ++k) r2 = batch_iteration(h0, S, (long)stride);
+        for (int k = 0; k < chunk && r2 == RISM_OK; ++k) r2 = batch_iteration(h0, S, (long)stride, rlist, rcap);
         e = hipStreamEndCapture(st, &g);
       }
       if (e == hipSuccess && r2 == RISM_OK) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
@@ -1549,7 +1585,7 @@ int rism::batch_solve_class(const rism_grid *grid, const rism_solvent *sv, const
       if (ex) {
         if (hipGraphLaunch(ex, st) != hipSuccess) { set_error("batch: graph launch failed"); rc = RISM_ERR_HIP; break; }
       } else {
-        for (int k = 0; k < chunk && rc == RISM_OK; ++k) rc = batch_iteration(h0, S, (long)stride);
+        for (int k = 0; k < chunk && rc == RISM_OK; ++k) rc = batch_iteration(h0, S, (long)stride, rlist, rcap);
       }
       for (int s = 0; s < S && rc == RISM_OK; ++s)
         if (hipMemcpyAsync(hs[s]->ctl_host, hs[s]->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st) != hipSuccess) {
@@ -1563,6 +1599,11 @@ int rism::batch_solve_class(const rism_grid *grid, const rism_solvent *sv, const
     }
   }
   if (ex) (void)hipGraphExecDestroy(ex);
+  if (rlist) {
+    (void)hipStreamSynchronize(st);
+    std::lock_guard<std::mutex> lk(g_device_mu);
+    (void)hipFree(rlist);
+  }
   const double t_solve = now_s() - t0;
   const double tb3 = now_s();
   // 4. per solute: a step-limit stop applies its pending update, then thermo
