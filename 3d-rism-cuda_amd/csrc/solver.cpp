@@ -470,9 +470,10 @@ int choose_geometry(rism_handle *h, LaunchGeom &g, size_t rsz, const int *ovr = 
   if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device) != hipSuccess)
     lds_max = 65536;
   const long cap = std::min<long>(env_long("RISM_LDS_CAP", 65536), lds_max);
-  // residual pass: 2048 elements per workgroup in FP32, 1024 in FP64 (its
-  // registers then allow four waves per SIMD)
-  const long xe = env_long("RISM_X_ELEMS", rsz == sizeof(double) ? 1024 : 2048);
+  // residual pass: 1024 elements per workgroup (four x lines at 256: its
+  // registers then allow four waves per SIMD, and the FP32 pass holds all
+  // of a group's operands in flight during the transform)
+  const long xe = env_long("RISM_X_ELEMS", 1024);
   const long lsx = line_stride(nx), lsy = line_stride(ny), lsz = line_stride(nz);
   const long ce = env_long("RISM_C_ELEMS", 1024);
   g.Lx = largest_divisor_le((long)ny * nz,
