@@ -162,8 +162,21 @@ template <bool INV, typename T2> struct Dft<16, INV, T2> {
     Dft<8, INV, T2>::run(o);
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      T2 w = mk<T2>(cs[k], INV ? sn[k] : -sn[k]);
-      T2 t = (k == 0) ? o[0] : cmul(o[k], w);
+      // w16^k: k = 4 is a quarter turn, k = 2 and 6 are the eighth turns
+      // (a sum and a difference times 1/sqrt 2), the rest general
+      T2 t;
+      if (k == 0) {
+        t = o[0];
+      } else if (k == 4) {
+        t = rot90<INV>(o[4]);
+      } else if (k == 2 || k == 6) {
+        const R h = cs[2];
+        const T2 q = o[k];
+        if (k == 2) t = INV ? mk<T2>((q.x - q.y) * h, (q.x + q.y) * h) : mk<T2>((q.x + q.y) * h, (q.y - q.x) * h);
+        else t = INV ? mk<T2>(-(q.x + q.y) * h, (q.x - q.y) * h) : mk<T2>((q.y - q.x) * h, -(q.x + q.y) * h);
+      } else {
+        t = cmul(o[k], mk<T2>(cs[k], INV ? sn[k] : -sn[k]));
+      }
       v[k] = cadd(e[k], t);
       v[k + 8] = csub(e[k], t);
     }
@@ -214,10 +227,15 @@ __device__ __forceinline__ void fft_stage(T2 *lds, int nlines, int ls, int n, in
       Dft<F, INV, T2>::run(v[k]);
       const int o = q0[k] + s * (F * ppv[k]);
       lds[lbase[k] + lpad(o)] = v[k][0];
+      if (m == 1) {  // last stage: every twiddle is 1
 #pragma unroll
-      for (int b = 1; b < F; ++b) {
-        const T2 w = tw[ppv[k] * b * tstep];  // tw lives in LDS
-        lds[lbase[k] + lpad(o + s * b)] = INV ? cmulc(v[k][b], w) : cmul(v[k][b], w);
+        for (int b = 1; b < F; ++b) lds[lbase[k] + lpad(o + s * b)] = v[k][b];
+      } else {
+#pragma unroll
+        for (int b = 1; b < F; ++b) {
+          const T2 w = tw[ppv[k] * b * tstep];  // tw lives in LDS
+          lds[lbase[k] + lpad(o + s * b)] = INV ? cmulc(v[k][b], w) : cmul(v[k][b], w);
+        }
       }
     }
   }
@@ -275,54 +293,70 @@ constexpr int ct_radix(int N, int k) {  // radix of stage k
 }
 constexpr int ct_stages(int N) { return (ct_log2(N) + 3) / 4; }
 
+// With N, S and F powers of two, every padded LDS position of a butterfly
+// is its first one plus a compile-time offset: input a sits a*S*M after
+// input 0 and output b sits S*b after output 0, and
+//   lpad(i0 + d) = lpad(i0) + d + (d >> 4)
+// holds for those d (for d a multiple of 16 always; for S*M < 16 because
+// i0 < S*M and the sum stays inside one 16-element block; for the outputs
+// because o mod 16 < S when S*F >= 16, and o mod 16 + S*(F-1) < 16 when S*F
+// divides 16).  The index arithmetic is unsigned, so the divisions by
+// powers of two are shifts.
 template <int F, bool INV, int N, int S, int EPT, typename T2>
-__device__ __forceinline__ void fft_stage_ct(T2 *lds, int ls, const T2 *__restrict__ tw, int tid) {
+__device__ __forceinline__ void fft_stage_ct(T2 *lds, const T2 *__restrict__ tw, int tid) {
+  static_assert((N & (N - 1)) == 0, "power-of-two length");
   constexpr int TOT = EPT * kThreads / F;  // butterflies in the tile
   constexpr int KMAX = (TOT + kThreads - 1) / kThreads;
   constexpr bool FULL = TOT % kThreads == 0;
-  constexpr int NBL = N / F;     // butterflies per line
-  constexpr int M = N / (S * F);
+  constexpr unsigned NBL = N / F;  // butterflies per line
+  constexpr unsigned M = N / (S * F);
+  constexpr unsigned LS = line_stride(N);
+  constexpr unsigned SM = S * M;
   T2 v[KMAX][F];
-  int lbase[KMAX], q0[KMAX], ppv[KMAX];
+  unsigned rb[KMAX], wb[KMAX], tb[KMAX];
 #pragma unroll
   for (int k = 0; k < KMAX; ++k) {
-    const int bf = tid + k * kThreads;
-    const int line = bf / NBL;
-    const int j = bf % NBL;
-    const int pp = j / S;
-    lbase[k] = line * ls;
-    q0[k] = j % S;
-    ppv[k] = pp;
+    const unsigned bf = (unsigned)tid + k * kThreads;
+    const unsigned line = bf / NBL, j = bf % NBL;
+    const unsigned pp = j / S, q0 = j % S;
+    const unsigned i0 = q0 + S * pp, o = q0 + S * F * pp;
+    rb[k] = line * LS + i0 + (i0 >> 4);
+    wb[k] = line * LS + o + (o >> 4);
+    tb[k] = pp * S;  // twiddle step (tstep = S)
   }
   const bool act = FULL || tid < TOT;
 #pragma unroll
   for (int k = 0; k < KMAX; ++k)
     if (act)
 #pragma unroll
-      for (int a = 0; a < F; ++a) v[k][a] = lds[lbase[k] + lpad(q0[k] + S * (ppv[k] + a * M))];
+      for (int a = 0; a < F; ++a) v[k][a] = lds[rb[k] + a * SM + ((a * SM) >> 4)];
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < KMAX; ++k) {
     if (act) {
       Dft<F, INV, T2>::run(v[k]);
-      const int o = q0[k] + S * (F * ppv[k]);
-      lds[lbase[k] + lpad(o)] = v[k][0];
+      lds[wb[k]] = v[k][0];
 #pragma unroll
       for (int b = 1; b < F; ++b) {
-        const T2 w = tw[ppv[k] * b * S];  // tstep = S
-        lds[lbase[k] + lpad(o + S * b)] = INV ? cmulc(v[k][b], w) : cmul(v[k][b], w);
+        if constexpr (M == 1) {  // last stage: pp = 0, every twiddle is 1
+          lds[wb[k] + S * b + ((S * b) >> 4)] = v[k][b];
+        } else {
+          const T2 w = tw[tb[k] * b];
+          lds[wb[k] + S * b + ((S * b) >> 4)] = INV ? cmulc(v[k][b], w) : cmul(v[k][b], w);
+        }
       }
     }
   }
   __syncthreads();
 }
 
+// lines of length N at the padded stride line_stride(N)
 template <int N, int EPT, bool INV, int K = 0, int S = 1, typename T2>
-__device__ __forceinline__ void fft_lines_ct(T2 *lds, int ls, const T2 *tw, int tid = threadIdx.x) {
+__device__ __forceinline__ void fft_lines_ct(T2 *lds, const T2 *tw, int tid = threadIdx.x) {
   if constexpr (K < ct_stages(N)) {
     constexpr int F = ct_radix(N, K);
-    fft_stage_ct<F, INV, N, S, EPT>(lds, ls, tw, tid);
-    fft_lines_ct<N, EPT, INV, K + 1, S * F>(lds, ls, tw, tid);
+    fft_stage_ct<F, INV, N, S, EPT>(lds, tw, tid);
+    fft_lines_ct<N, EPT, INV, K + 1, S * F>(lds, tw, tid);
   }
 }
 

@@ -255,7 +255,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_y(typename C2<R>::T *Z, const 
   }
   __syncthreads();
   if constexpr (NL > 0)
-    fft_lines_ct<NL, EPT, INV>(lds, ls, (const T2 *)ltw);
+    fft_lines_ct<NL, EPT, INV>(lds, (const T2 *)ltw);
   else
     fft_lines<EPT, INV>(lds, T, ls, py, ltw);
 #pragma unroll
@@ -283,21 +283,20 @@ __global__ __launch_bounds__(kThreads) void k_fft_y(typename C2<R>::T *Z, const 
 #define RISM_OZ_GROUP 2
 #endif
 constexpr int kXcds = 8;
-template <typename R, typename S, int EPT, int VC, int NL = 0>
+template <typename R, typename S, int EPT, int VC>
 __global__ __launch_bounds__(kThreads) void k_fft_z_oz(IterArgs<R, S> a) {
   const int sid = batch_solute(a.gate, a.rlist);  // this launch's solute (batched launches)
   if (sid < 0) return;
   const long so = (long)sid * a.astride;
-  // NL > 0: compile-time z length (power of two), one pack (VC == 2), T = EPT * kThreads / (2 NL)
   using T2 = typename C2<R>::T;
   using T4 = typename R4T<R>::T;
   if (!gate_open(soff(a.ctl, so), a.gate)) return;
   extern __shared__ unsigned char smem_raw[];
   T2 *lds = reinterpret_cast<T2 *>(smem_raw);
   const int nx = a.nx, ny = a.ny, V = a.V;
-  const int nz = NL > 0 ? NL : a.nz;
-  const int T = NL > 0 ? EPT * kThreads / (2 * (NL > 0 ? NL : 1)) : a.Tz;
-  const int P = NL > 0 ? 1 : a.P;
+  const int nz = a.nz;
+  const int T = a.Tz;
+  const int P = a.P;
   const long N = a.N, nxy = (long)nx * ny;
   const int ls = line_stride(nz);
   const int nxt = nx / T, ntile = nxt * (ny / 2);
@@ -315,23 +314,9 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_oz(IterArgs<R, S> a) {
     const int tile = q & 1, p = q >> 1;
     return (long)p * N + (long)z * nxy + (tile ? (long)yb * nx + xb : (long)ya * nx + xa) + i;
   };
-  // Compile-time lengths: element k of a thread sits ZR z rows after element
-  // k - 1 within its tile (tile B from k = EPT/2 on), so its address is a
-  // wave-uniform offset (zofs) plus one 32-bit lane offset shared by every k.
-  // The lane offset stays below (ZR - 1) * nxy + T <= 127 * 2^20 + 64
-  // elements (ZR = 2 NL / EPT <= 128 for NL <= 512, EPT >= 8).
-  constexpr int ZR = NL > 0 ? 2 * (NL > 0 ? NL : 1) / EPT : 1;
-  const long offa = (long)ya * nx + xa, offb = (long)yb * nx + xb;
-  auto zofs = [&](int k) -> long { return (long)((k % (EPT / 2)) * ZR) * nxy + (k < EPT / 2 ? offa : offb); };
-  const unsigned lofs = NL > 0 ? (unsigned)((threadIdx.x / T) * nxy + threadIdx.x % T) : 0u;
   T2 buf[EPT];
-  if constexpr (NL > 0) {
 #pragma unroll
-    for (int k = 0; k < EPT; ++k) buf[k] = lane_at((const T2 *)Z + zofs(k), lofs);
-  } else {
-#pragma unroll
-    for (int k = 0; k < EPT; ++k) buf[k] = Z[gofs(min((int)threadIdx.x + k * kThreads, nel - 1))];
-  }
+  for (int k = 0; k < EPT; ++k) buf[k] = Z[gofs(min((int)threadIdx.x + k * kThreads, nel - 1))];
   T2 *ltw = lds + 2 * P * T * ls, *lpre = ltw + nz;
   stage_tables(ltw, a.pz);
   __syncthreads();
@@ -352,10 +337,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_oz(IterArgs<R, S> a) {
     // it live across the k-space phase, at the cost of occupancy.
     int tidf = tid;
     asm volatile("" : "+v"(tidf));
-    if constexpr (NL > 0)
-      fft_lines_ct<NL, EPT, false>(lds, ls, (const T2 *)ltw, tidf);
-    else
-      fft_lines<EPT, false>(lds, 2 * P * T, ls, a.pz, ltw, tidf);
+    fft_lines<EPT, false>(lds, 2 * P * T, ls, a.pz, ltw, tidf);
     // k-space: pair (i, j) of tile A with (T-1-i, nz-1-j) of tile B.  |k|
     // and the spline interval are evaluated in FP64 in both precisions.
     const double tpx = 2.0 * M_PI / a.box[0], tpz = 2.0 * M_PI / a.box[2];
@@ -395,13 +377,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_oz(IterArgs<R, S> a) {
           for (int v1 = 0; v1 < VC; ++v1)
             if (v < V && v1 < V) cf[v][v1] = lane_at(chi + (long)(v * V + v1) * nseg, (unsigned)lo);
         T2 fk = mk<T2>((R)0, (R)0);
-        if (a.use_fk) {
-          if constexpr (NL > 0)  // j = tid / T + k ZR, x = xa + tid % T
-            fk = lane_at(fkh + ((long)k * ZR * (ny / 2) + ya) * nx + xa,
-                         (unsigned)((tido / T) * (ny / 2) * nx + tido % T));
-          else
-            fk = fkh[((long)j * (ny / 2) + ya) * nx + x];
-        }
+        if (a.use_fk) fk = fkh[((long)j * (ny / 2) + ya) * nx + x];
         const R sa = (R)jsign(j, a.pz.fixsign), sb = (R)jsign(jb, a.pz.fixsign);
         T2 c[VC + 1];
 #pragma unroll
@@ -455,26 +431,164 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_oz(IterArgs<R, S> a) {
     __syncthreads();
     int tidi = tid;
     asm volatile("" : "+v"(tidi));
-    if constexpr (NL > 0)
-      fft_lines_ct<NL, EPT, true>(lds, ls, (const T2 *)ltw, tidi);
-    else
-      fft_lines<EPT, true>(lds, 2 * P * T, ls, a.pz, ltw, tidi);
+    fft_lines<EPT, true>(lds, 2 * P * T, ls, a.pz, ltw, tidi);
     int tids = tid;
     asm volatile("" : "+v"(tids));
-    const unsigned lofs_s = NL > 0 ? (unsigned)((tids / T) * nxy + tids % T) : 0u;
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       const int e = tids + k * kThreads;
       if (e < nel) {
         const int i = e % T, z = (e / T) % nz, line = (e / T / nz) * T + i;
         const T2 v = cmulc(lds[line * ls + lpad(z)], lpre[z]);
-        if constexpr (NL > 0)
-          lane_at(Z + zofs(k), lofs_s) = v;
-        else
-          Z[gofs(e)] = v;
+        Z[gofs(e)] = v;
       }
     }
   }
+}
+
+// ---------------------------------------------------------------------
+// 3'. the z pass for one pack (V <= 2) on a power-of-two z length NL whose
+// tile pair fills the workgroup exactly (EPT elements per thread).  Thread
+// tid holds column i0 = tid % T of both tiles at rows z0 + k ZR (z0 = tid /
+// T, ZR = 2 NL / EPT), so every LDS position, phase factor and global
+// offset is a per-thread base plus a compile-time step, and each k-space
+// pair k joins row z0 + k ZR of tile A with row NL - 1 - z0 - k ZR of tile
+// B.  |k| and the spline interval are evaluated in FP64 in both precisions.
+// ---------------------------------------------------------------------
+template <typename R, typename S, int EPT, int NL>
+__global__ __launch_bounds__(kThreads) void k_fft_z_oz_ct(IterArgs<R, S> a) {
+  const int sid = batch_solute(a.gate, a.rlist);  // this launch's solute (batched launches)
+  if (sid < 0) return;
+  const long so = (long)sid * a.astride;
+  using T2 = typename C2<R>::T;
+  using T4 = typename R4T<R>::T;
+  if (!gate_open(soff(a.ctl, so), a.gate)) return;
+  extern __shared__ unsigned char smem_raw[];
+  T2 *lds = reinterpret_cast<T2 *>(smem_raw);
+  constexpr int T = EPT * kThreads / (2 * NL);  // columns per tile
+  constexpr int ZR = kThreads / T;              // rows between a thread's elements
+  constexpr int H = EPT / 2;                    // elements per tile and thread
+  constexpr unsigned LS = line_stride(NL);
+  static_assert(T >= 1 && ZR * H == NL && ZR % 2 == 0, "tile shape");
+  const int nx = a.nx, ny = a.ny;
+  const long N = a.N, nxy = (long)nx * ny;
+  const int nxt = nx / T, ntile = nxt * (ny / 2);
+  const int b = blockIdx.x;
+  const int t = (ntile % kXcds) ? b : (b % kXcds) * (ntile / kXcds) + b / kXcds;
+  const int xa = (t % nxt) * T, ya = t / nxt;
+  const int xb = nx - xa - T, yb = ny - 1 - ya;
+  T2 *__restrict__ Z = soff(a.Z, so);
+  const long offa = (long)ya * nx + xa, offb = (long)yb * nx + xb;
+  // wave-uniform part of element k's global offset; the lane part lofs stays
+  // below ZR * nxy + T <= 128 * 2^20 + 64 elements
+  auto zofs = [&](int k) -> long { return (long)((k % H) * ZR) * nxy + (k < H ? offa : offb); };
+  const unsigned ut = threadIdx.x;
+  const unsigned i0 = ut % T, z0 = ut / T;
+  const unsigned lofs = z0 * (unsigned)nxy + i0;
+  // LDS position of element k: line (k / H) T + i0, row z0 + (k % H) ZR
+  const unsigned la = i0 * LS + z0 + (z0 >> 4);
+  auto lpos = [&](int k) -> unsigned {
+    return la + (unsigned)((k / H) * T) * LS + (unsigned)((k % H) * ZR + (((k % H) * ZR) >> 4));
+  };
+  T2 buf[EPT];
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) buf[k] = lane_at((const T2 *)Z + zofs(k), lofs);
+  T2 *ltw = lds + 2 * T * LS, *lpre = ltw + NL;
+  stage_tables(ltw, a.pz);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) lds[lpos(k)] = cmul(buf[k], lpre[z0 + (k % H) * ZR]);
+  __syncthreads();
+  int tidf = threadIdx.x;
+  asm volatile("" : "+v"(tidf));
+  fft_lines_ct<NL, EPT, false>(lds, (const T2 *)ltw, tidf);
+  {
+    // k-space.  Pair k: A at lpos(k), B at row NL - 1 - z0 - k ZR of line
+    // 2T - 1 - i0 (its padded position steps down by the same amount as A's
+    // steps up: NL - 1 - z0 leaves at least k ZR mod 16 below its 16-block's
+    // end).  (-1)^(j + NL/2) is the same for every k (ZR is even), and B's
+    // sign is its negative (NL is even).
+    const unsigned w0 = NL - 1 - z0;
+    const unsigned lb = (2 * T - 1 - i0) * LS + w0 + (w0 >> 4);
+    const R sa = (R)jsign((int)z0, a.pz.fixsign), sb = -sa;
+    const double tpx = 2.0 * M_PI / a.box[0], tpz = 2.0 * M_PI / a.box[2];
+    const double kxd = tpx * ((double)(xa + (int)i0) + (0.5 - nx / 2));
+    const double kyd = 2.0 * M_PI / a.box[1] * ((double)ya - ny / 2.0 + 0.5);
+    const double hz = 0.5 - NL / 2;
+    const int nseg = a.ntab - 1;
+    const T4 *__restrict__ chi = a.chi;
+    const T2 *__restrict__ fkh = soff(a.fkh, so);
+    const double kxy2 = kxd * kxd + kyd * kyd, dk = a.dk, idk = 1.0 / a.dk;
+    int tido = threadIdx.x;
+    asm volatile("" : "+v"(tido));
+    const unsigned z0o = (unsigned)tido / T;
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      const int j = (int)z0o + k * ZR;
+      R tt;
+      int lo;
+      {
+        const double kz = tpz * ((double)j + hz);
+        const double kk = sqrt(kxy2 + kz * kz);
+        // spline interval on the uniform table k_i = dk (i + 1) (splint.cc:4-20)
+        lo = (int)(kk * idk) - 1;
+        lo = max(0, min(lo, nseg - 1));
+        if (lo < nseg - 1 && dk * (lo + 2) <= kk) ++lo;
+        if (lo > 0 && dk * (lo + 1) > kk) --lo;
+        tt = (R)(kk - dk * (lo + 1));
+      }
+      T4 cf[2][2];
+      const int V = a.V;
+#pragma unroll
+      for (int v = 0; v < 2; ++v)
+#pragma unroll
+        for (int v1 = 0; v1 < 2; ++v1)
+          if (v < V && v1 < V) cf[v][v1] = lane_at(chi + (long)(v * V + v1) * nseg, (unsigned)lo);
+      T2 fk = mk<T2>((R)0, (R)0);
+      if (a.use_fk)  // row j = z0 + k ZR, column xa + i0
+        fk = lane_at(fkh + ((long)k * ZR * (ny / 2) + ya) * nx + xa, (unsigned)(z0o * (ny / 2) * nx + (unsigned)tido % T));
+      const unsigned pa = la + (unsigned)(k * ZR + ((k * ZR) >> 4)), pb = lb - (unsigned)(k * ZR + ((k * ZR) >> 4));
+      const T2 za = cscale(lds[pa], sa);
+      const T2 zb = cscale(lds[pb], sb);
+      // C_0 = (za + conj zb)/2 ; C_1 = -i (za - conj zb)/2
+      T2 c[2];
+      c[0] = mk<T2>((R)0.5 * (za.x + zb.x), (R)0.5 * (za.y - zb.y));
+      c[1] = mk<T2>((R)0.5 * (za.y + zb.y), (R)-0.5 * (za.x - zb.x));
+#pragma unroll
+      for (int v = 0; v < 2; ++v) {
+        const R q = (R)a.qcf[v];
+        c[v] = mk<T2>(c[v].x - q * fk.x, c[v].y - q * fk.y);  // mqvfk (fk = 0 in delta mode)
+      }
+      T2 h[2];
+#pragma unroll
+      for (int v = 0; v < 2; ++v) {
+        R hr_ = 0, hi_ = 0;
+        if (v < V) {
+#pragma unroll
+          for (int v1 = 0; v1 < 2; ++v1)
+            if (v1 < V) {  // oz: h_v = sum_v' chi[v][v'] c_v'
+              const T4 q4 = cf[v][v1];
+              const R xv = ((q4.w * tt + q4.z) * tt + q4.y) * tt + q4.x;
+              hr_ += c[v1].x * xv;
+              hi_ += c[v1].y * xv;
+            }
+        }
+        h[v] = mk<T2>(hr_, hi_);
+      }
+      // H(k) = h0 + i h1 ; H(-k) = conj(h0) + i conj(h1)
+      const T2 ha = mk<T2>(h[0].x - h[1].y, h[0].y + h[1].x);
+      const T2 hb = mk<T2>(h[0].x + h[1].y, h[1].x - h[0].y);
+      lds[pa] = cscale(ha, sa);
+      lds[pb] = cscale(hb, sb);
+      if ((k + 1) % RISM_OZ_GROUP == 0) __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  __syncthreads();
+  int tidi = threadIdx.x;
+  asm volatile("" : "+v"(tidi));
+  fft_lines_ct<NL, EPT, true>(lds, (const T2 *)ltw, tidi);
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) lane_at(Z + zofs(k), lofs) = cmulc(lds[lpos(k)], lpre[z0 + (k % H) * ZR]);
 }
 
 // ---------------------------------------------------------------------
@@ -978,7 +1092,7 @@ hipError_t launch_fft_z_oz(const IterArgs<R, S> &a, const LaunchGeom &g, hipStre
   RISM_CT_SWITCH(nl, g.ept_z,
                  RISM_VC_SWITCH(a.V, RISM_EPT_SWITCH(g.ept_z, hipLaunchKernelGGL((k_fft_z_oz<R, S, E, VC>), grid, block,
                                                                                  g.lds_z, st, a))),
-                 hipLaunchKernelGGL((k_fft_z_oz<R, S, E, 2, NLV>), grid, block, g.lds_z, st, a))
+                 hipLaunchKernelGGL((k_fft_z_oz_ct<R, S, E, NLV>), grid, block, g.lds_z, st, a))
   return hipGetLastError();
 }
 

@@ -113,8 +113,14 @@ def test_reference_deck_fp64_and_mixed(gpu, deckdir, deck):
     print(f"{deck} mixed - fixed point: " + ", ".join(f"{k} {xmx[k] - xf[k]:.4g}" for k in keys), flush=True)
     # FP64: the oracle's trajectory (the step count) and the oracle's fixed
     # point (the same equations); the RMS-1e-6 states, reached along
-    # trajectories whose rounding differs, within the two balls
-    assert abs(st64 - orc_rec["steps"]) <= 1, (st64, orc_rec["steps"])
+    # trajectories whose rounding differs, within the two balls.  test4's
+    # trajectory turns on last-bit rounding: the same z/OZ and y kernels
+    # compiled with and without FMA contraction take 299, 308 and 325 steps
+    # (one source, two compilations with contraction, one without; all at
+    # the same fixed point), so its step count is held to 10%; test3's
+    # trajectory is stable to a step.
+    slack = 1 if deck == "test3" else max(1, round(0.1 * orc_rec["steps"]))
+    assert abs(st64 - orc_rec["steps"]) <= slack, (st64, orc_rec["steps"])
     # at RMS 1e-9 both states sit about 1e-3 of their RMS-1e-6 balls from the
     # fixed point: 0.05 J/mol (1e-5 L/mol) or 1% of the two balls
     floor = {"SFE_SC": 0.05, "SFE_GF": 0.05, "SE_LJ": 0.05, "SE_ES": 0.05, "PMV": 1e-5, "Correction_Term": 0.05}
