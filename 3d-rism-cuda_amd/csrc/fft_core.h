@@ -287,11 +287,13 @@ __device__ __forceinline__ void fft_lines(T2 *lds, int nlines, int ls, const Axi
 // arithmetic reduces to shifts and masks and no bounds test remains.
 // ---------------------------------------------------------------------
 constexpr int ct_log2(int n) { return n <= 1 ? 0 : 1 + ct_log2(n / 2); }
-constexpr int ct_radix(int N, int k) {  // radix of stage k
-  const int e = ct_log2(N), ns = (e + 3) / 4;
+// RL = log2 of the largest radix (4: radix-16 stages; 2: radix-4 stages,
+// which give a butterfly to every thread of a 1024-element tile)
+constexpr int ct_stages(int N, int RL = 4) { return (ct_log2(N) + RL - 1) / RL; }
+constexpr int ct_radix(int N, int k, int RL = 4) {  // radix of stage k
+  const int e = ct_log2(N), ns = ct_stages(N, RL);
   return 1 << (e / ns + (k < e % ns ? 1 : 0));
 }
-constexpr int ct_stages(int N) { return (ct_log2(N) + 3) / 4; }
 
 // With N, S and F powers of two, every padded LDS position of a butterfly
 // is its first one plus a compile-time offset: input a sits a*S*M after
@@ -350,13 +352,13 @@ __device__ __forceinline__ void fft_stage_ct(T2 *lds, const T2 *__restrict__ tw,
   __syncthreads();
 }
 
-// lines of length N at the padded stride line_stride(N)
-template <int N, int EPT, bool INV, int K = 0, int S = 1, typename T2>
+// lines of length N at the padded stride line_stride(N); radices up to 2^RL
+template <int N, int EPT, bool INV, int RL = 4, int K = 0, int S = 1, typename T2>
 __device__ __forceinline__ void fft_lines_ct(T2 *lds, const T2 *tw, int tid = threadIdx.x) {
-  if constexpr (K < ct_stages(N)) {
-    constexpr int F = ct_radix(N, K);
+  if constexpr (K < ct_stages(N, RL)) {
+    constexpr int F = ct_radix(N, K, RL);
     fft_stage_ct<F, INV, N, S, EPT>(lds, tw, tid);
-    fft_lines_ct<N, EPT, INV, K + 1, S * F>(lds, tw, tid);
+    fft_lines_ct<N, EPT, INV, RL, K + 1, S * F>(lds, tw, tid);
   }
 }
 

@@ -210,12 +210,13 @@ template <int W> __device__ __forceinline__ void stv(float *base, unsigned e, co
 // arithmetic: the pending update of dl, dG = G(A + dl) - G(A), B = R_ref -
 // dG and dc = dG - dl, packed two sites per complex value, forward x.
 // ---------------------------------------------------------------------
-template <int EPT, int VC, int HM, bool UPD>
+template <int EPT, int VC, int HM, bool UPD, int NX>
 __device__ __forceinline__ void mx_points_vec(const IterArgs<float, double> &a, float2 *lds, bool do_run, long so,
                                               const double *th, const long *toff, const long *roff, long rcur_off,
                                               double mix, int w, int nf) {
   constexpr int W = VC <= 2 ? EPT : EPT / 2;
-  const int nx = a.nx, L = a.Lc, ls = line_stride(nx), V = a.V;
+  const int nx = NX > 0 ? NX : a.nx, L = NX > 0 ? EPT * kThreads / (VC / 2 * NX) : a.Lc, ls = line_stride(nx);
+  const int V = a.V;
   const long N = a.N, VN = (long)V * N;
   const long base = (long)blockIdx.x * L * nx;
   const int nel = L * nx;
@@ -301,7 +302,7 @@ __device__ __forceinline__ void mx_points_vec(const IterArgs<float, double> &a, 
   }
 }
 
-template <int EPT, int VC, int HM>
+template <int EPT, int VC, int HM, int NX>
 __global__ __launch_bounds__(kThreads) void k_mx_closure_vec(IterArgs<float, double> a) {
   // the refresh list of the next step starts empty (the refresh chain of
   // this step, launched before this pass, has read it)
@@ -318,7 +319,10 @@ __global__ __launch_bounds__(kThreads) void k_mx_closure_vec(IterArgs<float, dou
   if (!do_upd && !do_run) return;
   extern __shared__ unsigned char smem_raw[];
   T2 *lds = reinterpret_cast<T2 *>(smem_raw);
-  const int nx = a.nx, L = a.Lc, ls = line_stride(nx), P = a.P;
+  // NX > 0: compile-time line length (a power of two) on a tile of exactly
+  // EPT * kThreads elements, VC / 2 packs
+  const int nx = NX > 0 ? NX : a.nx, P = NX > 0 ? VC / 2 : a.P;
+  const int L = NX > 0 ? EPT * kThreads / (VC / 2 * NX) : a.Lc, ls = line_stride(nx);
   const long N = a.N, VN = (long)a.V * N;
   const long base = (long)blockIdx.x * L * nx;
   const int nel = L * nx;
@@ -347,14 +351,19 @@ __global__ __launch_bounds__(kThreads) void k_mx_closure_vec(IterArgs<float, dou
     }
   }
   if (do_upd)
-    mx_points_vec<EPT, VC, HM, true>(a, lds, do_run, so, th, toff, roff, rcur_off, mix, w, nf);
+    mx_points_vec<EPT, VC, HM, true, NX>(a, lds, do_run, so, th, toff, roff, rcur_off, mix, w, nf);
   else
-    mx_points_vec<EPT, VC, HM, false>(a, lds, do_run, so, th, toff, roff, rcur_off, mix, w, nf);
+    mx_points_vec<EPT, VC, HM, false, NX>(a, lds, do_run, so, th, toff, roff, rcur_off, mix, w, nf);
   if (!do_run) return;
   T2 *ltw = lds + P * L * ls;
   stage_tables(ltw, a.px);
   __syncthreads();
-  fft_lines<EPT, false, 5>(lds, P * L, ls, a.pxr, ltw);
+  // radix-4 stages: at about 1024 elements a radix-16 stage would leave
+  // three waves in four idle
+  if constexpr (NX > 0)
+    fft_lines_ct<NX, EPT, false, 2>(lds, (const T2 *)ltw);
+  else
+    fft_lines<EPT, false, 5>(lds, P * L, ls, a.pxr, ltw);
   const float dv = (float)a.dv;
   T2 *__restrict__ Z = soff(a.Z, so);
   // EPT consecutive (pack, point) elements per thread, two per 16-byte
@@ -606,7 +615,7 @@ __global__ __launch_bounds__(kThreads) void k_mx_resid(IterArgs<R, double> a) {
 // flight while the inverse transform runs.  Same arithmetic as k_mx_resid;
 // the inner products are summed in another order.
 // ---------------------------------------------------------------------
-template <int EPT, int HM, int VC>
+template <int EPT, int HM, int VC, int NX>
 __global__ __launch_bounds__(kThreads) void k_mx_resid_vec(IterArgs<float, double> a) {
   const int sid = batch_solute(a.gate, a.rlist);
   if (sid < 0) return;
@@ -621,7 +630,11 @@ __global__ __launch_bounds__(kThreads) void k_mx_resid_vec(IterArgs<float, doubl
   if (!gate_open(ctl, a.gate)) return;
   extern __shared__ unsigned char smem_raw[];
   T2 *lds = reinterpret_cast<T2 *>(smem_raw);
-  const int nx = a.nx, L = a.Lx, ls = line_stride(nx), V = a.V, P = a.P;
+  // NX > 0: compile-time line length (a power of two) on a tile of exactly
+  // EPT * kThreads elements, VC / 2 packs
+  const int nx = NX > 0 ? NX : a.nx, P = NX > 0 ? VC / 2 : a.P;
+  const int L = NX > 0 ? EPT * kThreads / (VC / 2 * NX) : a.Lx, ls = line_stride(nx);
+  const int V = a.V;
   const long N = a.N, VN = (long)V * N;
   const int nel = L * nx;  // a multiple of 4 (nx is)
   const int groups = (int)((long)a.ny * a.nz / L);
@@ -687,7 +700,9 @@ __global__ __launch_bounds__(kThreads) void k_mx_resid_vec(IterArgs<float, doubl
     asm volatile("" : "+v"(tidf));
     // at 1024 elements a radix-16 stage would leave three waves in four
     // idle: the radix-4 plan then (its twiddle table is the same)
-    if constexpr (EPT <= 4)
+    if constexpr (NX > 0)
+      fft_lines_ct<NX, EPT, true, 2>(lds, (const T2 *)ltw, tidf);
+    else if constexpr (EPT <= 4)
       fft_lines<EPT, true, 5>(lds, P * L, ls, a.pxr, ltw, tidf);
     else
       fft_lines<EPT, true>(lds, P * L, ls, a.px, ltw, tidf);
@@ -752,6 +767,21 @@ __global__ __launch_bounds__(kThreads) void k_mx_resid_vec(IterArgs<float, doubl
   }
 }
 
+// compile-time x length for the FP32 chain's x passes: a power of two in
+// 64..512 on a tile of exactly 4 * kThreads elements (else 0: runtime plan)
+inline int ct_xlen(int nx, long tile_elems, int ept) {
+  if (ept != 4 || tile_elems != 4L * kThreads) return 0;
+  return (nx == 64 || nx == 128 || nx == 256 || nx == 512) ? nx : 0;
+}
+#define RISM_NX_SWITCH(nxc, CT, ...)                            \
+  switch (nxc) {                                                \
+    case 64: { constexpr int NXV = 64; CT; } break;             \
+    case 128: { constexpr int NXV = 128; CT; } break;           \
+    case 256: { constexpr int NXV = 256; CT; } break;           \
+    case 512: { constexpr int NXV = 512; CT; } break;           \
+    default: { __VA_ARGS__; } break;                            \
+  }
+
 template <typename R>
 hipError_t launch_mx_closure_fx_impl(const IterArgs<R, double> &a, const LaunchGeom &g, int hm, hipStream_t st) {
   const dim3 grid(g.cgroups, a.nsol > 1 ? a.nsol : 1), block(kThreads);
@@ -761,13 +791,18 @@ hipError_t launch_mx_closure_fx_impl(const IterArgs<R, double> &a, const LaunchG
   if constexpr (sizeof(R) == sizeof(float)) {
     const int W = a.V <= 2 ? g.ept_c : g.ept_c / 2;
     if (a.V <= 4 && (a.Lc * a.nx) % W == 0 && W <= 8) {
-#define RISM_MXCV(HMV)                                                                                           \
-  if (a.V <= 2) {                                                                                                \
-    constexpr int VC = 2;                                                                                        \
-    RISM_EPTX_SWITCH(g.ept_c, hipLaunchKernelGGL((k_mx_closure_vec<E, VC, HMV>), grid, block, g.lds_c, st, a)); \
-  } else {                                                                                                       \
-    constexpr int VC = 4;                                                                                        \
-    RISM_EPTX_SWITCH(g.ept_c, hipLaunchKernelGGL((k_mx_closure_vec<E, VC, HMV>), grid, block, g.lds_c, st, a)); \
+      const int nxc = ct_xlen(a.nx, (long)a.P * a.Lc * a.nx, g.ept_c);
+#define RISM_MXCV(HMV)                                                                                              \
+  if (a.V <= 2) {                                                                                                   \
+    constexpr int VC = 2;                                                                                           \
+    RISM_NX_SWITCH(nxc, hipLaunchKernelGGL((k_mx_closure_vec<4, VC, HMV, NXV>), grid, block, g.lds_c, st, a),      \
+                   RISM_EPTX_SWITCH(g.ept_c, hipLaunchKernelGGL((k_mx_closure_vec<E, VC, HMV, 0>), grid, block, \
+                                                                g.lds_c, st, a)));                                  \
+  } else {                                                                                                          \
+    constexpr int VC = 4;                                                                                           \
+    RISM_NX_SWITCH(nxc, hipLaunchKernelGGL((k_mx_closure_vec<4, VC, HMV, NXV>), grid, block, g.lds_c, st, a),      \
+                   RISM_EPTX_SWITCH(g.ept_c, hipLaunchKernelGGL((k_mx_closure_vec<E, VC, HMV, 0>), grid, block, \
+                                                                g.lds_c, st, a)));                                  \
   }
       switch (hm) {
         case 2: RISM_MXCV(2); break;
@@ -801,13 +836,18 @@ hipError_t launch_mx_resid_impl(const IterArgs<R, double> &a, const LaunchGeom &
   if constexpr (sizeof(R) == sizeof(float)) {
     const int W = a.V <= 2 ? g.ept_x : g.ept_x / 2;
     if (a.V <= 4 && (a.Lx * a.nx) % W == 0 && a.V * (1 + hm) * W <= 48) {
-#define RISM_MXRV(HMV)                                                                                         \
-  if (a.V <= 2) {                                                                                              \
-    constexpr int VC = 2;                                                                                      \
-    RISM_EPTX_SWITCH(g.ept_x, hipLaunchKernelGGL((k_mx_resid_vec<E, HMV, VC>), grid, block, g.lds_x, st, a)); \
-  } else {                                                                                                     \
-    constexpr int VC = 4;                                                                                      \
-    RISM_EPTX_SWITCH(g.ept_x, hipLaunchKernelGGL((k_mx_resid_vec<E, HMV, VC>), grid, block, g.lds_x, st, a)); \
+      const int nxc = ct_xlen(a.nx, (long)a.P * a.Lx * a.nx, g.ept_x);
+#define RISM_MXRV(HMV)                                                                                            \
+  if (a.V <= 2) {                                                                                                 \
+    constexpr int VC = 2;                                                                                         \
+    RISM_NX_SWITCH(nxc, hipLaunchKernelGGL((k_mx_resid_vec<4, HMV, VC, NXV>), grid, block, g.lds_x, st, a),      \
+                   RISM_EPTX_SWITCH(g.ept_x, hipLaunchKernelGGL((k_mx_resid_vec<E, HMV, VC, 0>), grid, block, \
+                                                                g.lds_x, st, a)));                                \
+  } else {                                                                                                        \
+    constexpr int VC = 4;                                                                                         \
+    RISM_NX_SWITCH(nxc, hipLaunchKernelGGL((k_mx_resid_vec<4, HMV, VC, NXV>), grid, block, g.lds_x, st, a),      \
+                   RISM_EPTX_SWITCH(g.ept_x, hipLaunchKernelGGL((k_mx_resid_vec<E, HMV, VC, 0>), grid, block, \
+                                                                g.lds_x, st, a)));                                \
   }
       switch (hm) {
         case 2: RISM_MXRV(2); break;
