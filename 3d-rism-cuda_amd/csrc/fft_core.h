@@ -227,15 +227,10 @@ __device__ __forceinline__ void fft_stage(T2 *lds, int nlines, int ls, int n, in
       Dft<F, INV, T2>::run(v[k]);
       const int o = q0[k] + s * (F * ppv[k]);
       lds[lbase[k] + lpad(o)] = v[k][0];
-      if (m == 1) {  // last stage: every twiddle is 1
 #pragma unroll
-        for (int b = 1; b < F; ++b) lds[lbase[k] + lpad(o + s * b)] = v[k][b];
-      } else {
-#pragma unroll
-        for (int b = 1; b < F; ++b) {
-          const T2 w = tw[ppv[k] * b * tstep];  // tw lives in LDS
-          lds[lbase[k] + lpad(o + s * b)] = INV ? cmulc(v[k][b], w) : cmul(v[k][b], w);
-        }
+      for (int b = 1; b < F; ++b) {
+        const T2 w = tw[ppv[k] * b * tstep];  // tw lives in LDS
+        lds[lbase[k] + lpad(o + s * b)] = INV ? cmulc(v[k][b], w) : cmul(v[k][b], w);
       }
     }
   }
