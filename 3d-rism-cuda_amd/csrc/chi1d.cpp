@@ -23,6 +23,17 @@
 //   isothermal compressibility: rho kT chi_T = 1 / (1 - sum_ag rho_a rho_g
 //   c_ag(k=0) / rho_tot), the xt of the header line (solvent_read.cc:41)
 //
+// DRISM (Perkyns & Pettitt, Chem. Phys. Lett. 190, 626 (1992)), opts.dielec
+// > 0: for the one polar species (density rho_p, dipole mu), its sites in a
+// frame centred on the centre of mass with z along mu,
+//   zeta_ag(k) = f_a h_c f_g,  f_a = j0(k x_a) j0(k y_a) j1(k z_a),
+//   h_c(k) = h_c0 exp(-k^2 a^2 / 4),  h_c0 = ((eps - 1) / y - 3) / rho_p,
+//   y = 4 pi beta rho_p mu^2 / 9,
+//   omega' = omega + rho zeta,  h = h' + zeta,  h' = omega' c omega' + omega' c rho h'.
+// At k -> 0 this gives the longitudinal dielectric constant eps (XRISM's
+// 1 + 3y is h_c0 = 0).  The solve reaches it from the XRISM solution,
+// scaling zeta up in 10 stages.
+//
 // This runs on the host: N is 4096-16384 and a solve is 0.1-1 s once per
 // solvent, a problem far too small to occupy a 256-CU device.  Physical
 // constants are those of src-cuda/physical.h:5-17.
@@ -114,6 +125,21 @@ bool gauss_solve(double *A, double *B, int n, int m) {
 }
 
 double j0(double x) { return std::fabs(x) < 1e-12 ? 1.0 : std::sin(x) / x; }
+double j1(double x) {
+  return std::fabs(x) < 1e-3 ? x / 3.0 - x * x * x / 30.0 : (std::sin(x) / x - std::cos(x)) / x;
+}
+
+// atomic mass (g/mol) of the element a site name starts with (two letters
+// when known: Na, Cl), 0 when unknown
+double element_mass(const char *name) {
+  static const struct { const char *e; double m; } tab[] = {
+      {"He", 4.0026}, {"Li", 6.94}, {"Na", 22.990}, {"Mg", 24.305}, {"Cl", 35.45}, {"Ca", 40.078},
+      {"Br", 79.904}, {"H", 1.008}, {"C", 12.011}, {"N", 14.007}, {"O", 15.999}, {"F", 18.998},
+      {"P", 30.974}, {"S", 32.06}, {"K", 39.098}, {"I", 126.90}};
+  for (const auto &t : tab)
+    if (std::strncmp(name, t.e, std::strlen(t.e)) == 0) return t.m;
+  return 0.0;
+}
 
 struct Chi1D {
   int S = 0, N = 0, P = 0, M = 0;  // sites, table length, site pairs, points per function
@@ -123,6 +149,8 @@ struct Chi1D {
   std::vector<double> wt, rho, r, k;
   std::vector<double> bu_lj, bu_sr, bul_k;  // [P][M]
   std::vector<double> omega;                // [M][S][S]
+  std::vector<double> zeta;                 // [M][S][S], DRISM (empty: XRISM)
+  double zscale = 0.0;                      // DRISM stage: zeta x zscale in the map
   std::vector<double> xre, xim;             // FFT buffers, length 2N
   Fft2 fft;
 
@@ -166,6 +194,95 @@ struct Chi1D {
     for (int i = 0; i < M; ++i) omega_at(st, k[i], &omega[(size_t)i * S * S]);
     xre.assign(2 * N, 0.0);
     xim.assign(2 * N, 0.0);
+  }
+
+  // DRISM zeta_ag(k) of the one polar species (file comment); false (with
+  // the error set) when the model has no or several polar species or a site
+  // of unknown element
+  bool set_zeta(const rism_chi_site *st, double eps, double a, double beta_c) {
+    std::vector<int> spc;
+    for (int s = 0; s < S; ++s)
+      if (std::find(spc.begin(), spc.end(), st[s].species) == spc.end()) spc.push_back(st[s].species);
+    int polar = -1;
+    double mu[3] = {0, 0, 0};
+    for (int sp : spc) {
+      double m[3] = {0, 0, 0};
+      for (int s = 0; s < S; ++s)
+        if (st[s].species == sp)
+          for (int d = 0; d < 3; ++d) m[d] += st[s].q * st[s].xyz[d];
+      if (std::sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]) > 1e-8) {
+        if (polar >= 0) {
+          set_error("DRISM: more than one polar species");
+          return false;
+        }
+        polar = sp;
+        std::copy(m, m + 3, mu);
+      }
+    }
+    if (polar < 0) {
+      set_error("DRISM: no polar species");
+      return false;
+    }
+    std::vector<int> idx;
+    double com[3] = {0, 0, 0}, mt = 0.0;
+    for (int s = 0; s < S; ++s)
+      if (st[s].species == polar) {
+        const double m = element_mass(st[s].name);
+        if (m <= 0.0) {
+          set_error("DRISM: a polar site's name must start with its element (for the centre of mass)");
+          return false;
+        }
+        idx.push_back(s);
+        for (int d = 0; d < 3; ++d) com[d] += m * st[s].xyz[d];
+        mt += m;
+      }
+    for (double &c : com) c /= mt;
+    const double mun = std::sqrt(mu[0] * mu[0] + mu[1] * mu[1] + mu[2] * mu[2]);
+    const double ez[3] = {mu[0] / mun, mu[1] / mun, mu[2] / mun};
+    double ex[3] = {0, 0, 0};
+    bool have_x = false;
+    for (int s : idx) {  // x axis: the first site off the dipole axis
+      double v[3], t[3];
+      for (int d = 0; d < 3; ++d) v[d] = st[s].xyz[d] - com[d];
+      const double vz = v[0] * ez[0] + v[1] * ez[1] + v[2] * ez[2];
+      for (int d = 0; d < 3; ++d) t[d] = v[d] - vz * ez[d];
+      const double tn = std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+      if (tn > 1e-8) {
+        for (int d = 0; d < 3; ++d) ex[d] = t[d] / tn;
+        have_x = true;
+        break;
+      }
+    }
+    if (!have_x) {  // linear molecule: any axis normal to the dipole
+      const double u[3] = {std::fabs(ez[0]) < 0.9 ? 1.0 : 0.0, std::fabs(ez[0]) < 0.9 ? 0.0 : 1.0, 0.0};
+      ex[0] = ez[1] * u[2] - ez[2] * u[1];
+      ex[1] = ez[2] * u[0] - ez[0] * u[2];
+      ex[2] = ez[0] * u[1] - ez[1] * u[0];
+      const double n = std::sqrt(ex[0] * ex[0] + ex[1] * ex[1] + ex[2] * ex[2]);
+      for (double &e : ex) e /= n;
+    }
+    const double ey[3] = {ez[1] * ex[2] - ez[2] * ex[1], ez[2] * ex[0] - ez[0] * ex[2], ez[0] * ex[1] - ez[1] * ex[0]};
+    const double rp = rho[idx[0]];
+    const double y = 4.0 * M_PI * beta_c * rp * mun * mun / 9.0;
+    const double hc0 = ((eps - 1.0) / y - 3.0) / rp;
+    std::vector<double> fx(S), fy(S), fz(S);
+    for (int s : idx) {
+      double v[3];
+      for (int d = 0; d < 3; ++d) v[d] = st[s].xyz[d] - com[d];
+      fx[s] = v[0] * ex[0] + v[1] * ex[1] + v[2] * ex[2];
+      fy[s] = v[0] * ey[0] + v[1] * ey[1] + v[2] * ey[2];
+      fz[s] = v[0] * ez[0] + v[1] * ez[1] + v[2] * ez[2];
+    }
+    zeta.assign((size_t)M * S * S, 0.0);
+    std::vector<double> f(S);
+    for (int i = 0; i < M; ++i) {
+      const double kk = k[i], hc = hc0 * std::exp(-kk * kk * a * a / 4.0);
+      for (int s = 0; s < S; ++s) f[s] = 0.0;
+      for (int s : idx) f[s] = j0(kk * fx[s]) * j0(kk * fy[s]) * j1(kk * fz[s]);
+      for (int s : idx)
+        for (int g : idx) zeta[(size_t)i * S * S + s * S + g] = f[s] * hc * f[g];
+    }
+    return true;
   }
 
   void omega_at(const rism_chi_site *st, double kk, double *w) const {
@@ -232,9 +349,16 @@ struct Chi1D {
       cs[e] = h - ts[e];
     }
     transform(cs.data(), csk.data(), true);
-    std::vector<double> C(S * S), WC(S * S), A(S * S), B(S * S);
+    std::vector<double> C(S * S), WC(S * S), A(S * S), B(S * S), Wz(S * S);
+    const bool dr_on = !zeta.empty() && zscale != 0.0;
     for (int i = 0; i < M; ++i) {
       const double *W = &omega[(size_t)i * S * S];
+      const double *Z = dr_on ? &zeta[(size_t)i * S * S] : nullptr;
+      if (dr_on) {  // omega' = omega + rho zeta
+        for (int a = 0; a < S; ++a)
+          for (int g = 0; g < S; ++g) Wz[a * S + g] = W[a * S + g] + rho[a] * zscale * Z[a * S + g];
+        W = Wz.data();
+      }
       for (int a = 0; a < S; ++a)
         for (int g = 0; g < S; ++g) {
           const size_t e = (size_t)pidx[a * S + g] * M + i;
@@ -255,7 +379,7 @@ struct Chi1D {
         }
       if (!gauss_solve(A.data(), B.data(), S, S)) return false;
       for (int p = 0; p < P; ++p) {
-        const double h = B[pa[p] * S + pg[p]];
+        const double h = B[pa[p] * S + pg[p]] + (dr_on ? zscale * Z[pa[p] * S + pg[p]] : 0.0);
         tsk[(size_t)p * M + i] = h - csk[(size_t)p * M + i];
         if (hk_out) hk_out[(size_t)p * M + i] = h;
       }
@@ -394,6 +518,8 @@ extern "C" void rism_chi_default_opts(rism_chi_opts *o) {
   o->nhist = 10;
   o->mix = 0.3;
   o->chargeup = 4;
+  o->dielec = 0.0;
+  o->adbcor = 0.5;
 }
 
 extern "C" int rism_chi_solve(const rism_chi_site *sites, int nsite, const rism_chi_opts *o, double *chi,
@@ -403,11 +529,16 @@ extern "C" int rism_chi_solve(const rism_chi_site *sites, int nsite, const rism_
   if (rc) return rc;
   if (!o || o->ntab < 16 || (o->ntab & (o->ntab - 1)) || o->ntab > (1 << 22) || !(o->dr > 0.0) ||
       !(o->temper > 0.0) || o->nhist < 1 || !(o->mix > 0.0) || o->maxstep < 1 || o->chargeup < 0 ||
-      (o->closure != RISM_CLOSURE_KH && o->closure != RISM_CLOSURE_HNC)) {
-    set_error("1D RISM: bad options (ntab a power of two >= 16, dr, T, mix > 0, nhist, maxstep >= 1, KH/HNC)");
+      (o->closure != RISM_CLOSURE_KH && o->closure != RISM_CLOSURE_HNC) || o->dielec < 0.0 ||
+      (o->dielec > 0.0 && !(o->adbcor > 0.0))) {
+    set_error("1D RISM: bad options (ntab a power of two >= 16, dr, T, mix > 0, nhist, maxstep >= 1, KH/HNC, "
+              "dielec >= 0, adbcor > 0)");
     return RISM_ERR_ARG;
   }
   Chi1D s(sites, nsite, *o);
+  if (o->dielec > 0.0 &&
+      !s.set_zeta(sites, o->dielec, o->adbcor, kHartree * kBohr / (kBoltzmann * o->temper)))
+    return RISM_ERR_ARG;
   std::vector<double> ts((size_t)s.P * s.M, 0.0);
   int total = 0, st = 0;
   double r = 0.0;
@@ -417,6 +548,13 @@ extern "C" int rism_chi_solve(const rism_chi_site *sites, int nsite, const rism_
     rc = s.solve(ts, o->tol, o->maxstep, o->nhist, o->mix, &st, &r);
     total += st;
     if (rc == RISM_SOLVE_DIVERGED || rc < 0) break;
+  }
+  // DRISM: from the XRISM solution, zeta scaled up in 10 stages
+  constexpr int kZetaStages = 10;
+  for (int stage = 1; !s.zeta.empty() && stage <= kZetaStages && rc >= 0 && rc != RISM_SOLVE_DIVERGED; ++stage) {
+    s.zscale = (double)stage / kZetaStages;
+    rc = s.solve(ts, o->tol, o->maxstep, o->nhist, o->mix, &st, &r);
+    total += st;
   }
   if (steps) *steps = total;
   if (rms) *rms = r;

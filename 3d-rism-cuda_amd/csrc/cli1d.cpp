@@ -4,7 +4,8 @@
 // file (its site rows, ntab, dr and temperature); options override the grid.
 //
 //   rism1d-amd [-c KH|HNC] [-n ntab] [-r dr] [-T K] [-t tol] [-u stages]
-//              [-m vectors] [-x mix] model.xvk out.xvk
+//              [-m vectors] [-x mix] [-e eps [-a adbcor]] model.xvk out.xvk
+// -e eps selects DRISM to dielectric constant eps (XRISM without it).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -17,7 +18,7 @@
 static void usage() {
   std::fprintf(stderr,
                "usage: rism1d-amd [-c KH|HNC] [-n ntab] [-r dr] [-T K] [-t tol] [-u chargeup] [-m nhist] "
-               "[-x mix] model.xvk out.xvk\n");
+               "[-x mix] [-e eps (DRISM) [-a adbcor]] model.xvk out.xvk\n");
   std::exit(1);
 }
 
@@ -43,6 +44,8 @@ int main(int argc, char **argv) {
       case 'u': o.chargeup = std::atoi(v); break;
       case 'm': o.nhist = std::atoi(v); break;
       case 'x': o.mix = std::atof(v); break;
+      case 'e': o.dielec = std::atof(v); break;
+      case 'a': o.adbcor = std::atof(v); break;
       default: usage();
     }
   }
@@ -73,7 +76,7 @@ int main(int argc, char **argv) {
     std::fprintf(stderr, "%s\n", rism_last_error());
     return 1;
   }
-  std::printf("%d sites (%d reduced), ntab %d, dr %.4f, %.2f K: %d steps, rms %.3e, xt %.8e 1/Pa, %.2f s\n", ns, V,
-              o.ntab, o.dr, o.temper, steps, rms, xt, sec);
+  std::printf("%s, %d sites (%d reduced), ntab %d, dr %.4f, %.2f K: %d steps, rms %.3e, xt %.8e 1/Pa, %.2f s\n",
+              o.dielec > 0.0 ? "DRISM" : "XRISM", ns, V, o.ntab, o.dr, o.temper, steps, rms, xt, sec);
   return 0;
 }

@@ -104,7 +104,7 @@ class ChiSite(C.Structure):
 class ChiOpts(C.Structure):
     _fields_ = [("ntab", C.c_int), ("dr", C.c_double), ("temper", C.c_double), ("closure", C.c_int),
                 ("tol", C.c_double), ("maxstep", C.c_int), ("nhist", C.c_int), ("mix", C.c_double),
-                ("chargeup", C.c_int)]
+                ("chargeup", C.c_int), ("dielec", C.c_double), ("adbcor", C.c_double)]
 
 
 class DeckC(C.Structure):
@@ -382,14 +382,17 @@ def make_sites(rows) -> C.Array:
 
 
 def chi_solve(sites, ntab: int = 16384, dr: float = 0.01, temper: float = 298.15, closure: int = KH,
-              tol: float = 1e-10, maxstep: int = 10000, nhist: int = 10, mix: float = 0.3, chargeup: int = 4):
-    """chi_vv(k) of a solvent model by the library's 1D XRISM solve
-    (rism_chi_solve): returns (Solvent, info dict with steps, rms, xt, status)."""
+              tol: float = 1e-10, maxstep: int = 10000, nhist: int = 10, mix: float = 0.3, chargeup: int = 4,
+              dielec: float = 0.0, adbcor: float = 0.5):
+    """chi_vv(k) of a solvent model by the library's 1D XRISM solve, or DRISM
+    to the dielectric constant `dielec` (> 0) (rism_chi_solve): returns
+    (Solvent, info dict with steps, rms, xt, status)."""
     L = lib()
     o = ChiOpts()
     L.rism_chi_default_opts(C.byref(o))
     o.ntab, o.dr, o.temper, o.closure, o.tol = ntab, dr, temper, closure, tol
     o.maxstep, o.nhist, o.mix, o.chargeup = maxstep, nhist, mix, chargeup
+    o.dielec, o.adbcor = dielec, adbcor
     V = L.rism_chi_nreduced(sites, len(sites))
     chi = np.zeros((ntab, max(V, 1), max(V, 1)))
     xt, rms, steps = C.c_double(), C.c_double(), C.c_int()

@@ -304,8 +304,9 @@ int rism_batch_solve(int n, const rism_grid *grids, const rism_solvent *solvent,
 /* The reference reads this table (solvent_read.cc:10-86) and never computes
    it.  These entry points compute it with a 1D XRISM solve (KH or HNC) on
    the DST-I grid r_j = j dr, k_i = i pi/(ntab dr), the scheme that reproduces
-   the reference's own example .xvk files, and write it in that format.  They
-   run on the host (a 4096-16384 point problem, once per solvent). */
+   the reference's own example .xvk files, or with its dielectrically
+   consistent form DRISM (rism_chi_opts.dielec), and write it in that format.
+   They run on the host (a 4096-16384 point problem, once per solvent). */
 typedef struct rism_chi_site {  /* one .xvk site row (solvent_read.cc:42-47) */
   char name[8];
   int species;    /* molecule index, 1-based: sites of one species form one rigid molecule */
@@ -327,10 +328,13 @@ typedef struct rism_chi_opts {
   int nhist;      /* Anderson (DIIS form) vectors */
   double mix;     /* mixing of the residual */
   int chargeup;   /* stages n: Coulomb scaled by 0, 1/n, .., 1 (0: one full-charge stage) */
+  double dielec;  /* > 0: DRISM (Perkyns & Pettitt) to this dielectric constant for the one
+                     polar species; 0: XRISM */
+  double adbcor;  /* DRISM: smoothing length a of h_c(k) = h_c0 exp(-k^2 a^2 / 4), Angstrom */
 } rism_chi_opts;
 
 /* ntab 16384, dr 0.01, 298.15 K, KH, tol 1e-10, maxstep 10000, 10 vectors,
-   mix 0.3, 4 charge-up stages */
+   mix 0.3, 4 charge-up stages, XRISM (dielec 0), adbcor 0.5 */
 void rism_chi_default_opts(rism_chi_opts *o);
 /* number of reduced sites V = max |sym| */
 int rism_chi_nreduced(const rism_chi_site *sites, int nsite);

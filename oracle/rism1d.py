@@ -19,6 +19,21 @@ dk = pi/(ntab*dr)) gives; that is the scheme restated here:
   charge-up: the Coulomb terms are scaled by lam = 1/n .. 1 (or 0 .. 1),
   each stage starting from the previous stage's t_s.
 
+DRISM (Perkyns & Pettitt, Chem. Phys. Lett. 190, 626 (1992)), `dielectric`
+given: for the one polar species (molecular density rho_p, dipole mu), with
+its sites in a frame centred on the centre of mass, z along mu,
+  zeta_ag(k) = f_a(k) h_c(k) f_g(k),  f_a = j0(k x_a) j0(k y_a) j1(k z_a),
+  h_c(k) = h_c0 exp(-k^2 a^2 / 4),
+  omega' = omega + rho zeta,  h' = h - zeta,  h' = omega' c omega' + omega' c rho h'.
+As k -> 0, chi = (1 - rho omega' c)^-1 omega' and the charge is a left
+eigenvector of 1 - rho omega c_sr (omega(0) sums the charge of a molecule,
+zero), so (4 pi beta rho / k^2) q.chi.q -> 3y'/(1 + 3y') with
+3y' = 3y (1 + rho h_c0 / 3), y = 4 pi beta rho mu^2 / 9: the longitudinal
+relation 1 - 1/eps then gives eps = 1 + 3y', so
+  h_c0 = ((eps - 1) / y - 3) / rho_p,
+which is zero (XRISM) at eps = 1 + 3y.  The solve reaches it from the XRISM
+solution by scaling zeta up in stages.
+
 Physical constants are those of src-cuda/physical.h:5-17.
 """
 from __future__ import annotations
@@ -26,6 +41,16 @@ from __future__ import annotations
 import numpy as np
 from scipy import fft as sfft
 from scipy.special import erfc
+
+# atomic masses (g/mol) for the centre of mass of the polar species in DRISM,
+# by the element leading the site name
+MASS = {"H": 1.008, "He": 4.0026, "Li": 6.94, "C": 12.011, "N": 14.007, "O": 15.999, "F": 18.998,
+        "Na": 22.990, "Mg": 24.305, "P": 30.974, "S": 32.06, "Cl": 35.45, "K": 39.098, "Ca": 40.078,
+        "Br": 79.904, "I": 126.90}
+
+# a planted mistake for the tests that show the DRISM checks catch one
+# ("no_rho": omega' = omega + zeta; "eps_over_y": h_c0 = (eps / y - 3) / rho)
+_DRISM_MUTANT = None
 
 NA = 6.02214076e23
 BOHR = 0.5291771500001
@@ -69,11 +94,16 @@ def dst1(f):
     return 0.5 * sfft.dst(f, type=1, axis=-1)
 
 
+def _element(name):
+    """The element a site name starts with (two letters when known: Na, Cl)."""
+    return name[:2] if name[:2] in MASS else name[:1]
+
+
 class Rism1D:
     """XRISM/DRISM solver on the DST-I grid; `sites` as in an .xvk header."""
 
     def __init__(self, sites, temp, ntab, dr, closure="KH", eta=1.0,
-                 hard_sphere=None):
+                 hard_sphere=None, dielectric=None, adbcor=0.5):
         self.sites, self.T, self.N, self.dr = sites, float(temp), int(ntab), float(dr)
         self.closure, self.eta = closure.upper(), eta
         S = len(sites)
@@ -104,6 +134,48 @@ class Rism1D:
         self.omega = self._omega(self.k)
         self.lam = 1.0
         self.t = np.zeros((S, S, N - 1))
+        self.zeta = np.zeros((S, S, N - 1))
+        self.zfull = self._zeta(dielectric, adbcor) if dielectric else None
+
+    def _zeta(self, eps, a):
+        """zeta_ag(k) of the one polar species (module docstring)."""
+        q = self.q
+        spc = sorted({s.species for s in self.sites})
+        polar = []
+        for sp in spc:
+            idx = [i for i, s in enumerate(self.sites) if s.species == sp]
+            mu = sum(q[i] * self.sites[i].xyz for i in idx)
+            if np.linalg.norm(mu) > 1e-8:
+                polar.append((idx, mu))
+        if len(polar) != 1:
+            raise ValueError("DRISM needs exactly one polar species")
+        idx, mu = polar[0]
+        xyz = np.array([self.sites[i].xyz for i in idx])
+        m = np.array([MASS[_element(self.sites[i].name)] for i in idx])
+        r = xyz - (m[:, None] * xyz).sum(0) / m.sum()
+        mun = np.linalg.norm(mu)
+        ez = mu / mun
+        ex = None
+        for v in r:  # x axis: the first site off the dipole axis
+            t = v - (v @ ez) * ez
+            if np.linalg.norm(t) > 1e-8:
+                ex = t / np.linalg.norm(t)
+                break
+        if ex is None:
+            ex = np.cross(ez, [1.0, 0.0, 0.0] if abs(ez[0]) < 0.9 else [0.0, 1.0, 0.0])
+            ex /= np.linalg.norm(ex)
+        ey = np.cross(ez, ex)
+        rho = self.rho[idx[0]]
+        self.y = 4.0 * np.pi * self.beta_c * rho * mun ** 2 / 9.0
+        self.hc0 = ((eps - (0.0 if _DRISM_MUTANT == "eps_over_y" else 1.0)) / self.y - 3.0) / rho
+        k = self.k
+        j1 = lambda x: np.where(np.abs(x) < 1e-3, x / 3.0 - x ** 3 / 30.0,
+                                (np.sin(x) / np.where(x == 0, 1, x) - np.cos(x)) / np.where(x == 0, 1, x))
+        f = np.zeros((self.S, len(k)))
+        for n, i in enumerate(idx):
+            f[i] = np.sinc(k * (r[n] @ ex) / np.pi) * np.sinc(k * (r[n] @ ey) / np.pi) * j1(k * (r[n] @ ez))
+        hc = self.hc0 * np.exp(-(k * a) ** 2 / 4.0)
+        return f[:, None, :] * f[None, :, :] * hc[None, None, :]
 
     # ------------------------------------------------------------------
     def _omega(self, k):
@@ -138,12 +210,13 @@ class Rism1D:
     def rism_k(self, cs_k):
         """h(k) from c_s(k): (I - omega c rho) h = omega c omega per k."""
         c = cs_k - self.lam * self.bu_l_k
-        W = np.moveaxis(self.omega, -1, 0)
+        rz = 1.0 if _DRISM_MUTANT == "no_rho" else self.rho[:, None, None]
+        W = np.moveaxis(self.omega + rz * self.zeta, -1, 0)
         Ck = np.moveaxis(c, -1, 0)
         rho = self.rho
         A = np.eye(self.S)[None] - W @ Ck * rho[None, None, :]
         B = W @ Ck @ W
-        return np.moveaxis(np.linalg.solve(A, B), 0, -1)
+        return np.moveaxis(np.linalg.solve(A, B), 0, -1) + self.zeta
 
     def map(self, ts):
         h = self.closure_h(ts)
@@ -193,13 +266,19 @@ class Rism1D:
             ts = sum(cf[i] * (T[i] + mix * Rs[i]) for i in range(n))
         raise RuntimeError(f"1D RISM did not converge: rms {rms:.3e} after {maxstep} steps")
 
-    def solve_chargeup(self, nstage=4, from_zero=True, **kw):
-        """Stages lam = 0 (if from_zero), 1/nstage, ..., 1; returns total steps."""
+    def solve_chargeup(self, nstage=4, from_zero=True, zstages=10, **kw):
+        """Stages lam = 0 (if from_zero), 1/nstage, ..., 1, then (DRISM) zeta
+        scaled by 1/zstages, ..., 1; returns total steps."""
         lams = ([0.0] if from_zero else []) + [i / nstage for i in range(1, nstage + 1)]
         total = 0
+        self.zeta = 0.0 * self.zeta
         for lam in lams:
             self.lam = lam
             total += self.solve(**kw)[0]
+        if self.zfull is not None:
+            for i in range(1, zstages + 1):
+                self.zeta = self.zfull * (i / zstages)
+                total += self.solve(**kw)[0]
         return total, self.rms
 
     # ------------------------------------------------------------------

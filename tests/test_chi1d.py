@@ -11,6 +11,10 @@ Pins:
     a four-site electrolyte);
   * exact limits: the last table row is the bare omega, the XRISM dielectric
     constant is the ideal-dipole 1 + 3y;
+  * DRISM: the table's longitudinal dielectric constant at k -> 0 is the
+    requested one (two targets), DRISM at eps = 1 + 3y is XRISM, the library
+    equals the oracle's restatement, and two planted mistakes in the oracle
+    (no density on zeta, eps in place of eps - 1) fail the dielectric pin;
   * .xvk writer -> the library's reader round trip, and error paths.
 No GPU is needed (the generator runs on the host).
 """
@@ -188,3 +192,95 @@ def test_cli_writes_the_reference_table(tmp_path):
     ref = rism3d.read_xvk(example("h2o_vv.xvk"))
     assert np.abs(got.chi - ref.chi).max() < 2e-5
     assert got.xt == pytest.approx(ref.xt, rel=1e-5)
+
+
+# ---------------------------------------------------------------- DRISM
+def _water_charge_eps(sv, T, mult=(1, 2)):
+    """Longitudinal dielectric constant of a water table at k -> 0:
+    X(k) = (4 pi beta rho / k^2) sum_ag q_a q_g chi_ag, extrapolated as
+    X0 + b k^2 from the first two rows; eps = 1 / (1 - X0)."""
+    beta_c = r1.HARTREE * r1.BOHR / (r1.KB * T)
+    rho, q, n = sv.rho[0], sv.q, np.array(mult)
+    k = sv.dk * np.arange(1, 3)
+    X = [4 * np.pi * beta_c * rho * sum(n[v] * q[v] * q[w] * sv.chi[i, v, w] for v in range(2) for w in range(2))
+         / k[i] ** 2 for i in range(2)]
+    X0 = (X[0] * k[1] ** 2 - X[1] * k[0] ** 2) / (k[1] ** 2 - k[0] ** 2)
+    return 1.0 / (1.0 - X0)
+
+
+def _water_y(T):
+    beta_c = r1.HARTREE * r1.BOHR / (r1.KB * T)
+    sites = _small_model("water")
+    mu = sum(s.q * np.array(s.xyz) for s in sites)
+    return 4 * np.pi / 9 * beta_c * sites[0].dens * 6.02214076e23 * 1e-27 * float(mu @ mu)
+
+
+@pytest.mark.parametrize("eps", [78.4, 40.0])
+def test_drism_recovers_the_dielectric_constant(eps):
+    sites = _small_model("water")
+    ntab, dr, T = 4096, 0.04, 298.15
+    sv, info = rism3d.chi_solve(sites, ntab, dr, T, tol=1e-10, dielec=eps)
+    assert info["status"] == rism3d.SOLVE_CONVERGED
+    assert _water_charge_eps(sv, T) == pytest.approx(eps, rel=1e-4)
+    # the last row is still the bare omega (zeta vanishes at k_max), and the
+    # compressibility stays within a few percent of XRISM's (zeta(0) = 0)
+    x, _ = rism3d.chi_solve(sites, ntab, dr, T, tol=1e-10)
+    np.testing.assert_allclose(sv.chi[-1], x.chi[-1], atol=1e-14)
+    assert sv.xt == pytest.approx(x.xt, rel=0.05)
+    assert abs(_water_charge_eps(x, T) - eps) > 10  # XRISM alone is far from it
+
+
+def test_drism_at_the_ideal_dipole_value_is_xrism():
+    sites = _small_model("water")
+    ntab, dr, T = 1024, 0.04, 298.15
+    x, _ = rism3d.chi_solve(sites, ntab, dr, T, tol=1e-11)
+    d, _ = rism3d.chi_solve(sites, ntab, dr, T, tol=1e-11, dielec=1 + 3 * _water_y(T))
+    assert np.abs(d.chi - x.chi).max() < 1e-9 * np.abs(x.chi).max()
+
+
+def test_drism_library_matches_oracle():
+    sites = _small_model("water")
+    ntab, dr, T = 1024, 0.04, 298.15
+    sv, info = rism3d.chi_solve(sites, ntab, dr, T, tol=1e-11, chargeup=2, dielec=78.4)
+    o = r1.Rism1D(_oracle_sites(sites), T, ntab, dr, closure="KH", dielectric=78.4)
+    steps, _ = o.solve_chargeup(2, tol=1e-11)
+    assert info["status"] == rism3d.SOLVE_CONVERGED
+    ref = o.reduced()
+    assert np.abs(sv.chi - ref).max() < 1e-8 * max(1.0, np.abs(ref).max())
+    assert sv.xt == pytest.approx(o.compressibility(), rel=1e-8)
+
+
+@pytest.mark.parametrize("mutant", [None, "no_rho", "eps_over_y"])
+def test_drism_planted_mistakes_fail_the_dielectric_pin(mutant, monkeypatch):
+    monkeypatch.setattr(r1, "_DRISM_MUTANT", mutant)
+    sites = _small_model("water")
+    ntab, dr, T = 2048, 0.08, 298.15
+    o = r1.Rism1D(_oracle_sites(sites), T, ntab, dr, closure="KH", dielectric=78.4)
+    try:
+        o.solve_chargeup(4, tol=1e-9, maxstep=1500)
+    except (RuntimeError, FloatingPointError):
+        assert mutant is not None  # the mistaken equations do not even converge
+        return
+    red = o.reduced()
+
+    class Tab:
+        chi, rho, q, dk = red, o.rho[[0, 1]], o.q[[0, 1]], o.dk
+    eps = _water_charge_eps(Tab, T)
+    if mutant is None:
+        assert eps == pytest.approx(78.4, rel=1e-4)
+    else:
+        assert abs(eps - 78.4) > 0.5, (mutant, eps)
+
+
+def test_drism_errors():
+    L = rism3d.lib()
+    o = rism3d.ChiOpts()
+    L.rism_chi_default_opts(C.byref(o))
+    o.ntab, o.dielec = 64, 78.4
+    chi = np.zeros(16 * 64)
+    ions = rism3d.make_sites([("Na", 1, 1, 2.35, 543.92, 1.0, (0.0, 0.0, 0.0), 0.15),
+                              ("Cl", 2, 2, 4.40, 418.40, -1.0, (0.0, 0.0, 0.0), 0.15)])
+    assert L.rism_chi_solve(ions, 2, C.byref(o), rism3d._dptr(chi), None, None, None) == rism3d.ERR_ARG
+    assert "polar" in rism3d.last_error()
+    o.dielec = -1.0
+    assert L.rism_chi_solve(_small_model("water"), 3, C.byref(o), rism3d._dptr(chi), None, None, None) == rism3d.ERR_ARG
