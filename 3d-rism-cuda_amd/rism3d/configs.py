@@ -18,8 +18,9 @@ with the reference's integer-rounded shift (src-cuda/solute.cu:13-43).
       side, rounded up to a 2,3,5-smooth multiple of 4 within 48..96.
   c4  512^3 / 256 A box, 5000 atoms, net charge -12.
 
-Solvents: chi_vv(k) comes from the library's own 1D XRISM-KH solve
-(rism_chi_solve, 16384 points at dr 0.01 A, 298.15 K):
+Solvents: chi_vv(k) comes from the library's own 1D DRISM-KH solve
+(rism_chi_solve, 16384 points at dr 0.01 A, 298.15 K, dielectric constant
+78.4; dielec=0 gives XRISM, the scheme of the reference's own tables):
   water  3-site rigid water: O sigma 3.1507 A, eps 0.1521 kcal/mol,
          q -0.834; H sigma 0.4 A, eps 0.046 kcal/mol, q +0.417; OH 0.9572 A,
          HOH 104.52 deg; rho 0.03334 1/A^3
@@ -72,15 +73,21 @@ def solvent_rows(kind: str):
     raise ValueError(f"unknown solvent {kind!r}")
 
 
-def solvent_file(kind: str = "water", ntab: int = 16384, dr: float = 0.01, temper: float = 298.15) -> str:
-    """Path of the .xvk of `kind`, computed by the library's 1D solve on first
-    use and cached beside the library."""
+WATER_DIELEC = 78.4  # water at 298.15 K
+
+
+def solvent_file(kind: str = "water", ntab: int = 16384, dr: float = 0.01, temper: float = 298.15,
+                 dielec: float = WATER_DIELEC) -> str:
+    """Path of the .xvk of `kind`, computed by the library's 1D DRISM-KH solve
+    to dielectric constant `dielec` (0: XRISM) on first use and cached beside
+    the library."""
     d = os.path.join(os.path.dirname(LIB_PATH), "solvents")
-    path = os.path.join(d, f"{kind}_{ntab}_{dr:g}_{temper:g}.xvk")
+    scheme = f"drism{dielec:g}" if dielec > 0 else "xrism"
+    path = os.path.join(d, f"{kind}_{ntab}_{dr:g}_{temper:g}_{scheme}.xvk")
     if not os.path.exists(path):
         os.makedirs(d, exist_ok=True)
         sites = make_sites(solvent_rows(kind))
-        sv, info = chi_solve(sites, ntab, dr, temper)
+        sv, info = chi_solve(sites, ntab, dr, temper, dielec=dielec)
         if info["status"] != 0:
             raise RuntimeError(f"1D RISM for {kind} did not converge: {info}")
         tmp = f"{path}.{os.getpid()}.tmp"

@@ -16,7 +16,7 @@ The JSON line also carries:
                 deck, the whole job measured (--cpu-sample: a bounded sample,
                 scaled); Delta-mu, PMV and g of both GPU paths are reported
                 against this oracle run.
-Solvent: 3-site water, chi_vv(k) from the library's own 1D XRISM-KH solve
+Solvent: 3-site water, chi_vv(k) from the library's own 1D DRISM-KH solve (eps 78.4)
 (16384 points, dr 0.01 A, 298.15 K; rism3d.configs.solvent_file), computed
 before any timing and cached.
 """
@@ -220,7 +220,7 @@ def side_workload(which, rism3d, configs):
     w = configs.config(4)
     out = {"metric": "s-to-converge 512^3 / 256 A / 5000-atom KH job (potentials + solve + thermo)",
            "unit": "s", "higher_is_better": False,
-           "solvent": "water + 0.15 M NaCl (4 reduced sites), chi from the library's 1D XRISM-KH solve"}
+           "solvent": "water + 0.15 M NaCl (4 reduced sites), chi from the library's 1D DRISM-KH solve (eps 78.4)"}
     for name, prec in (("mixed", rism3d.MIXED), ("fp64", rism3d.FP64)):
         # buffers a destroyed handle left for reuse would not show as new
         # device memory: return them first, so device_GB is this path's own
@@ -334,7 +334,7 @@ def main():
         "dtype": "f32 fields and changes / f64 reference",
         "data": "synthetic",
         "config": {"workload": w.name, "grid": list(w.n), "box_A": list(w.box), "atoms": len(w.solute.q),
-                   "solvent": "3-site water, chi from the library's 1D XRISM-KH solve (16384 pts, dr 0.01 A)", "closure": "KH",
+                   "solvent": "3-site water, chi from the library's 1D DRISM-KH solve (eps 78.4, 16384 pts, dr 0.01 A)", "closure": "KH",
                    "tol": 1e-6, "anderson": {"nhist": w.nhist, "count": 2, "m": 0.6, "mp": 0.2},
                    "parallelism": "1 GPU"},
         "iterations_mixed": mixed["iterations"],

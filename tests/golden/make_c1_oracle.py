@@ -1,6 +1,6 @@
 """Writes tests/golden/c1_oracle.npz: the FP64 oracle's solution of the
 256^3 / 128 A, 846-atom KH workload (rism3d.configs.config(1), water chi from
-the library's 1D XRISM-KH solve, Anderson 2 0.6 0.2).
+the library's 1D DRISM-KH solve (eps 78.4), Anderson 2 0.6 0.2).
 
 Two states are recorded:
   tol6   the oracle's own solve from the initial estimate to RMS <= 1e-6

@@ -3,7 +3,7 @@
 oracle's solution of the same deck (tests/golden/c1_oracle.npz, written by
 tests/golden/make_c1_oracle.py):
 
-  tol6  the oracle's solve to RMS 1e-6 (358 steps);
+  tol6  the oracle's solve to RMS 1e-6 (337 steps, DRISM-KH water);
   conv  the oracle's solve to RMS 1e-10 (the fixed point).
 
 The FP64 path follows the oracle's Anderson trajectory: the same step
@@ -13,7 +13,7 @@ The mixed path converges to RMS 1e-6 along another trajectory, so it is
 held to the fixed point: Delta-mu (SFE_SC) within 0.01 kcal/mol
 (41.84 J/mol), and g within twice the tolerance ball.  The ball is the
 oracle's own distance between its RMS-1e-6 state and the fixed point:
-0.43 J/mol in SFE_SC, and 1.2e-2 in g (max-abs over the grid), which is
+0.62 J/mol in SFE_SC, and 9.4e-3 in g (max-abs over the grid), which is
 above the 1e-4 a g(r) bar at RMS 1e-6 would ask for; no solver that stops
 at RMS 1e-6 meets that bar here, the oracle included.  g is compared on the
 committed sub-lattice (every second point of the central 96^3, +-24 A

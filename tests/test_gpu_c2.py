@@ -1,6 +1,6 @@
 """The adjoint-gradient workload at full size: 128^3 / 64 A with the seeded
 200-atom KH solute (rism3d.configs.config(2)), water from the library's 1D
-XRISM-KH table.
+DRISM-KH table.
 
 FP64: rism_adjoint_grad (the transposed fixed-point solve) against central
 finite differences of complete FP64 re-solves to RMS 1e-11 on 20 seeded

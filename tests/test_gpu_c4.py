@@ -1,7 +1,7 @@
 """The large electrolyte workload at full size: 512^3 / 256 A with the
 seeded 5000-atom solute of net charge -12 (rism3d.configs.config(4)),
 water plus 0.15 M NaCl (5 sites, V = 4 reduced; chi from the library's 1D
-XRISM-KH solve), KH, 10 Anderson histories.
+DRISM-KH solve), KH, 10 Anderson histories.
 
 Size-independent properties (an oracle run at this size takes hours on the
 host): both paths converge to RMS 1e-6; Delta-mu of the mixed path is within

@@ -91,7 +91,7 @@ def test_mixed_solve_matches_converged_oracle(gpu, case):
 @pytest.mark.parametrize("prec", [rism3d.FP64, rism3d.MIXED])
 def test_config0_64_matches_oracle(gpu, prec):
     """The seeded 64^3 / 32 A / 16-atom workload (random walk, H = 5) with the
-    library's own XRISM-KH water table, GPU against the FP64 oracle."""
+    library's own DRISM-KH water table, GPU against the FP64 oracle."""
     from rism3d import configs
     print(f"config0 prec={prec}", flush=True)
     w = configs.config(0)
