@@ -357,6 +357,21 @@ __device__ __forceinline__ void fft_lines_ct(T2 *lds, const T2 *tw, int tid = th
   }
 }
 
+// stages K..KEND-1 of the compile-time plan (the caller runs the others)
+constexpr int ct_sprod(int N, int K, int RL) {  // product of the radices before stage K
+  int s = 1;
+  for (int k = 0; k < K; ++k) s *= ct_radix(N, k, RL);
+  return s;
+}
+template <int N, int EPT, bool INV, int RL, int K, int KEND, typename T2>
+__device__ __forceinline__ void fft_lines_ct_range(T2 *lds, const T2 *tw, int tid) {
+  if constexpr (K < KEND) {
+    constexpr int F = ct_radix(N, K, RL);
+    fft_stage_ct<F, INV, N, ct_sprod(N, K, RL), EPT>(lds, tw, tid);
+    fft_lines_ct_range<N, EPT, INV, RL, K + 1, KEND>(lds, tw, tid);
+  }
+}
+
 // sign (-1)^(j + n/2) applied after a forward / before an inverse pass
 __device__ __forceinline__ int jsign(int j, int fixsign) { return (j & 1) ? -fixsign : fixsign; }
 

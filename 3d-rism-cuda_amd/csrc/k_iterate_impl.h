@@ -448,12 +448,15 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_oz(IterArgs<R, S> a) {
 
 // ---------------------------------------------------------------------
 // 3'. the z pass for one pack (V <= 2) on a power-of-two z length NL whose
-// tile pair fills the workgroup exactly (EPT elements per thread).  Thread
-// tid holds column i0 = tid % T of both tiles at rows z0 + k ZR (z0 = tid /
-// T, ZR = 2 NL / EPT), so every LDS position, phase factor and global
-// offset is a per-thread base plus a compile-time step, and each k-space
-// pair k joins row z0 + k ZR of tile A with row NL - 1 - z0 - k ZR of tile
-// B.  |k| and the spline interval are evaluated in FP64 in both precisions.
+// tile pair fills the workgroup exactly (EPT elements per thread).  The
+// first forward stage runs on the values as loaded and the last inverse
+// stage stores to global memory, so the spectrum passes through LDS only
+// between stages.  In the k-space step thread tid holds column i0 = tid % T
+// of both tiles at rows z0 + k ZR (z0 = tid / T, ZR = 2 NL / EPT): every
+// LDS position, phase factor and global offset is a per-thread base plus a
+// compile-time step, and pair k joins row z0 + k ZR of tile A with row
+// NL - 1 - z0 - k ZR of tile B.  |k| and the spline interval are evaluated
+// in FP64 in both precisions.
 // ---------------------------------------------------------------------
 template <typename R, typename S, int EPT, int NL>
 __global__ __launch_bounds__(kThreads) void k_fft_z_oz_ct(IterArgs<R, S> a) {
@@ -479,31 +482,58 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_oz_ct(IterArgs<R, S> a) {
   const int xb = nx - xa - T, yb = ny - 1 - ya;
   T2 *__restrict__ Z = soff(a.Z, so);
   const long offa = (long)ya * nx + xa, offb = (long)yb * nx + xb;
-  // wave-uniform part of element k's global offset; the lane part lofs stays
-  // below ZR * nxy + T <= 128 * 2^20 + 64 elements
-  auto zofs = [&](int k) -> long { return (long)((k % H) * ZR) * nxy + (k < H ? offa : offb); };
+  // k-space pairs of a thread: column i0 = tid % T of both tiles at rows
+  // z0 + k ZR, z0 = tid / T; global offsets are a wave-uniform part plus a
+  // 32-bit lane part below NL * nxy <= 2^27 elements
   const unsigned ut = threadIdx.x;
   const unsigned i0 = ut % T, z0 = ut / T;
-  const unsigned lofs = z0 * (unsigned)nxy + i0;
-  // LDS position of element k: line (k / H) T + i0, row z0 + (k % H) ZR
-  const unsigned la = i0 * LS + z0 + (z0 >> 4);
-  auto lpos = [&](int k) -> unsigned {
-    return la + (unsigned)((k / H) * T) * LS + (unsigned)((k % H) * ZR + (((k % H) * ZR) >> 4));
-  };
-  T2 buf[EPT];
-#pragma unroll
-  for (int k = 0; k < EPT; ++k) buf[k] = lane_at((const T2 *)Z + zofs(k), lofs);
+  const unsigned la = i0 * LS + z0 + (z0 >> 4);  // LDS position of row z0 of tile A
   T2 *ltw = lds + 2 * T * LS, *lpre = ltw + NL;
+  // The first forward stage runs on the loaded values and the last inverse
+  // stage writes global memory: butterfly bf of the first stage (radix F0,
+  // M0 = NL/F0 per line) is column bf % T of tile bf / (T M0) at rows
+  // j + M0 a, j = (bf / T) % M0, so a thread loads exactly its butterfly's
+  // inputs (lanes take consecutive columns: each load covers a few whole
+  // row segments), and the last stage's outputs q0 + SL b go straight out.
+  constexpr int NST = ct_stages(NL);
+  constexpr int F0 = ct_radix(NL, 0), M0 = NL / F0;
+  constexpr int FL = ct_radix(NL, NST - 1), SL = NL / FL;
+  constexpr int NB0 = 2 * T * M0, KB0 = (NB0 + kThreads - 1) / kThreads;
+  constexpr int NBL = 2 * T * SL, KBL = (NBL + kThreads - 1) / kThreads;
+  static_assert(NST >= 2, "first and last stage are distinct");
+  T2 v0[KB0][F0];
+#pragma unroll
+  for (int kb = 0; kb < KB0; ++kb) {
+    const unsigned bf = ut + kb * kThreads;
+    const unsigned i = bf % T, rest = bf / T, j = rest % M0, tile = rest / M0;
+    if (NB0 % kThreads == 0 || bf < NB0)
+#pragma unroll
+      for (int q = 0; q < F0; ++q)
+        v0[kb][q] = lane_at((const T2 *)Z + (long)(M0 * q) * nxy + (tile ? offb : offa), j * (unsigned)nxy + i);
+  }
   stage_tables(ltw, a.pz);
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < EPT; ++k) lds[lpos(k)] = cmul(buf[k], lpre[z0 + (k % H) * ZR]);
+  for (int kb = 0; kb < KB0; ++kb) {
+    const unsigned bf = ut + kb * kThreads;
+    const unsigned i = bf % T, rest = bf / T, j = rest % M0, tile = rest / M0;
+    if (NB0 % kThreads == 0 || bf < NB0) {
+#pragma unroll
+      for (int q = 0; q < F0; ++q) v0[kb][q] = cmul(v0[kb][q], lpre[j + M0 * q]);
+      Dft<F0, false, T2>::run(v0[kb]);
+      // stage 0 (S = 1): outputs F0 j + b, twiddle tw[j b]
+      const unsigned wb = (tile * T + i) * LS + F0 * j + ((F0 * j) >> 4);
+      lds[wb] = v0[kb][0];
+#pragma unroll
+      for (int q = 1; q < F0; ++q) lds[wb + q + (q >> 4)] = cmul(v0[kb][q], ltw[j * q]);
+    }
+  }
   __syncthreads();
   int tidf = threadIdx.x;
   asm volatile("" : "+v"(tidf));
-  fft_lines_ct<NL, EPT, false>(lds, (const T2 *)ltw, tidf);
+  fft_lines_ct_range<NL, EPT, false, 4, 1, NST>(lds, (const T2 *)ltw, tidf);
   {
-    // k-space.  Pair k: A at lpos(k), B at row NL - 1 - z0 - k ZR of line
+    // k-space.  Pair k: A at row z0 + k ZR of line i0, B at row NL - 1 - z0 - k ZR of line
     // 2T - 1 - i0 (its padded position steps down by the same amount as A's
     // steps up: NL - 1 - z0 leaves at least k ZR mod 16 below its 16-block's
     // end).  (-1)^(j + NL/2) is the same for every k (ZR is even), and B's
@@ -586,9 +616,27 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_oz_ct(IterArgs<R, S> a) {
   __syncthreads();
   int tidi = threadIdx.x;
   asm volatile("" : "+v"(tidi));
-  fft_lines_ct<NL, EPT, true>(lds, (const T2 *)ltw, tidi);
+  fft_lines_ct_range<NL, EPT, true, 4, 0, NST - 1>(lds, (const T2 *)ltw, tidi);
+  // last inverse stage (S = SL, no twiddle): butterfly q0 of line i + tile T
+  // reads q0 + SL c and writes rows q0 + SL b of global memory
+  int tids = threadIdx.x;
+  asm volatile("" : "+v"(tids));
 #pragma unroll
-  for (int k = 0; k < EPT; ++k) lane_at(Z + zofs(k), lofs) = cmulc(lds[lpos(k)], lpre[z0 + (k % H) * ZR]);
+  for (int kb = 0; kb < KBL; ++kb) {
+    const unsigned bf = (unsigned)tids + kb * kThreads;
+    const unsigned i = bf % T, rest = bf / T, q0 = rest % SL, tile = rest / SL;
+    if (NBL % kThreads == 0 || bf < NBL) {
+      T2 u[FL];
+      const unsigned rb = (tile * T + i) * LS + q0 + (q0 >> 4);
+#pragma unroll
+      for (int c = 0; c < FL; ++c) u[c] = lds[rb + SL * c + ((SL * c) >> 4)];
+      Dft<FL, true, T2>::run(u);
+#pragma unroll
+      for (int c = 0; c < FL; ++c)
+        lane_at(Z + (long)(SL * c) * nxy + (tile ? offb : offa), q0 * (unsigned)nxy + i) =
+            cmulc(u[c], lpre[q0 + SL * c]);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------
