@@ -17,20 +17,20 @@ hipError_t launch_mx_resid<double>(const IterArgs<double, double> &a, const Laun
 
 // dir 0: T = ts, A = ts - u, dl = 0 (a new reference at the state);
 // dir 1: ts = T + dl
-__global__ __launch_bounds__(kThreads) void k_mx_convert(double *ts, double *T, double *A, float *dl,
+__global__ __launch_bounds__(kThreads) void k_mx_convert(double *ts, double *T, float *A, float *dl,
                                                          const double *u, long n, int dir) {
   const long i = (long)blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
   if (dir == 0) {
     T[i] = ts[i];
-    A[i] = ts[i] - u[i];
+    A[i] = ref_f32(ts[i] - u[i]);
     dl[i] = 0.0f;
   } else {
     ts[i] = T[i] + (double)dl[i];
   }
 }
 
-hipError_t launch_mx_convert(double *ts, double *T, double *A, float *dl, const double *u, long n, int dir,
+hipError_t launch_mx_convert(double *ts, double *T, float *A, float *dl, const double *u, long n, int dir,
                              hipStream_t st) {
   const long nb = (n + kThreads - 1) / kThreads;
   hipLaunchKernelGGL(k_mx_convert, dim3((unsigned)nb), dim3(kThreads), 0, st, ts, T, A, dl, u, n, dir);

@@ -7,8 +7,8 @@
 //   c_s = G(ts - u) - 1 - ts,   h = K c (FFT, chi, inverse FFT),
 //   R = h + 1 - G(ts - u).
 // The state is ts = T + dl: T (FP64) is ts at the last refresh and dl (FP32)
-// the change since then; A = T - u (FP64) is the closure argument at the
-// refresh.  (ts itself is kept: near the solute's core u reaches 1e29, and
+// the change since then; A = T - u (stored in FP32) is the closure argument
+// at the refresh.  (ts itself is kept: near the solute's core u reaches 1e29, and
 // ts - u cannot carry ts there, while c_s = -1 - ts needs it.)  A refresh
 // step (R = double, gate 2) folds dl into T and evaluates the map in FP64,
 // storing R_ref = R (FP32).  Every other step (R = float, gate 1)
@@ -64,7 +64,7 @@ __device__ __forceinline__ void mx_points(const IterArgs<R, double> &a, typename
   const long base = (long)blockIdx.x * L * nx;
   const int nel = L * nx;
   const T2 *__restrict__ pre = (const T2 *)a.px.pre;
-  double *__restrict__ Ab = soff(a.A, so);
+  float *__restrict__ Ab = soff(a.A, so);
   double *__restrict__ Tb = soff(a.ts, so);  // the refresh chain's T
   float *__restrict__ dlb = soff(a.dl, so);
   const float *__restrict__ hx = soff(a.hx, so);
@@ -83,7 +83,7 @@ __device__ __forceinline__ void mx_points(const IterArgs<R, double> &a, typename
       for (int v = 0; v < VC; ++v) {
         if (v < V) {
           const long vb = (long)v * N + base;
-          A[c][v] = F64 ? lane_at(Tb + vb, e) : lane_at(Ab + vb, e);
+          A[c][v] = F64 ? lane_at(Tb + vb, e) : (double)lane_at(Ab + vb, e);
           d0[c][v] = (double)lane_at(dlb + vb, e);
           if constexpr (F64) u[c][v] = lane_at(uu + vb, e);
           else rf[c][v] = (double)lane_at(rref + vb, e);
@@ -130,7 +130,7 @@ __device__ __forceinline__ void mx_points(const IterArgs<R, double> &a, typename
             if (UPD && valid) lane_at(hxw + (long)w * VN + vb, e) = (float)(d - d0[c][v]);
             if (valid && (UPD || do_run)) {
               lane_at(Tb + vb, e) = Tn;
-              lane_at(Ab + vb, e) = An;
+              lane_at(Ab + vb, e) = ref_f32(An);
               lane_at(dlb + vb, e) = 0.0f;
             }
             if (do_run) cs[v] = (R)(closure_g<double>(An, a.closure) - 1.0 - Tn);
@@ -221,7 +221,7 @@ __device__ __forceinline__ void mx_points_vec(const IterArgs<float, double> &a, 
   const long base = (long)blockIdx.x * L * nx;
   const int nel = L * nx;
   const float2 *__restrict__ pre = (const float2 *)a.px.pre;
-  const double *__restrict__ Ab = soff(a.A, so);
+  const float *__restrict__ Ab = soff(a.A, so);
   float *__restrict__ dlb = soff(a.dl, so);
   const float *__restrict__ hx = soff(a.hx, so);
   float *__restrict__ hxw = soff(a.hx, so);
@@ -235,7 +235,7 @@ __device__ __forceinline__ void mx_points_vec(const IterArgs<float, double> &a, 
   for (int v = 0; v < VC; ++v) {
     if (v < V) {
       const long vb = (long)v * N + base;
-      double A[W];
+      float A[W];
       float d0[W], rf[W], r[W], hti[UPD ? HM : 1][W], hri[UPD ? HM : 1][W];
       ldv<W>(A, Ab + vb, e0);
       ldv<W>(d0, dlb + vb, e0);
@@ -270,7 +270,7 @@ __device__ __forceinline__ void mx_points_vec(const IterArgs<float, double> &a, 
         hxo[j] = (float)((double)dn - (double)d0[j]);
         cs[v][j] = 0.0f;
         if (do_run) {
-          const double dg = closure_g<double>(A[j] + (double)dn, a.closure) - closure_g<double>(A[j], a.closure);
+          const double dg = closure_g<double>((double)A[j] + (double)dn, a.closure) - closure_g<double>((double)A[j], a.closure);
           bo[j] = (float)((double)rf[j] - dg);
           cs[v][j] = (float)(dg - (double)dn);
         }
@@ -528,7 +528,7 @@ __global__ __launch_bounds__(kThreads) void k_mx_resid(IterArgs<R, double> a) {
 #pragma unroll
     for (int k0 = 0; k0 < KPT; k0 += CH) {
       double hv[CH][VC + 1];
-      double ov[CH][VC];  // FP64: A (g = G(A)); FP32: B
+      double ov[CH][VC];  // FP64: A = T - u (g = G(A)); FP32: B
       float hq[CH][VC][PREH ? HM : 1];
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
@@ -539,7 +539,7 @@ __global__ __launch_bounds__(kThreads) void k_mx_resid(IterArgs<R, double> a) {
         for (int v = 0; v < VC; ++v)
           if (v < V) {
             const long vb = (long)v * N + base;
-            if constexpr (F64) ov[c][v] = lane_at(soff(a.A, so) + vb, e);
+            if constexpr (F64) ov[c][v] = lane_at(soff(a.ts, so) + vb, e) - (double)lane_at(soff(a.u, so) + vb, e);
             else ov[c][v] = (double)lane_at(soff(a.bq, so) + vb, e);
 #pragma unroll
             for (int s = 0; s < (PREH ? HM : 0); ++s)

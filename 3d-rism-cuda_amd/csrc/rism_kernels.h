@@ -107,6 +107,15 @@ template <typename P> __device__ __forceinline__ P soff(P p, long o) {
   return (P)((const char *)p + o);
 }
 
+// The mixed path's closure argument at the reference, A = T - u, rounded to
+// FP32 for storage.  The FP32 chain only forms G(A + dl) - G(A) from it, whose
+// error from rounding A is second order (G''(A) dA dl); the refresh chain
+// evaluates T - u in FP64 itself.  Inside the solute's cores u reaches 1e30,
+// so the value is clamped to the FP32 range (G is 0 there either way).
+__device__ __forceinline__ float ref_f32(double a) {
+  return (float)fmin(fmax(a, -3.0e38), 3.0e38);
+}
+
 // Residual / history storage type of the FP64 path's passes (the mixed
 // path's state is described with IterArgs::A below).
 template <typename R> struct HistT;
@@ -150,8 +159,8 @@ struct IterArgs {
   double *rms_hist;
   // ---- mixed precision on differences to an FP64 reference (gate 1, 2) ----
   // ts = T + dl: T [V][N] (FP64, in the ts field) is ts at the last refresh,
-  // dl [V][N] (FP32) the change since; A [V][N] (FP64) = T - u, the closure
-  // argument at the refresh; rref [V][N] (FP32) the refresh step's FP64
+  // dl [V][N] (FP32) the change since; A [V][N] (FP32, ref_f32) = T - u, the
+  // closure argument at the refresh; rref [V][N] (FP32) the refresh step's FP64
   // residual, hx [H][V][N] (FP32) the step increments of ts, rring
   // [H+1][V][N] (FP32) the residual ring (Ctl::rmap).  The FP32 chain
   // transforms dc = G(A + dl) - G(A) - dl (the change of c_s, whose
@@ -172,7 +181,7 @@ struct IterArgs {
   // (batch_solute), and the FP32 chain's closure pass empties it again.
   // nullptr: blockIdx.y is the solute.
   int *rlist;
-  double *A;
+  float *A;
   float *dl;
   float *rref;
   float *bq;            // B, closure pass -> residual pass
@@ -213,7 +222,7 @@ template <typename R>
 hipError_t launch_mx_resid(const IterArgs<R, double> &a, const LaunchGeom &g, int hm, hipStream_t st);
 // t_s <-> state conversions of the mixed path: dir 0: T = ts, A = ts - u,
 // dl = 0; dir 1: ts = T + dl
-hipError_t launch_mx_convert(double *ts, double *T, double *A, float *dl, const double *u, long n, int dir,
+hipError_t launch_mx_convert(double *ts, double *T, float *A, float *dl, const double *u, long n, int dir,
                              hipStream_t st);
 // ---- adjoint of the converged fixed point (k_adjoint.hip) ----
 struct AdjArgs {

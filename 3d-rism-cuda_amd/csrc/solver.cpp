@@ -131,7 +131,8 @@ struct rism_handle {
   // mixed precision state (k_mixed_impl.h): reference T of t_s and A of
   // the closure argument, the change dl, the refresh residual R_ref, the closure
   // pass's B, the increment history and the residual ring
-  double *mT = nullptr, *mA = nullptr;
+  double *mT = nullptr;
+  float *mA = nullptr;  // A = T - u, FP32
   float *mdl = nullptr, *mrref = nullptr, *mbq = nullptr, *mhx = nullptr, *mring = nullptr;
   double *ts = nullptr;
   void *res = nullptr, *hist_t = nullptr, *hist_r = nullptr;  // FP64, or FP32 in the mixed path
@@ -1231,7 +1232,7 @@ int create_impl(const rism_grid *grid, const rism_solvent *sv, const rism_solute
   std::memset(h->ctl_host, 0, sizeof(Ctl));
   if ((rc = alloc_precision<double>(h))) return rc;
   if (o->precision == RISM_PREC_MIXED) {
-    // references T, A (FP64), change, R_ref, B (FP32), increment history and
+    // reference T (FP64), A, change, R_ref, B (FP32), increment history and
     // the residual ring of H + 1 slots (FP32)
     if ((rc = alloc_precision<float>(h)) || (rc = dalloc_t(h, &h->mT, VN)) || (rc = dalloc_t(h, &h->mA, VN)) ||
         (rc = dalloc_t(h, &h->mdl, VN)) ||
@@ -1831,7 +1832,7 @@ template <typename R> int autotune_impl(rism_handle *h, int reps) {
   HIP_OK(hipMemsetAsync(h->ts, 0, (size_t)h->V * h->N * sizeof(double), h->stream));
   if (h->opts.precision == RISM_PREC_MIXED) {  // a finite state for the FP32 chain to time
     const size_t VN = (size_t)h->V * h->N;
-    HIP_OK(hipMemsetAsync(h->mA, 0, VN * sizeof(double), h->stream));
+    HIP_OK(hipMemsetAsync(h->mA, 0, VN * sizeof(float), h->stream));
     HIP_OK(hipMemsetAsync(h->mdl, 0, VN * sizeof(float), h->stream));
     HIP_OK(hipMemsetAsync(h->mrref, 0, VN * sizeof(float), h->stream));
     h->mx_live = false;

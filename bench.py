@@ -50,7 +50,7 @@ def kernel_bytes(V, P, H, mixed):
     FP64 path: the state t_s, the closure potential u, the residual R and
     the H-slot histories of t_s and R in FP64; the spectrum in complex FP64.
     Mixed path, FP32 chain (every step but the few FP64 refreshes): the
-    closure pass reads the reference A (FP64), the change dl (FP32, also
+    closure pass reads the reference A = T - u (FP32), the change dl (FP32, also
     written), the current R and H history residuals from the ring, H
     increments and R_ref (FP32), writes one increment and B (FP32) and the
     spectrum (complex FP32); the residual pass reads the spectrum, B and the
@@ -58,7 +58,7 @@ def kernel_bytes(V, P, H, mixed):
     """
     if mixed:
         return {
-            "closure_fx": V * (8 + 4 + 4 + 4 + 8 * H + 4 + 4 + 4) + 8 * P,
+            "closure_fx": V * (4 + 4 + 4 + 4 + 8 * H + 4 + 4 + 4) + 8 * P,
             "fft_y_fwd": 16 * P,
             "fft_z_oz": 16 * P,
             "fft_y_inv": 16 * P,
