@@ -30,6 +30,16 @@ namespace rism {
 // sigma_uv / 2) for VC sites; beyond V a padding record that contributes
 // nothing), so the inner loop reads broadcast LDS values instead of waiting
 // on a scalar load per atom and site.
+//
+// The SE sums over the atoms with r >= sigma_uv / 2 are the full sums the
+// potential already forms (the LJ terms, and the Coulomb sum de) minus the
+// terms of the excluded atoms.  Those are gathered in a branch taken only
+// where a point lies within xr_u of atom u, xr_u = max(max_v sigma_uv / 2,
+// 0.01 A) (record (u, site 0), fourth component): beyond xr_u every term of
+// the full sums is the SE's own (exact r, no floor, r >= 1e-5), so the
+// exclusion costs one compare per pair instead of a compare and two 64-bit
+// selects per pair and site.  Points with no atom within xr_u get the same
+// SE sums, added in the same order, as a per-pair selection would give.
 template <int VC>
 __global__ __launch_bounds__(kThreads) void k_potentials(PotArgs a) {
   __shared__ double4 s_at[kAtomChunk];
@@ -45,9 +55,9 @@ __global__ __launch_bounds__(kThreads) void k_potentials(PotArgs a) {
   const double ry = (y - ny / 2) * a.dr[1];
   const double rz = (z - nz / 2) * a.dr[2];
   const double cut2 = 1.0e-2 * 1.0e-2;  // LJ distance floor 0.01 A (cal_LJ.cu:63)
-  double lj[VC], sel[VC], seq[VC];
+  double lj[VC], xl[VC], xq[VC];         // xl, xq: full-sum terms the SE excludes
 #pragma unroll
-  for (int v = 0; v < VC; ++v) lj[v] = sel[v] = seq[v] = 0.0;
+  for (int v = 0; v < VC; ++v) lj[v] = xl[v] = xq[v] = 0.0;
   double e = 0.0, f = 0.0, s = 0.0;
   const int V = a.V, nu = a.natu;
   for (int c0 = 0; c0 < nu; c0 += kAtomChunk) {
@@ -56,38 +66,36 @@ __global__ __launch_bounds__(kThreads) void k_potentials(PotArgs a) {
     for (int i = threadIdx.x; i < nc; i += kThreads) s_at[i] = a.atoms[c0 + i];
     for (int i = threadIdx.x; i < nc * VC; i += kThreads) {
       const int u = i / VC, v = i - u * VC;
+      double xr = 0.0;
+      if (v == 0) {  // the atom's exclusion radius rides on its first site record
+        xr = 1.0e-2;
+        for (int w = 0; w < V; ++w) xr = fmax(xr, a.sigh[w * nu + c0 + u]);
+      }
       if (v < V) {
         const double2 p = a.lj[v * nu + c0 + u];
-        s_lj[i] = make_double4(p.x, p.y, a.sigh[v * nu + c0 + u], 0.0);
-      } else {  // padding site: r >= 1e300 never holds, so its SE terms stay zero
-        s_lj[i] = make_double4(0.0, 0.0, 1.0e300, 0.0);
+        s_lj[i] = make_double4(p.x, p.y, a.sigh[v * nu + c0 + u], xr);
+      } else {  // padding site: r >= 1e300 never holds
+        s_lj[i] = make_double4(0.0, 0.0, 1.0e300, xr);
       }
     }
     __syncthreads();
-    // Per atom: distance, LJ and SE terms, then the Coulomb terms behind
-    // their branch on r (an exp(-r) only below 38 A); every sum adds the
-    // atoms in order.  tl: the potential's LJ term (distance floored at
-    // 0.01 A); ts / tq: the SE's LJ and Coulomb terms (exact r, zero for
-    // r < sigma_uv / 2).
-    auto geom = [&](int k, double *tl, double *ts, double *tq, double &ra, double &rinv) {
+    // Per atom: distance, the LJ term per site (tl, distance floored at
+    // 0.01 A), then the Coulomb terms behind their branch on r (an exp(-r)
+    // only below 38 A); every sum adds the atoms in order.
+    for (int k = 0; k < nc; ++k) {
       const double4 at = s_at[k];
       const double dx = rx - at.x, dy = ry - at.y, dz = rz - at.z;
       const double r2 = dx * dx + dy * dy + dz * dz;
-      rinv = rsqrt_f64(fmax(r2, 1.0e-300));
-      ra = r2 * rinv;
+      const double rinv = rsqrt_f64(fmax(r2, 1.0e-300));
+      const double ra = r2 * rinv;
       const double ir2 = rinv * rinv;
-      const double qr = at.w * rinv;
+      double tl[VC];
 #pragma unroll
       for (int v = 0; v < VC; ++v) {
         const double4 p = s_lj[k * VC + v];
         const double s2 = p.x * ir2;
         const double s6 = s2 * s2 * s2;
-        double t = p.y * s6 * (s6 - 1.0);
-        tl[v] = t;
-        asm volatile("" : "+v"(t));  // computed for every lane, then selected: no divergent branch
-        const bool in = ra >= p.z;
-        ts[v] = in ? t : 0.0;
-        tq[v] = in ? qr : 0.0;
+        tl[v] = p.y * s6 * (s6 - 1.0);
       }
       if (r2 < cut2) {  // within 0.01 A of the atom (rare): the potential's floor
 #pragma unroll
@@ -98,11 +106,11 @@ __global__ __launch_bounds__(kThreads) void k_potentials(PotArgs a) {
           tl[v] = p.y * s6 * (s6 - 1.0);
         }
       }
-    };
-    auto coulomb = [&](int k, double ra, double rinv) {
-      const double qa = s_at[k].w;
+#pragma unroll
+      for (int v = 0; v < VC; ++v) lj[v] += tl[v];
+      const double qa = at.w;
+      const double qr = qa * rinv;
       if (ra >= 1.0e-5) {
-        const double qr = qa * rinv;
         e += qr;
         if (ra < 38.0) {
           const double ex = exp(-ra);
@@ -115,20 +123,18 @@ __global__ __launch_bounds__(kThreads) void k_potentials(PotArgs a) {
         f += qa;
         s -= qa;
       }
-    };
-    // One atom per step: with the SE terms the per-atom work has enough
-    // independent chains, and two atoms per step (112 against 92 VGPRs)
-    // measured slower.
-    for (int k = 0; k < nc; ++k) {
-      double la[VC], sa[VC], qa[VC], raa, ria;
-      geom(k, la, sa, qa, raa, ria);
+      if (ra < s_lj[k * VC].w) {  // within the atom's exclusion radius (rare)
 #pragma unroll
-      for (int v = 0; v < VC; ++v) {
-        lj[v] += la[v];
-        sel[v] += sa[v];
-        seq[v] += qa[v];
+        for (int v = 0; v < VC; ++v) {
+          const double4 p = s_lj[k * VC + v];
+          const bool in = ra >= p.z;  // the SE keeps this pair (exact r)
+          const double s2 = p.x * ir2;
+          const double s6 = s2 * s2 * s2;
+          const double te = p.y * s6 * (s6 - 1.0);
+          xl[v] += tl[v] - (in ? te : 0.0);
+          xq[v] += (ra >= 1.0e-5 ? qr : 0.0) - (in ? qr : 0.0);
+        }
       }
-      coulomb(k, raa, ria);
     }
   }
   if (!live) return;
@@ -136,8 +142,8 @@ __global__ __launch_bounds__(kThreads) void k_potentials(PotArgs a) {
   for (int v = 0; v < VC; ++v)
     if (v < V) {
       a.ulj[(long)v * a.N + ip] = lj[v] * a.ikbt;
-      a.sel[(long)v * a.N + ip] = sel[v];
-      a.seq[(long)v * a.N + ip] = seq[v];
+      a.sel[(long)v * a.N + ip] = lj[v] - xl[v];
+      a.seq[(long)v * a.N + ip] = e - xq[v];
     }
   a.de[ip] = e * a.ubeta;
   a.fr[ip] = f * a.ubeta;
