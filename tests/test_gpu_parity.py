@@ -99,6 +99,42 @@ def test_potentials_match_oracle(gpu, tmp_path):
     s.close()
 
 
+def test_se_exclusion_edge_atoms_match_oracle(gpu):
+    """The SE sums exclude the atoms with r < sigma_uv / 2 (cal_euv.cu:45-61);
+    the potentials pass forms them as full sums minus the excluded terms.
+    Atoms that reach every case of that branch: one exactly on a grid point
+    (r = 0: the LJ floor and the 1e-5 A Coulomb cut), one 0.004 A from a grid
+    point (inside the 0.01 A floor), a bare charge (sigma = eps = 0, so the
+    exclusion radius is sigma_v / 4 and its LJ terms are zero), and two
+    ordinary atoms.  The SE of the same state t agrees with the oracle."""
+    print("SE exclusion edge atoms 32^3", flush=True)
+    n, box = (32, 32, 32), (16.0, 16.0, 16.0)  # grid points at multiples of 0.5 A
+    xyz = np.array([[1.0, -1.5, 0.5], [-2.0 + 0.003, 1.0, -0.5 + 0.0025], [2.3, 2.2, -2.1],
+                    [-1.2, -0.7, 1.9], [0.4, 2.6, -1.1]])
+    sig = np.array([3.2, 2.9, 0.0, 3.5, 2.6])
+    eps = np.array([500.0, 300.0, 0.0, 650.0, 250.0])
+    q = np.array([0.4, -0.2, -0.3, 0.25, -0.15])
+    su = rism3d.Solute(sig, eps, q, xyz)
+    s = rism3d.Solver(n, box, _solvent(), su, rism3d.Options(maxstep=1))
+    s.build_potentials()
+    s.solve()  # one step: the handle holds a state, replaced below
+    o = orc.Oracle.from_arrays(n, box, example("h2o_vv.xvk"), sig, eps, q, xyz).setup()
+    fr = o.field(orc.F_FR)
+    qv = _solvent().q
+    t0 = np.concatenate([qv[v] * fr for v in range(len(qv))])
+    t0 = t0 + 0.05 * np.random.default_rng(5).standard_normal(t0.shape)
+    s.set_t(t0)
+    x = s.thermo()
+    o.set_t(t0)
+    o.calculate(1.0)
+    ox = o.thermo()
+    s.close()
+    print({k: (x[k], ox[k]) for k in ("SE", "SE_LJ", "SE_ES")}, flush=True)
+    for k in ("SE", "SE_LJ", "SE_ES"):
+        assert np.isfinite(x[k])
+        assert x[k] == pytest.approx(ox[k], rel=1e-9, abs=1e-9), k
+
+
 def test_calculate_matches_oracle(gpu, deckdir):
     """One fixed-point map t -> R (calculate.cu) from the initial estimate."""
     print("calculate test1 deck", flush=True)
