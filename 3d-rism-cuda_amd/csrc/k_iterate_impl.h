@@ -669,6 +669,10 @@ static __device__ void anderson_solve_block(Ctl *ctl, const double *partials, in
   const int hm = HM;
   __shared__ double tot[NV];
   __shared__ double wred[kThreads / 64][NV];
+  // thread 0's small system lives in LDS: indexed by the filled slot count,
+  // it would otherwise sit in scratch (private memory), one global-memory
+  // round trip per element access
+  __shared__ double sA[HM * HM], sc[HM], sth[HM];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   // thread t sums rows t, t + 256, ... (fixed order); four rows in flight
   double s[NV];
@@ -756,7 +760,7 @@ static __device__ void anderson_solve_block(Ctl *ctl, const double *partials, in
     ctl->upd_mix = 1.0;
   } else {
     const int nf = ctl->filled < H ? ctl->filled : H;
-    double A[kHMax * kHMax], c[kHMax], th[kHMax];
+    double *A = sA, *c = sc, *th = sth;
     for (int s = 0; s < nf; ++s) c[s] = tot[1 + s];
     int k = 1 + hm;
     for (int s = 0; s < hm; ++s)
