@@ -54,6 +54,14 @@ __device__ __forceinline__ double wave_sum(double v) {
 #ifndef RISM_CLOS_CH
 #define RISM_CLOS_CH 2
 #endif
+// The mixed path's vectorised closure pass takes its line groups from the
+// last to the first.  The residual pass before it writes the current
+// residual in group order, so the last groups' residuals are still in the
+// 256 MB Infinity Cache when the closure pass starts on them, and the
+// spectrum it writes last (the first groups) is what the forward y pass
+// reads first.  Each group's work is the same in either order.  (The FP64
+// closure pass measured slower in that order, and keeps the forward one.)
+__device__ __forceinline__ long closure_group() { return (long)(gridDim.x - 1 - blockIdx.x); }
 template <typename R, typename S, int EPT, int VC, int HM, bool UPD>
 __device__ __forceinline__ void closure_points(const IterArgs<R, S> &a, typename C2<R>::T *lds, bool do_run,
                                                const double *th, const long *off, double mix, int w, int nf) {

@@ -218,7 +218,7 @@ __device__ __forceinline__ void mx_points_vec(const IterArgs<float, double> &a, 
   const int nx = NX > 0 ? NX : a.nx, L = NX > 0 ? EPT * kThreads / (VC / 2 * NX) : a.Lc, ls = line_stride(nx);
   const int V = a.V;
   const long N = a.N, VN = (long)V * N;
-  const long base = (long)blockIdx.x * L * nx;
+  const long base = closure_group() * L * nx;
   const int nel = L * nx;
   const float2 *__restrict__ pre = (const float2 *)a.px.pre;
   const float *__restrict__ Ab = soff(a.A, so);
@@ -324,7 +324,7 @@ __global__ __launch_bounds__(kThreads) void k_mx_closure_vec(IterArgs<float, dou
   const int nx = NX > 0 ? NX : a.nx, P = NX > 0 ? VC / 2 : a.P;
   const int L = NX > 0 ? EPT * kThreads / (VC / 2 * NX) : a.Lc, ls = line_stride(nx);
   const long N = a.N, VN = (long)a.V * N;
-  const long base = (long)blockIdx.x * L * nx;
+  const long base = closure_group() * L * nx;
   const int nel = L * nx;
   int w = 0, nf = 0;
   double mix = 0.0;
