@@ -721,6 +721,7 @@ static __device__ void anderson_solve_block(Ctl *ctl, const double *partials, in
   ctl->step += 1;
   ctl->total += 1;
   rms_hist[ctl->total - 1] = rms;
+  ctl->rms_recent[(ctl->total - 1) % kRmsRecent] = rms;
   ctl->rms = rms;
   if (!isfinite(rms)) {  // iterate.cu:54-57
     ctl->status = 1;

@@ -26,6 +26,7 @@ namespace rism {
 constexpr int kMaxV = 8;    // reduced solvent sites
 constexpr int kAtomChunk = 128;  // solute atoms staged in LDS per pass of the grid x atom kernels
 constexpr int kHMax = 16;   // Anderson history slots
+constexpr int kRmsRecent = 16;  // rms values the control block carries (Ctl::rms_recent)
 
 // 1/sqrt(x) in FP64 for x > 0: the hardware estimate refined by two Newton
 // steps y <- y + y (1 - x y^2) / 2 (each roughly doubles the correct bits),
@@ -80,6 +81,10 @@ struct Ctl {
   int refresh_iter;
   int since_ref, best_at;
   double ref_rms, best_rms;
+  // rms of the last kRmsRecent steps, step k (over all stages, from 0) in
+  // slot k % kRmsRecent: the host reads them with the control block after
+  // each chunk of at most kRmsRecent steps, with no copy of its own
+  double rms_recent[kRmsRecent];
 };
 
 // Which chain of kernels runs at the current step.  gate 0: the FP64 path

@@ -915,9 +915,15 @@ template <typename R> int solve_impl(rism_handle *h, const char *tuv_path, int *
       if (int rc = get_ctl(h)) return rc;
       // progress lines (iterate.cu:63)
       if (c.total > printed) {
+        // the control block carries the last kRmsRecent steps' rms (a
+        // launch group has at most that many steps); rms_hist the rest
         std::vector<double> tail(c.total - printed);
-        HIP_OK(hcopy(h, tail.data(), h->rms_hist + printed, tail.size() * sizeof(double),
-                         hipMemcpyDeviceToHost));
+        if (c.total - printed <= kRmsRecent) {
+          for (int k = printed; k < c.total; ++k) tail[k - printed] = c.rms_recent[k % kRmsRecent];
+        } else {
+          HIP_OK(hcopy(h, tail.data(), h->rms_hist + printed, tail.size() * sizeof(double),
+                           hipMemcpyDeviceToHost));
+        }
         for (size_t k = 0; k < tail.size(); ++k) {
           h->rms_log.push_back(tail[k]);
           if (verbose)
