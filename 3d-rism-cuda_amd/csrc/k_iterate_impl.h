@@ -797,9 +797,11 @@ static __device__ void anderson_solve_block(Ctl *ctl, const double *partials, in
   ctl->write = (w + 1) % H;
   if (ctl->filled < H) ctl->filled += 1;
   ctl->upd_iter = iter + 1;
-  if (ctl->step >= ctl->maxstep) {  // iterate.cu:76-79 (the update still applies)
+  if (ctl->step >= ctl->maxstep) {  // iterate.cu:76-79 (the update still applies, once, from the host)
     ctl->status = 2;
     ctl->run_iter = -1;
+    ctl->upd_hold = iter + 1;
+    ctl->upd_iter = -1;
   } else {
     ctl->run_iter = iter + 1;
   }

@@ -57,6 +57,10 @@ struct Ctl {
   int nhist;
   int upd_nf;     // slots 0..upd_nf-1 enter the pending update
   int upd_w;      // slot the pending update writes
+  int upd_hold;   // step of the update a step-limit stop leaves pending: the
+                  // host applies it once (update-only launch); no first pass
+                  // applies it on its own, so launches queued beyond the
+                  // stop leave the state alone
   double upd_theta[kHMax];
   double upd_mix;
   double tol, m, mp;

@@ -843,6 +843,7 @@ template <typename R> int solve_impl(rism_handle *h, const char *tuv_path, int *
   c.mp = h->opts.mp;
   c.npts = (double)VN;
   c.upd_iter = -1;
+  c.upd_hold = -1;
   c.refresh_iter = -1;
   if (kMixed) {  // residual ring: slot H holds the current R, slot s history s
     c.ring = 1;
@@ -936,12 +937,14 @@ template <typename R> int solve_impl(rism_handle *h, const char *tuv_path, int *
         // iterate.cu:64-66 writes t after the Anderson update of this step:
         // apply the pending update now (one first pass), then clear it so
         // the next iteration does not apply it again
-        if (c.upd_iter == c.cur) {
+        if (c.upd_iter == c.cur || (c.status == RISM_SOLVE_MAXSTEP && c.upd_hold == c.cur)) {
           const int keep_run = c.run_iter;
           c.run_iter = -1;  // update only
+          c.upd_iter = c.cur;
           if (int rc = put_ctl(h)) return rc;
           if (int rc = launch_update_only(h)) return rc;
           c.upd_iter = -1;
+          c.upd_hold = -1;
           c.run_iter = keep_run;
           if (int rc = put_ctl(h)) return rc;
         }
@@ -952,10 +955,12 @@ template <typename R> int solve_impl(rism_handle *h, const char *tuv_path, int *
     }
     // a step-limit stop still applies its pending update (iterate.cu:61 runs
     // before the loop ends)
-    if (c.upd_iter >= 0 && c.upd_iter == c.cur) {
+    if (c.status == RISM_SOLVE_MAXSTEP && c.upd_hold == c.cur) {
       c.run_iter = -1;  // update only, no closure/FFT
+      c.upd_iter = c.cur;
       if (int rc = put_ctl(h)) return rc;
       if (int rc = launch_update_only(h)) return rc;
+      c.upd_hold = -1;
     }
     status = c.status;
     if (status == RISM_SOLVE_DIVERGED) {
@@ -1620,8 +1625,9 @@ int rism::batch_solve_class(const rism_grid *grid, const rism_solvent *sv, const
     if (!h || rc != RISM_OK) { status[s] = rc ? rc : RISM_ERR_STATE; continue; }
     Ctl &c = *h->ctl_host;
     int r = RISM_OK;
-    if (c.status == RISM_SOLVE_MAXSTEP && c.upd_iter >= 0 && c.upd_iter == c.cur) {
+    if (c.status == RISM_SOLVE_MAXSTEP && c.upd_hold == c.cur) {
       c.run_iter = -1;
+      c.upd_iter = c.cur;
       r = put_ctl(h);
       if (!r) r = launch_update_only(h);
     }

@@ -198,3 +198,26 @@ def test_ksave_writes_tuv_every_k_steps(gpu, deckdir):
     t_loaded = s.field(rism3d.FIELD_T).reshape(-1)
     s.close()
     assert np.abs(t_loaded - o.field(orc.F_T)).max() <= 1e-5 * np.abs(t_loaded).max()
+
+
+@pytest.mark.parametrize("maxstep", [10, 23])
+def test_step_limit_stop_applies_the_pending_update_once(gpu, deckdir, maxstep):
+    """A step-limit stop inside a launch chunk (test1 runs chunks of 16
+    steps): the stop leaves the Anderson update of its last step pending
+    (iterate.cu:61, 76-79), to be applied exactly once, whatever else of the
+    chunk was queued behind it.  t after the stop equals the oracle's t after
+    a run of the same number of steps."""
+    print(f"step limit {maxstep}", flush=True)
+    s = rism3d.Solver.from_deck(rism3d.read_deck("test1.inp"), maxstep=maxstep, tol=1e-12).build_potentials()
+    st, steps, _ = s.solve()
+    t = s.field(rism3d.FIELD_T).reshape(-1)
+    s.close()
+    assert st == rism3d.SOLVE_MAXSTEP and steps == maxstep
+    o = orc.Oracle.from_deck("test1.inp").read_xvk("h2o_vv.xvk").setup()
+    o.set_iter(1e-12, maxstep, 2, 0.6, 0.2)
+    _, osteps, _ = o.iterate()
+    assert osteps == maxstep
+    ot = o.field(orc.F_T)
+    err = np.abs(t - ot).max() / max(1.0, np.abs(ot).max())
+    print(f"max |t - t_oracle| / max|t| = {err:.3e}", flush=True)
+    assert err < 1e-9
