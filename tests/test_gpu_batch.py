@@ -99,3 +99,28 @@ def test_batch_reports_errors_per_solute(gpu, batch):
     res, _ = rism3d.batch_solve(bad, sv, rism3d.Options(), workers=2)
     assert res[0]["status"] < 0
     assert res[1]["status"] == rism3d.SOLVE_CONVERGED
+
+
+def test_batch_step_limit_stop_matches_single_solves(gpu, batch, monkeypatch):
+    """maxstep 7 inside the batched solves' 16-step chunks: every solute
+    stops with the update of step 7 applied once (iterate.cu:61, 76-79), in
+    both batch paths, as its standalone solve with the same limit does."""
+    sv, ws, systems = batch
+    sub = systems[:6]
+    opts = rism3d.Options(maxstep=7, tol=1e-12)
+    single = []
+    for n, box, su in sub:
+        s = rism3d.Solver(n, box, sv, su, opts)
+        s.build_potentials()
+        st, steps, _ = s.solve()
+        single.append((st, steps, s.thermo()))
+        s.close()
+    for mode in ("1", "0"):
+        monkeypatch.setenv("RISM_BATCH_LAUNCH", mode)
+        print(f"batch step limit, batched launches={mode}", flush=True)
+        res, _ = rism3d.batch_solve(sub, sv, opts, workers=2)
+        for i, (st, steps, x) in enumerate(single):
+            assert st == rism3d.SOLVE_MAXSTEP and steps == 7
+            assert res[i]["status"] == rism3d.SOLVE_MAXSTEP and res[i]["steps"] == 7, (mode, i)
+            for k in ("SFE_SC", "SFE_GF", "PMV"):
+                assert x[k] == pytest.approx(res[i][k], rel=1e-12, abs=1e-9), (mode, i, k)
