@@ -2354,6 +2354,14 @@ int rism_adjoint_grad(rism_handle *h, double rtol, double *dxyz, double *dq, dou
       if (c.status != -1 || it >= h->opts.maxstep + 1) break;
     }
   }
+  if (c.status == RISM_SOLVE_MAXSTEP && c.upd_hold == c.cur) {
+    // a step-limit stop: its pending update, held by the solve kernel, once
+    c.upd_iter = c.cur;
+    c.run_iter = -1;
+    HIP_OK(hipMemcpyAsync(h->adj_ctl, &c, sizeof c, hipMemcpyHostToDevice, st));
+    HIP_OK(launch_adj_update(a, h->hm, st));
+    HIP_OK(hipStreamSynchronize(st));
+  }
   if (steps) *steps = c.total;
   if (c.status == RISM_SOLVE_DIVERGED || c.status == -1) {
     set_error("adjoint solve did not converge");
