@@ -233,3 +233,23 @@ def test_milc_workflow_on_the_reference_deck(gpu, deckdir):
     want = milc.milc(milc.read_xmu("mobley_3053621.xmu"), milc.read_xmu("mobley_3053621.xmu0"))
     assert f"SFE(MILC)= {want['Sub']!r} (J/mol)" in r.stdout
     assert f"SFE(MILC)= {want['HNC']!r} (J/mol)" in r.stdout
+
+
+def test_rms_history_of_chunked_solve_matches_oracle_progress_lines(gpu, deckdir):
+    """The residual history of a solve replayed in chunks of 16 steps as HIP
+    graphs (the host reads each chunk's rms from the control block's window
+    of the last 16 steps) equals the oracle's progress lines step for step
+    (iterate.cu:63, printed with 6 significant digits)."""
+    print("rms history test1", flush=True)
+    r = subprocess.run([orc.EXE, "test1.inp"], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:]
+    orms = [float(ln.split()[-1]) for ln in r.stdout.splitlines() if ln.startswith(" Step = ")]
+    s = rism3d.Solver.from_deck(rism3d.read_deck("test1.inp")).build_potentials()
+    st, steps, rms = s.solve()
+    hist = s.rms_history()
+    s.close()
+    assert st == rism3d.SOLVE_CONVERGED and steps > 16
+    assert len(hist) == steps and hist[-1] == rms
+    assert abs(len(orms) - steps) <= 1
+    n = min(len(orms), steps)
+    np.testing.assert_allclose(hist[:n], orms[:n], rtol=1e-5)
