@@ -671,7 +671,7 @@ static __device__ void leg_solve_dev(double *A, double *x, double *b, int n) {
 // One workgroup: reduce the per-workgroup partials (fixed order), rms,
 // convergence test, Picard / Anderson coefficients (leg), control block.
 template <int HM>
-static __device__ void anderson_solve_block(Ctl *ctl, const double *partials, int nblk, double *rms_hist,
+static __device__ void anderson_solve_block(Ctl *gctl, const double *partials, int nblk, double *rms_hist,
                                             int *rlist, int sid) {
   constexpr int NV = 1 + HM + HM * (HM + 1) / 2;
   const int hm = HM;
@@ -681,6 +681,15 @@ static __device__ void anderson_solve_block(Ctl *ctl, const double *partials, in
   // it would otherwise sit in scratch (private memory), one global-memory
   // round trip per element access
   __shared__ double sA[HM * HM], sc[HM], sth[HM];
+  // The control block is staged in LDS by all threads at once, thread 0
+  // works on the staged copy (its many dependent reads then wait on LDS, not
+  // on global memory) and all threads write it back whole.
+  __shared__ Ctl sctl;
+  static_assert(sizeof(Ctl) % 4 == 0, "control block copied in 4-byte words");
+  constexpr int kCtlWords = (int)(sizeof(Ctl) / 4);
+  for (int i = threadIdx.x; i < kCtlWords; i += kThreads)
+    reinterpret_cast<unsigned *>(&sctl)[i] = reinterpret_cast<const unsigned *>(gctl)[i];
+  Ctl *ctl = &sctl;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   // thread t sums rows t, t + 256, ... (fixed order); four rows in flight
   double s[NV];
@@ -713,98 +722,102 @@ static __device__ void anderson_solve_block(Ctl *ctl, const double *partials, in
     tot[k] = t;
   }
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  const int iter = ctl->cur;
-  ctl->cur = iter + 1;
-  if (ctl->ring) ctl->rlast = ctl->rcur;
-  const double rms = sqrt(tot[0] / ctl->npts);  // cal_rms.cu:18-20
-  ctl->step += 1;
-  ctl->total += 1;
-  rms_hist[ctl->total - 1] = rms;
-  ctl->rms_recent[(ctl->total - 1) % kRmsRecent] = rms;
-  ctl->rms = rms;
-  if (!isfinite(rms)) {  // iterate.cu:54-57
-    ctl->status = 1;
-    ctl->run_iter = -1;
-    ctl->upd_iter = -1;
-    return;
-  }
-  if (rms <= ctl->tol) {  // iterate.cu:58-59
-    ctl->status = 0;
-    ctl->run_iter = -1;
-    ctl->upd_iter = -1;
-    return;
-  }
-  const int H = ctl->nhist;
-  const int w = ctl->write;
-  if (ctl->ring) {  // mixed precision: refinement schedule (see Ctl)
-    if (ctl->refresh_iter == iter) {
-      ctl->ref_rms = rms;
-      ctl->best_rms = rms;
-      ctl->best_at = ctl->total;
-      ctl->since_ref = 0;
-    } else {
-      ctl->since_ref += 1;
-      if (rms < ctl->best_rms) {
+  if (threadIdx.x == 0) do {  // one pass; break leaves it
+    const int iter = ctl->cur;
+    ctl->cur = iter + 1;
+    if (ctl->ring) ctl->rlast = ctl->rcur;
+    const double rms = sqrt(tot[0] / ctl->npts);  // cal_rms.cu:18-20
+    ctl->step += 1;
+    ctl->total += 1;
+    rms_hist[ctl->total - 1] = rms;
+    ctl->rms_recent[(ctl->total - 1) % kRmsRecent] = rms;
+    ctl->rms = rms;
+    if (!isfinite(rms)) {  // iterate.cu:54-57
+      ctl->status = 1;
+      ctl->run_iter = -1;
+      ctl->upd_iter = -1;
+      break;
+    }
+    if (rms <= ctl->tol) {  // iterate.cu:58-59
+      ctl->status = 0;
+      ctl->run_iter = -1;
+      ctl->upd_iter = -1;
+      break;
+    }
+    const int H = ctl->nhist;
+    const int w = ctl->write;
+    if (ctl->ring) {  // mixed precision: refinement schedule (see Ctl)
+      if (ctl->refresh_iter == iter) {
+        ctl->ref_rms = rms;
         ctl->best_rms = rms;
         ctl->best_at = ctl->total;
-      }
-      // (a refresh step itself never asks for the next one: its rms is the
-      // new reference and since_ref is 0)
-      if (ctl->step < ctl->maxstep &&
-          (rms < 0.1 * ctl->ref_rms || (ctl->since_ref >= 24 && ctl->total - ctl->best_at >= 24))) {
-        bool filed = true;
-        if (rlist) {  // batched launches: a place on the refresh list, else again next step
-          const int slot = atomicAdd(rlist, 1);
-          filed = slot < rlist[1];
-          if (filed) rlist[2 + slot] = sid;
+        ctl->since_ref = 0;
+      } else {
+        ctl->since_ref += 1;
+        if (rms < ctl->best_rms) {
+          ctl->best_rms = rms;
+          ctl->best_at = ctl->total;
         }
-        if (filed) ctl->refresh_iter = iter + 1;
+        // (a refresh step itself never asks for the next one: its rms is the
+        // new reference and since_ref is 0)
+        if (ctl->step < ctl->maxstep &&
+            (rms < 0.1 * ctl->ref_rms || (ctl->since_ref >= 24 && ctl->total - ctl->best_at >= 24))) {
+          bool filed = true;
+          if (rlist) {  // batched launches: a place on the refresh list, else again next step
+            const int slot = atomicAdd(rlist, 1);
+            filed = slot < rlist[1];
+            if (filed) rlist[2 + slot] = sid;
+          }
+          if (filed) ctl->refresh_iter = iter + 1;
+        }
       }
     }
-  }
-  if (ctl->count > 0) {  // Picard warm-up, anderson.cu:43-49
-    ctl->count -= 1;
-    ctl->upd_nf = 0;
-    ctl->upd_mix = 1.0;
-  } else {
-    const int nf = ctl->filled < H ? ctl->filled : H;
-    double *A = sA, *c = sc, *th = sth;
-    for (int s = 0; s < nf; ++s) c[s] = tot[1 + s];
-    int k = 1 + hm;
-    for (int s = 0; s < hm; ++s)
-      for (int s2 = s; s2 < hm; ++s2) {
-        if (s < nf && s2 < nf) {
-          A[s * nf + s2] = tot[k];
-          A[s2 * nf + s] = tot[k];
+    if (ctl->count > 0) {  // Picard warm-up, anderson.cu:43-49
+      ctl->count -= 1;
+      ctl->upd_nf = 0;
+      ctl->upd_mix = 1.0;
+    } else {
+      const int nf = ctl->filled < H ? ctl->filled : H;
+      double *A = sA, *c = sc, *th = sth;
+      for (int s = 0; s < nf; ++s) c[s] = tot[1 + s];
+      int k = 1 + hm;
+      for (int s = 0; s < hm; ++s)
+        for (int s2 = s; s2 < hm; ++s2) {
+          if (s < nf && s2 < nf) {
+            A[s * nf + s2] = tot[k];
+            A[s2 * nf + s] = tot[k];
+          }
+          ++k;
         }
-        ++k;
-      }
-    if (nf > 0) leg_solve_dev(A, th, c, nf);
-    const int recent = (w - 1 + H) % H;  // mp on the newest slot (anderson.cu:54-55)
-    for (int s = 0; s < nf; ++s) ctl->upd_theta[s] = th[s] + (s == recent ? ctl->mp : 0.0);
-    ctl->upd_nf = nf;
-    ctl->upd_mix = ctl->m;
-  }
-  ctl->upd_w = w;
-  if (ctl->ring) {  // file R into slot w by relabelling; the update reads the old map
-    ctl->upd_rcur = ctl->rcur;
-    for (int s = 0; s < H; ++s) ctl->upd_rmap[s] = ctl->rmap[s];
-    const int freed = ctl->rmap[w];
-    ctl->rmap[w] = ctl->rcur;
-    ctl->rcur = freed;
-  }
-  ctl->write = (w + 1) % H;
-  if (ctl->filled < H) ctl->filled += 1;
-  ctl->upd_iter = iter + 1;
-  if (ctl->step >= ctl->maxstep) {  // iterate.cu:76-79 (the update still applies, once, from the host)
-    ctl->status = 2;
-    ctl->run_iter = -1;
-    ctl->upd_hold = iter + 1;
-    ctl->upd_iter = -1;
-  } else {
-    ctl->run_iter = iter + 1;
-  }
+      if (nf > 0) leg_solve_dev(A, th, c, nf);
+      const int recent = (w - 1 + H) % H;  // mp on the newest slot (anderson.cu:54-55)
+      for (int s = 0; s < nf; ++s) ctl->upd_theta[s] = th[s] + (s == recent ? ctl->mp : 0.0);
+      ctl->upd_nf = nf;
+      ctl->upd_mix = ctl->m;
+    }
+    ctl->upd_w = w;
+    if (ctl->ring) {  // file R into slot w by relabelling; the update reads the old map
+      ctl->upd_rcur = ctl->rcur;
+      for (int s = 0; s < H; ++s) ctl->upd_rmap[s] = ctl->rmap[s];
+      const int freed = ctl->rmap[w];
+      ctl->rmap[w] = ctl->rcur;
+      ctl->rcur = freed;
+    }
+    ctl->write = (w + 1) % H;
+    if (ctl->filled < H) ctl->filled += 1;
+    ctl->upd_iter = iter + 1;
+    if (ctl->step >= ctl->maxstep) {  // iterate.cu:76-79 (the update still applies, once, from the host)
+      ctl->status = 2;
+      ctl->run_iter = -1;
+      ctl->upd_hold = iter + 1;
+      ctl->upd_iter = -1;
+    } else {
+      ctl->run_iter = iter + 1;
+    }
+  } while (0);
+  __syncthreads();
+  for (int i = threadIdx.x; i < kCtlWords; i += kThreads)
+    reinterpret_cast<unsigned *>(gctl)[i] = reinterpret_cast<const unsigned *>(&sctl)[i];
 }
 
 template <int HM>
